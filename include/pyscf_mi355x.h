@@ -1,0 +1,154 @@
+/*
+ * C-ABI of libpyscf_mi355x.so: MI355X (gfx950) kernels for the restricted
+ * Kohn-Sham B3LYP Fock build and SCF loop of pyscf-ipu's nanoDFT.
+ *
+ * Conventions
+ *   - Every pointer argument named d_* is device memory owned by the caller;
+ *     the library never frees caller memory.  h_* pointers are host memory.
+ *   - `stream` is a hipStream_t (NULL = default stream); all calls are
+ *     asynchronous on it unless stated otherwise.
+ *   - Return value: PSCF_OK (0) or a negative PSCF_E* code; the message of the
+ *     last error on the calling thread is returned by pscf_last_error().
+ *   - dtype selects the arithmetic of the floating-point operands:
+ *     PSCF_F64 (double) or PSCF_F32 (float).
+ *   - Molecules use libcint's atm/bas/env layout (atm: natm x 6 int32,
+ *     bas: nbas x 8 int32, env: double), s and p shells only (l <= 1).
+ *   - Workspaces (d_ws, ws_bytes) are caller-provided scratch; the matching
+ *     *_workspace_bytes() query returns the size needed.
+ */
+#ifndef PYSCF_MI355X_H
+#define PYSCF_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSCF_OK 0
+#define PSCF_EINVAL -1
+#define PSCF_EHIP -2
+#define PSCF_EUNSUPPORTED -3
+
+#define PSCF_F64 0
+#define PSCF_F32 1
+
+const char* pscf_last_error(void);
+int pscf_abi_version(void);
+
+/* ---------------------------------------------------------------- ERIs ----
+ * pscf_eri_s8: all unique spherical ERIs (ij|kl), p>=q, r>=s, pq>=rs, packed as
+ * d_out[pq*(pq+1)/2 + rs] (PySCF aosym="s8" layout, PySCF 2.2.1
+ * `mol.intor("int2e_sph", aosym="s8")` as called at
+ * pyscf_ipu/nanoDFT/nanoDFT.py:285).  The caller passes the shell pairs
+ * (i, j) sorted by pair class (ss, then ps with the p shell first, then pp) in
+ * d_pairs (npair x 2 int32) and the three class counts; every s8 element of the
+ * basis is written exactly once.  d_out has nao(nao+1)/2 * (nao(nao+1)/2 + 1)/2
+ * elements of dtype.
+ */
+int pscf_eri_s8(int dtype, const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env,
+                const int* d_pairs, const int* h_class_counts, int nao, void* d_out, void* stream);
+
+/* pscf_eri_quartets: contracted shell quartets of an explicit list, the work of
+ * pyscf_ipu/electron_repulsion/direct.py:513 `electron_repulsion_integrals`
+ * (int2e_sph.cpp:6750 per quartet).  Quartet q = d_quartets[4q..4q+3] writes
+ * di*dj*dk*dl values at d_out + d_offsets[q], element (a,b,c,d) at
+ * a + di*(b + dj*(c + dk*d)) (libcint's column-major block order). */
+int pscf_eri_quartets(int dtype, const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env,
+                      const int* d_quartets, const int64_t* d_offsets, int nq, void* d_out, void* stream);
+
+/* pscf_int1e: overlap S, kinetic T and nuclear attraction V (all nuclei) in
+ * FP64, nao x nao row-major each (the role of mol.intor_symmetric('int1e_ovlp'
+ * /'int1e_kin'/'int1e_nuc'), pyscf_ipu/nanoDFT/nanoDFT.py:236-238). */
+int pscf_int1e(const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env, int nao,
+               double* d_S, double* d_T, double* d_V, void* stream);
+
+/* ----------------------------------------------------------------- J/K ----
+ * pscf_jk_sparse: drop-in for sparse_symmetric_einsum
+ * (pyscf_ipu/nanoDFT/sparse_symmetric_ERI.py:86-137 with the index table of
+ * compute_indices.cpp:54-95): d_out (N x N) = sum over the 8 J patterns minus
+ * hyb/2 times the 8 K patterns of gather(dm) * eri -> scatter-add, for nnz
+ * ERIs already divided by their permutation multiplicity, with int16 AO
+ * indices (i, j, k, l) per ERI (d_idx: nnz x 4).  dm need not be symmetric.
+ */
+size_t pscf_jk_sparse_workspace_bytes(int dtype, int N, int64_t nnz);
+int pscf_jk_sparse(int dtype, const void* d_eri, const int16_t* d_idx, int64_t nnz, const void* d_dm, int N,
+                   double hyb, void* d_out, void* d_ws, size_t ws_bytes, void* stream);
+
+/* pscf_jk_s8: the same dJK = J - hyb/2 K from the packed s8 array of
+ * pscf_eri_s8 (raw ERIs, not divided by multiplicity), indices regenerated
+ * from each element's position; dm must be symmetric.  Equal to
+ * pscf_jk_sparse on the nonzero subset. */
+size_t pscf_jk_s8_workspace_bytes(int dtype, int N);
+int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_dm, double hyb, void* d_out, void* d_ws,
+               size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------- XC ----
+ * pscf_ao_eval: AO values and gradients on grid points, the role of
+ * mol.eval_gto('GTOval_sph_deriv1', coords, 4) (nanoDFT.py:219-220).
+ * d_coords: g x 3 double (Bohr).  d_ao: 4 x nao x g of dtype (component,
+ * AO, point; point index fastest). */
+int pscf_ao_eval(int dtype, const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env, int nao,
+                 const double* d_coords, int g, void* d_ao, void* stream);
+
+/* pscf_becke_weights: per-point Becke fuzzy-cell weights
+ * w = vol * P_owner / sum_i P_i with Becke's three-times-smoothed step and the
+ * Treutler size adjustment table d_adjust (natm x natm, a_ij = -a_ji), the
+ * partition step of PySCF's Grids.build (pyscf_ipu/pyscf_utils/build_grid.py:
+ * 34-100, gen_grid_partition).  d_coords: g x 3, d_owner: g atom indices,
+ * d_vol: g un-partitioned quadrature weights, d_atom_xyz: natm x 3 (Bohr). */
+int pscf_becke_weights(const double* d_coords, const int* d_owner, const double* d_vol, int g,
+                       const double* d_atom_xyz, const double* d_adjust, int natm, double* d_w, void* stream);
+
+/* pscf_b3lyp: drop-in for b3lyp(rho, EPSILON_B3LYP)
+ * (pyscf_ipu/exchange_correlation/b3lyp.py:13-39): d_rho is 4 x g
+ * (rho, grad rho / 2), outputs per point e_xc (energy per particle),
+ * v_rho = dE/drho and v_norms = dE/d|grad rho|^2. */
+int pscf_b3lyp(int dtype, const void* d_rho, int g, double eps, void* d_exc, void* d_vrho, void* d_vnorm,
+               void* stream);
+
+/* pscf_b3lyp_host: the same pointwise B3LYP code evaluated on the host in
+ * FP64 (h_* host arrays, same layout); no GPU needed. */
+int pscf_b3lyp_host(const double* h_rho, int g, double eps, double* h_exc, double* h_vrho, double* h_vnorm);
+
+/* pscf_xc: drop-in for exchange_correlation(dm, grid_AO, grid_weights)
+ * (pyscf_ipu/nanoDFT/nanoDFT.py:105-123).  d_ao holds AO values/gradients
+ * with element (c, mu, p) at d_ao[c*ao_sc + mu*ao_smu + p*ao_sp] (so both the
+ * reference's (4, g, N) layout and pscf_ao_eval's (4, N, g) layout work).
+ * Outputs: *d_exc (one double, E_xc) and d_vxc (N x N, dtype, V + V^T). */
+size_t pscf_xc_workspace_bytes(int dtype, int N, int g);
+int pscf_xc(int dtype, const void* d_ao, int64_t ao_sc, int64_t ao_smu, int64_t ao_sp, int g, int N,
+            const void* d_dm, const void* d_weights, double eps, double* d_exc, void* d_vxc, void* d_ws,
+            size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------ SCF ----
+ * pscf_scf_step: one nanoDFT_iteration up to the new density
+ * (pyscf_ipu/nanoDFT/nanoDFT.py:64-72 with DIIS :338-364 and linalg_eigh
+ * :370-393), FP64, one workgroup:
+ *   H = H_core + dJK + V_xc; DIIS(9) extrapolation (errvec = S D H - (S D H)^T,
+ *   c = pinv(B)[0]); eigh(L^-1 H L^-T) by warm-started parallel Jacobi;
+ *   C = L^-T U; D = 2 C_occ C_occ^T.
+ * d_state: pscf_scf_state_bytes(N) bytes, zero-filled before iteration 0 by
+ * the caller, carried between calls (DIIS history, previous eigenvectors).
+ * dJK / V_xc are of dtype; everything else FP64.  Outputs d_D (N x N),
+ * d_mo_energy (N), d_C (N x N); the H actually diagonalised (after DIIS) is
+ * written to d_H (N x N). */
+size_t pscf_scf_state_bytes(int N);
+int pscf_scf_step(int dtype, int iteration, int N, int nocc, const double* d_Hcore, const double* d_S,
+                  const double* d_Linv, const void* d_dJK, const void* d_Vxc, double* d_D, double* d_H,
+                  double* d_mo_energy, double* d_C, void* d_state, int use_diis, void* stream);
+
+/* pscf_energy: E = sum D*H_core + 1/2 sum D*dJK + E_xc + E_nuc (nanoDFT.py:20-38)
+ * in FP64 with compensated summation; d_out receives
+ * [E, E_core, E_JK, E_xc, E_nuc] (5 doubles). */
+int pscf_energy(int dtype, int N, const double* d_D, const double* d_Hcore, const void* d_dJK,
+                const double* d_exc, double e_nuc, double* d_out, void* stream);
+
+/* pscf_cast: elementwise FP64 <-> FP32 conversion helper (n elements). */
+int pscf_cast(int src_dtype, const void* d_src, int dst_dtype, void* d_dst, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,431 @@
+// Grid exchange-correlation path on gfx950: AO values/gradients, Becke cell
+// weights, B3LYP pointwise, and the fused rho -> B3LYP -> V_xc kernel.
+//
+// Reference: exchange_correlation (pyscf_ipu/nanoDFT/nanoDFT.py:105-123):
+//   X = phi0 D; rho_c = sum_mu X phi_c (rho, grad rho / 2); b3lyp;
+//   E_xc = sum rho w e_xc; V = phi0^T sum_c phi_c v_c w with
+//   v = [v_rho/2, 4 v_sigma grad rho / 2]; V_xc = V + V^T.
+// Fused form: each workgroup walks 64-point tiles of the grid, keeping D and
+// the current tile of phi0 in LDS, forming X, rho, the pointwise potential
+// and Y = sum_c phi_c v_c w in LDS, and accumulating its share of
+// V = phi0^T Y in registers (4x4 register blocks); one partial V per
+// workgroup is reduced in a fixed order by a second kernel.
+#include "b3lyp.h"
+#include "common.h"
+
+namespace pscf {
+
+static int xc_max_lds() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 65536;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+      cached = v;
+    else
+      cached = 65536;
+  }
+  return cached;
+}
+
+// ---------------------------------------------------------------- AO eval ---
+template <typename T>
+__global__ void __launch_bounds__(256) ao_eval_kernel(const int* __restrict__ atm, const int* __restrict__ bas,
+                                                      int nbas, const double* __restrict__ env,
+                                                      const double* __restrict__ coords, int g, int nao,
+                                                      T* __restrict__ ao) {
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= g) return;
+  const double x = coords[3 * pt], y = coords[3 * pt + 1], z = coords[3 * pt + 2];
+  const int64_t cs = (int64_t)nao * g;
+  int mu = 0;
+  for (int ib = 0; ib < nbas; ++ib) {
+    const int* b = bas + ib * BAS_SLOTS;
+    const double* R = env + atm[b[0] * ATM_SLOTS + 1];
+    const int l = b[1], np = b[2];
+    const double* e = env + b[5];
+    const double* c = env + b[6];
+    const double dx = x - R[0], dy = y - R[1], dz = z - R[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double rad = 0.0, drad = 0.0;
+    for (int k = 0; k < np; ++k) {
+      const double ek = c[k] * exp(-e[k] * r2);
+      rad += ek;
+      drad += e[k] * ek;
+    }
+    drad *= -2.0;
+    if (l == 0) {
+      rad *= ANG_S;
+      drad *= ANG_S;
+      ao[(int64_t)mu * g + pt] = (T)rad;
+      ao[cs + (int64_t)mu * g + pt] = (T)(drad * dx);
+      ao[2 * cs + (int64_t)mu * g + pt] = (T)(drad * dy);
+      ao[3 * cs + (int64_t)mu * g + pt] = (T)(drad * dz);
+      mu += 1;
+    } else {
+      rad *= ANG_P;
+      drad *= ANG_P;
+      const double d[3] = {dx, dy, dz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int64_t o = (int64_t)(mu + a) * g + pt;
+        ao[o] = (T)(d[a] * rad);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ao[(1 + q) * cs + o] = (T)((a == q ? rad : 0.0) + d[a] * d[q] * drad);
+      }
+      mu += 3;
+    }
+  }
+}
+
+// ------------------------------------------------------------ Becke cells ---
+// w = vol * P_owner / sum_i P_i with Becke's three-fold smoothed step and
+// Treutler's Bragg-radius adjustment a_ij (pyscf_ipu/pyscf_utils/build_grid.py:34-100).
+__global__ void __launch_bounds__(256) becke_kernel(const double* __restrict__ coords, const int* __restrict__ owner,
+                                                    const double* __restrict__ vol, int g,
+                                                    const double* __restrict__ atom_xyz, const double* __restrict__ adj,
+                                                    int natm, double* __restrict__ w) {
+  extern __shared__ double sh[];
+  double* axyz = sh;             // natm * 3
+  double* rinv = sh + 3 * natm;  // natm * natm (1 / R_ij)
+  const double* a = adj;         // natm * natm, read through the cache
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x) axyz[i] = atom_xyz[i];
+  for (int i = threadIdx.x; i < natm * natm; i += blockDim.x) {
+    const int p = i / natm, q = i % natm;
+    const double dx = atom_xyz[3 * p] - atom_xyz[3 * q], dy = atom_xyz[3 * p + 1] - atom_xyz[3 * q + 1],
+                 dz = atom_xyz[3 * p + 2] - atom_xyz[3 * q + 2];
+    rinv[i] = p == q ? 0.0 : 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  __syncthreads();
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= g) return;
+  const double x = coords[3 * pt], y = coords[3 * pt + 1], z = coords[3 * pt + 2];
+  const int own = owner[pt];
+  double psum = 0.0, pown = 0.0;
+  for (int i = 0; i < natm; ++i) {
+    const double dxi = x - axyz[3 * i], dyi = y - axyz[3 * i + 1], dzi = z - axyz[3 * i + 2];
+    const double ri = sqrt(dxi * dxi + dyi * dyi + dzi * dzi);
+    double pi = 1.0;
+    for (int j = 0; j < natm; ++j) {
+      if (j == i) continue;
+      const double dxj = x - axyz[3 * j], dyj = y - axyz[3 * j + 1], dzj = z - axyz[3 * j + 2];
+      const double rj = sqrt(dxj * dxj + dyj * dyj + dzj * dzj);
+      // mu_ij with the adjustment of the ordered pair (i, j)
+      double gg = (ri - rj) * rinv[i * natm + j];
+      gg = gg + a[i * natm + j] * (1.0 - gg * gg);
+      gg = (3.0 - gg * gg) * gg * 0.5;
+      gg = (3.0 - gg * gg) * gg * 0.5;
+      gg = (3.0 - gg * gg) * gg * 0.5;
+      pi *= 0.5 * (1.0 - gg);
+    }
+    psum += pi;
+    if (i == own) pown = pi;
+  }
+  w[pt] = vol[pt] * pown / psum;
+}
+
+// --------------------------------------------------------------- B3LYP -----
+template <typename T>
+__global__ void b3lyp_kernel(const T* __restrict__ rho, int g, T eps, T* __restrict__ exc, T* __restrict__ vrho,
+                             T* __restrict__ vnorm) {
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= g) return;
+  T e, v, vn;
+  b3lyp_point<T>(rho[pt], rho[g + pt], rho[2 * g + pt], rho[3 * g + pt], eps, e, v, vn);
+  exc[pt] = e;
+  vrho[pt] = v;
+  vnorm[pt] = vn;
+}
+
+// ------------------------------------------------------------- fused XC ----
+constexpr int XC_TILE = 64;
+constexpr int XC_THREADS = 256;
+constexpr int XC_MAX_BLOCKS = 256;
+
+template <typename T>
+struct XCArgs {
+  const T* ao;
+  int64_t sc, smu, sp;
+  int g, N;
+  const T* dm;
+  const T* w;
+  T eps;
+  T* part_v;        // [blocks][N*N]
+  double* part_e;   // [blocks]
+};
+
+// LDS layout: D[N][N] | phi0[N][TILE+1] | XY[N][TILE+1] | coef[4][TILE] | red[8]
+template <typename T>
+__global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.N, TP = XC_TILE + 1;
+  T* D = reinterpret_cast<T*>(smem);
+  T* phi0 = D + N * N;
+  T* XY = phi0 + N * TP;
+  T* coef = XY + N * TP;
+  double* red = reinterpret_cast<double*>(coef + 4 * XC_TILE);
+  for (int x = threadIdx.x; x < N * N; x += blockDim.x) D[x] = a.dm[x];
+  const int tid = threadIdx.x;
+  const int gl = tid & (XC_TILE - 1);  // point within tile
+  const int sl = tid / XC_TILE;        // slice 0..3
+  // V register blocks: 4x4 tiles over padded N, tiles t = tid, tid + 256, ...
+  const int NB = (N + 3) / 4;
+  const int ntile = NB * NB;
+  constexpr int MAXT = 2;  // supports NB*NB <= 512 (N <= 88)
+  T vacc[MAXT][16];
+#pragma unroll
+  for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) vacc[u][k] = T(0);
+  double e_loc = 0.0;
+  const int ntiles = (a.g + XC_TILE - 1) / XC_TILE;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int p0 = tile * XC_TILE;
+    const int pt = p0 + gl;
+    const bool valid = pt < a.g;
+    __syncthreads();
+    // stage phi0 tile
+    for (int mu = sl; mu < N; mu += 4) phi0[mu * TP + gl] = valid ? a.ao[mu * a.smu + (int64_t)pt * a.sp] : T(0);
+    __syncthreads();
+    // X[nu][p] = sum_mu phi0[mu][p] D[mu][nu]  (slice sl owns nu = sl, sl+4, ...)
+    for (int nu = sl; nu < N; nu += 4) {
+      T s = T(0);
+      for (int mu = 0; mu < N; ++mu) s += phi0[mu * TP + gl] * D[mu * N + nu];
+      XY[nu * TP + gl] = s;
+    }
+    __syncthreads();
+    // rho_c[p] = sum_nu X[nu][p] phi_c[nu][p]; slice sl = component c
+    {
+      T r = T(0);
+      if (valid) {
+        if (sl == 0) {
+          for (int nu = 0; nu < N; ++nu) r += XY[nu * TP + gl] * phi0[nu * TP + gl];
+        } else {
+          const T* pc = a.ao + sl * a.sc + (int64_t)pt * a.sp;
+          for (int nu = 0; nu < N; ++nu) r += XY[nu * TP + gl] * pc[nu * a.smu];
+        }
+      }
+      coef[sl * XC_TILE + gl] = r;
+    }
+    __syncthreads();
+    if (sl == 0) {
+      T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0);
+      if (valid) {
+        const T r0 = coef[gl], hx = coef[XC_TILE + gl], hy = coef[2 * XC_TILE + gl], hz = coef[3 * XC_TILE + gl];
+        T e, vr, vn;
+        b3lyp_point<T>(r0, hx, hy, hz, a.eps, e, vr, vn);
+        const T wt = a.w[pt];
+        e_loc += (double)r0 * (double)wt * (double)e;
+        c0 = T(0.5) * vr * wt;
+        const T f = T(4) * vn * wt;
+        c1 = f * hx;
+        c2 = f * hy;
+        c3 = f * hz;
+      }
+      coef[gl] = c0;
+      coef[XC_TILE + gl] = c1;
+      coef[2 * XC_TILE + gl] = c2;
+      coef[3 * XC_TILE + gl] = c3;
+    }
+    __syncthreads();
+    // Y[nu][p] = c0 phi0 + sum_c c_c phi_c
+    {
+      const T c0 = coef[gl], c1 = coef[XC_TILE + gl], c2 = coef[2 * XC_TILE + gl], c3 = coef[3 * XC_TILE + gl];
+      for (int nu = sl; nu < N; nu += 4) {
+        T y = T(0);
+        if (valid) {
+          const int64_t o = nu * a.smu + (int64_t)pt * a.sp;
+          y = c0 * phi0[nu * TP + gl] + c1 * a.ao[a.sc + o] + c2 * a.ao[2 * a.sc + o] + c3 * a.ao[3 * a.sc + o];
+        }
+        XY[nu * TP + gl] = y;
+      }
+    }
+    __syncthreads();
+    // V[mu][nu] += sum_p phi0[mu][p] Y[nu][p]
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) {
+      const int t = tid + u * XC_THREADS;
+      if (t < ntile) {
+        const int bm = (t / NB) * 4, bn = (t % NB) * 4;
+        for (int p = 0; p < XC_TILE; ++p) {
+          T am[4], bv[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            am[k] = (bm + k < N) ? phi0[(bm + k) * TP + p] : T(0);
+            bv[k] = (bn + k < N) ? XY[(bn + k) * TP + p] : T(0);
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vacc[u][i * 4 + j] += am[i] * bv[j];
+        }
+      }
+    }
+  }
+  // write this block's partial V and E
+  T* dst = a.part_v + (int64_t)blockIdx.x * N * N;
+#pragma unroll
+  for (int u = 0; u < MAXT; ++u) {
+    const int t = tid + u * XC_THREADS;
+    if (t < ntile) {
+      const int bm = (t / NB) * 4, bn = (t % NB) * 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (bm + i < N && bn + j < N) dst[(bm + i) * N + bn + j] = vacc[u][i * 4 + j];
+    }
+  }
+  // block reduction of E (only slice-0 lanes carry values)
+  for (int off = 32; off > 0; off >>= 1) e_loc += __shfl_xor(e_loc, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = e_loc;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int k = 0; k < XC_THREADS / 64; ++k) s += red[k];
+    a.part_e[blockIdx.x] = s;
+  }
+}
+
+template <typename T>
+__global__ void xc_reduce_kernel(const T* __restrict__ part_v, const double* __restrict__ part_e, int nb, int N,
+                                 T* __restrict__ vxc, double* __restrict__ exc) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NN = N * N;
+  if (x < NN) {
+    const int i = x / N, j = x - i * N;
+    T s = T(0);
+    for (int b = 0; b < nb; ++b) s += part_v[(int64_t)b * NN + x] + part_v[(int64_t)b * NN + j * N + i];
+    vxc[x] = s;
+  }
+  if (x == 0) {
+    double s = 0.0, c = 0.0;  // Neumaier
+    for (int b = 0; b < nb; ++b) {
+      const double v = part_e[b], t = s + v;
+      c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    *exc = s + c;
+  }
+}
+
+template <typename T>
+static size_t xc_lds(int N) {
+  return sizeof(T) * ((size_t)N * N + 2 * (size_t)N * (XC_TILE + 1) + 4 * XC_TILE) + 8 * sizeof(double);
+}
+
+static int xc_blocks(int g) {
+  const int ntiles = (g + XC_TILE - 1) / XC_TILE;
+  return ntiles < XC_MAX_BLOCKS ? (ntiles < 1 ? 1 : ntiles) : XC_MAX_BLOCKS;
+}
+
+template <typename T>
+static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int N, const T* dm, const T* w,
+                   double eps, double* exc, T* vxc, void* ws, size_t ws_bytes, hipStream_t st) {
+  const size_t lds = xc_lds<T>(N);
+  PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_xc: N=%d needs %zu B LDS", N, lds);
+  const int NB = (N + 3) / 4;
+  PSCF_REQUIRE(NB * NB <= 2 * XC_THREADS, "pscf_xc: N=%d too large for the register-blocked V (max 88)", N);
+  const int nb = xc_blocks(g);
+  const size_t need = sizeof(T) * (size_t)nb * N * N + sizeof(double) * nb;
+  PSCF_REQUIRE(ws && ws_bytes >= need, "pscf_xc: workspace too small (%zu < %zu)", ws_bytes, need);
+  XCArgs<T> a;
+  a.ao = ao;
+  a.sc = sc;
+  a.smu = smu;
+  a.sp = sp;
+  a.g = g;
+  a.N = N;
+  a.dm = dm;
+  a.w = w;
+  a.eps = (T)eps;
+  a.part_v = (T*)ws;
+  a.part_e = (double*)((char*)ws + sizeof(T) * (size_t)nb * N * N);
+  if (lds > 65536)
+    PSCF_HIP(hipFuncSetAttribute((const void*)xc_fused_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  xc_fused_kernel<T><<<nb, XC_THREADS, lds, st>>>(a);
+  PSCF_HIP(hipGetLastError());
+  xc_reduce_kernel<T><<<(N * N + 255) / 256, 256, 0, st>>>(a.part_v, a.part_e, nb, N, vxc, exc);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+}  // namespace pscf
+
+using namespace pscf;
+
+extern "C" int pscf_ao_eval(int dtype, const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env,
+                            int nao, const double* d_coords, int g, void* d_ao, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_ao_eval: bad dtype");
+  PSCF_REQUIRE(natm > 0 && nbas > 0 && nao > 0 && g >= 0, "pscf_ao_eval: bad sizes");
+  if (g == 0) return PSCF_OK;
+  PSCF_REQUIRE(d_atm && d_bas && d_env && d_coords && d_ao, "pscf_ao_eval: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    ao_eval_kernel<double><<<(g + 255) / 256, 256, 0, st>>>(d_atm, d_bas, nbas, d_env, d_coords, g, nao, (double*)d_ao);
+  else
+    ao_eval_kernel<float><<<(g + 255) / 256, 256, 0, st>>>(d_atm, d_bas, nbas, d_env, d_coords, g, nao, (float*)d_ao);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_becke_weights(const double* d_coords, const int* d_owner, const double* d_vol, int g,
+                                  const double* d_atom_xyz, const double* d_adjust, int natm, double* d_w,
+                                  void* stream) {
+  PSCF_REQUIRE(g >= 0 && natm > 0 && natm <= 88, "pscf_becke_weights: natm=%d out of range (1..88)", natm);
+  if (g == 0) return PSCF_OK;
+  PSCF_REQUIRE(d_coords && d_owner && d_vol && d_atom_xyz && d_adjust && d_w, "pscf_becke_weights: null pointer");
+  const size_t lds = sizeof(double) * (3 * (size_t)natm + (size_t)natm * natm);
+  becke_kernel<<<(g + 255) / 256, 256, lds, (hipStream_t)stream>>>(d_coords, d_owner, d_vol, g, d_atom_xyz, d_adjust,
+                                                                  natm, d_w);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_b3lyp(int dtype, const void* d_rho, int g, double eps, void* d_exc, void* d_vrho, void* d_vnorm,
+                          void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_b3lyp: bad dtype");
+  PSCF_REQUIRE(g >= 0, "pscf_b3lyp: g < 0");
+  if (g == 0) return PSCF_OK;
+  PSCF_REQUIRE(d_rho && d_exc && d_vrho && d_vnorm, "pscf_b3lyp: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    b3lyp_kernel<double><<<(g + 255) / 256, 256, 0, st>>>((const double*)d_rho, g, eps, (double*)d_exc,
+                                                          (double*)d_vrho, (double*)d_vnorm);
+  else
+    b3lyp_kernel<float><<<(g + 255) / 256, 256, 0, st>>>((const float*)d_rho, g, (float)eps, (float*)d_exc,
+                                                         (float*)d_vrho, (float*)d_vnorm);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+// Host evaluation of the same pointwise code (FP64), for CPU-side checks of
+// the hand-derived derivatives.
+extern "C" int pscf_b3lyp_host(const double* h_rho, int g, double eps, double* h_exc, double* h_vrho,
+                               double* h_vnorm) {
+  PSCF_REQUIRE(g >= 0 && (g == 0 || (h_rho && h_exc && h_vrho && h_vnorm)), "pscf_b3lyp_host: bad arguments");
+  for (int p = 0; p < g; ++p)
+    b3lyp_point<double>(h_rho[p], h_rho[g + p], h_rho[2 * g + p], h_rho[3 * g + p], eps, h_exc[p], h_vrho[p],
+                        h_vnorm[p]);
+  return PSCF_OK;
+}
+
+extern "C" size_t pscf_xc_workspace_bytes(int dtype, int N, int g) {
+  const size_t es = dtype == PSCF_F64 ? 8 : 4;
+  const size_t nb = (size_t)xc_blocks(g);
+  return es * nb * N * N + sizeof(double) * nb;
+}
+
+extern "C" int pscf_xc(int dtype, const void* d_ao, int64_t ao_sc, int64_t ao_smu, int64_t ao_sp, int g, int N,
+                       const void* d_dm, const void* d_weights, double eps, double* d_exc, void* d_vxc, void* d_ws,
+                       size_t ws_bytes, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_xc: bad dtype");
+  PSCF_REQUIRE(g > 0 && N > 0, "pscf_xc: empty grid or basis");
+  PSCF_REQUIRE(d_ao && d_dm && d_weights && d_exc && d_vxc, "pscf_xc: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    return xc_impl<double>((const double*)d_ao, ao_sc, ao_smu, ao_sp, g, N, (const double*)d_dm,
+                           (const double*)d_weights, eps, d_exc, (double*)d_vxc, d_ws, ws_bytes, st);
+  return xc_impl<float>((const float*)d_ao, ao_sc, ao_smu, ao_sp, g, N, (const float*)d_dm, (const float*)d_weights,
+                        eps, d_exc, (float*)d_vxc, d_ws, ws_bytes, st);
+}

@@ -1,0 +1,297 @@
+"""nanoDFT on the MI355X: restricted Kohn-Sham B3LYP SCF (drop-in for pyscf_ipu.nanoDFT.nanoDFT).
+
+Kept names and meanings (pyscf_ipu/nanoDFT/nanoDFT.py):
+  energy(density_matrix, H_core, diff_JK, E_xc, E_nuc)          :20-38
+  exchange_correlation(density_matrix, grid_AO, grid_weights)   :105-123
+  get_JK(density_matrix, ERI, dense_ERI, backend)               :125-139
+  nanoDFT(mol, opts) -> (energies (its, 5) eV, (logged_energies, hlgaps,
+                          mo_energy, mo_coeff, grid_coords, grid_weights))  :259-336
+  hlgap(L_inv, H, n_electrons_half, _np)                        :366-368
+  nanoDFT_options(...)                                          :548-618
+  main()                                                        :620-643
+
+Per molecule everything after the host set-up runs in the library's kernels:
+int1e, ERIs (packed s8), AO values, Becke weights, and per iteration the fused
+XC kernel, the s8 J/K kernel, the one-workgroup SCF step (DIIS + Jacobi eigh
++ density) and the FP64 energy reduction.  The host keeps the grid templates,
+the Cholesky of S and the initial guess, and reads one energy per iteration
+to stop at |dE| < conv_tol (the reference always runs `its` iterations).
+"""
+from __future__ import annotations
+
+import argparse
+import time
+from collections import namedtuple
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+import torch
+
+from .. import _lib, device, grids, guess
+from ..electron_repulsion import direct
+from ..gto import Mole, build_mol
+from . import sparse_symmetric_ERI as ssE
+
+HARTREE_TO_EV = 27.2114079527
+EPSILON_B3LYP = 1e-20
+HYB_B3LYP = 0.2
+
+
+def energy(density_matrix, H_core, diff_JK, E_xc, E_nuc, _np=np):
+    E_core = _np.sum(density_matrix * H_core)
+    E_J_K = _np.sum(density_matrix * diff_JK)
+    E = E_core + E_J_K / 2 + E_xc + E_nuc
+    return _np.array([E, E_core, E_J_K / 2, E_xc, E_nuc])
+
+
+def hlgap(L_inv, H, n_electrons_half, _np=np):
+    mo_energy = _np.linalg.eigh(L_inv @ H @ L_inv.T)[0]
+    return _np.abs(mo_energy[n_electrons_half] - mo_energy[n_electrons_half - 1])
+
+
+def _dt(x):
+    return torch.float64 if x.dtype == torch.float64 else torch.float32
+
+
+def exchange_correlation(density_matrix, grid_AO, grid_weights, eps: float = EPSILON_B3LYP):
+    """(E_xc, V_xc) for grid_AO in the reference's (4, g, N) layout (device or host)."""
+    is_np = not isinstance(density_matrix, torch.Tensor)
+    ao = grid_AO if isinstance(grid_AO, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(grid_AO))
+    dt = _dt(ao)
+    code = _lib.F64 if dt == torch.float64 else _lib.F32
+    ao = device.to_dev(ao, dt)
+    _, g, N = ao.shape
+    dm = device.to_dev(density_matrix if not is_np else np.asarray(density_matrix), dt).reshape(N, N)
+    w = device.to_dev(grid_weights if isinstance(grid_weights, torch.Tensor) else np.asarray(grid_weights), dt)
+    exc = device.empty(1, torch.float64)
+    vxc = device.empty((N, N), dt)
+    wsb = _lib.lib().pscf_xc_workspace_bytes(code, N, g)
+    ws = device.workspace(wsb)
+    _lib.call("pscf_xc", code, device.ptr(ao), g * N, 1, N, g, N, device.ptr(dm), device.ptr(w), float(eps),
+              device.ptr(exc), device.ptr(vxc), device.ptr(ws), wsb, device.stream())
+    if is_np:
+        return float(exc.item()), vxc.cpu().numpy()
+    return exc[0], vxc
+
+
+def get_JK(density_matrix, ERI, dense_ERI=False, backend="mi355x"):
+    """dJK = J - K/2 * HYB_B3LYP.  ERI: dense (N,N,N,N) when dense_ERI, else
+    (nonzero_distinct_ERI, nonzero_indices) as nanoDFT prepares them."""
+    N = density_matrix.shape[0]
+    if dense_ERI:
+        raise NotImplementedError("dense_ERI: the MI355X path keeps ERIs in packed s8 form; "
+                                  "pass (values, indices) or use the s8 driver")
+    out = ssE.sparse_symmetric_einsum(ERI[0], ERI[1], density_matrix, backend)
+    return out.reshape(N, N)
+
+
+# ------------------------------------------------------------------ driver ---
+@dataclass
+class SCFResult:
+    energy: float                 # final total energy (Ha)
+    energies: np.ndarray          # (n_iter, 5) [E, E_core, E_JK, E_xc, E_nuc] Ha
+    hlgaps: np.ndarray            # (n_iter,) Ha
+    mo_energy: np.ndarray
+    mo_coeff: np.ndarray
+    dm: np.ndarray
+    iterations: int
+    converged: bool
+    timings: dict = field(default_factory=dict)
+
+
+class RKS:
+    """Plain form: one molecule, one sequence of launches per SCF phase.
+
+    dtype "fp64": every kernel in FP64.  dtype "fp32": ERIs, J/K and XC in FP32;
+    SCF step (DIIS, eigh, density) and the energy reduction in FP64."""
+
+    def __init__(self, mol: Mole, level: int = 0, dtype: str = "fp64", grid: Optional[grids.Grid] = None):
+        self.mol = mol
+        self.level = level
+        self.dtype = dtype
+        self.tdt, self.code, _ = device.DTYPES[dtype]
+        self.grid = grid
+        self._setup_done = False
+
+    # -- set-up ----------------------------------------------------------------
+    def setup(self, dm0: Optional[np.ndarray] = None):
+        mol = self.mol
+        t0 = time.perf_counter()
+        dev = device.require_gpu()
+        self.N = N = mol.nao_nr()
+        self.nocc = mol.nelectron // 2
+        atm, bas, env = direct._mol_device(mol)
+        self.dev_mol = (atm, bas, env)
+        f64 = torch.float64
+        S, T, V = device.empty((N, N), f64), device.empty((N, N), f64), device.empty((N, N), f64)
+        _lib.call("pscf_int1e", device.ptr(atm), mol.natm, device.ptr(bas), mol.nbas, device.ptr(env), N,
+                  device.ptr(S), device.ptr(T), device.ptr(V), device.stream())
+        self.S = S
+        self.Hcore = (T + V).contiguous()
+        S_h = S.cpu().numpy()
+        L = np.linalg.cholesky(S_h)
+        self.L_inv_h = np.linalg.inv(L)
+        self.L_inv = device.to_dev(self.L_inv_h)
+        # grid: atom-centred templates on the host, Becke weights on the device
+        g = self.grid if self.grid is not None else grids.atomic_grids(mol, self.level)
+        self.g = g.size
+        self.coords = device.to_dev(g.coords)
+        if g.weights is None:
+            owner, vol = device.to_dev(g.owner.astype(np.int32)), device.to_dev(g.vol)
+            axyz = device.to_dev(mol.atom_coords())
+            adj = device.to_dev(grids.becke_adjust_table(mol.atom_charges()))
+            w64 = device.empty(g.size, f64)
+            _lib.call("pscf_becke_weights", device.ptr(self.coords), device.ptr(owner), device.ptr(vol), g.size,
+                      device.ptr(axyz), device.ptr(adj), mol.natm, device.ptr(w64), device.stream())
+        else:
+            w64 = device.to_dev(g.weights)
+        self.weights64 = w64
+        self.weights = w64.to(self.tdt)
+        self.ao = device.empty((4, N, g.size), self.tdt)
+        _lib.call("pscf_ao_eval", self.code, device.ptr(atm), mol.natm, device.ptr(bas), mol.nbas, device.ptr(env), N,
+                  device.ptr(self.coords), g.size, device.ptr(self.ao), device.stream())
+        # ERIs, packed s8
+        self.eri = direct.eri_s8(mol, self.dtype, dev_mol=self.dev_mol)
+        # work buffers
+        self.D = device.to_dev(guess.minao(mol, S_h) if dm0 is None else dm0, f64)
+        self.Dt = self.D.to(self.tdt) if self.tdt != f64 else self.D
+        self.dJK = device.empty((N, N), self.tdt)
+        self.Vxc = device.empty((N, N), self.tdt)
+        self.exc = device.empty(1, f64)
+        self.H = device.empty((N, N), f64)
+        self.C = device.empty((N, N), f64)
+        self.mo_e = device.empty(N, f64)
+        self.E = device.empty(5, f64)
+        self.state = device.zeros(_lib.lib().pscf_scf_state_bytes(N) // 8, f64)
+        self.ws_xc_b = _lib.lib().pscf_xc_workspace_bytes(self.code, N, self.g)
+        self.ws_xc = device.workspace(self.ws_xc_b)
+        self.ws_jk_b = _lib.lib().pscf_jk_s8_workspace_bytes(self.code, N)
+        self.ws_jk = device.workspace(self.ws_jk_b)
+        self.E_nuc = mol.energy_nuc()
+        self._setup_done = True
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # -- kernels of one iteration ---------------------------------------------------
+    def fock_parts(self):
+        """V_xc, E_xc and dJK of the current density (self.D)."""
+        N = self.N
+        if self.tdt != torch.float64:
+            _lib.call("pscf_cast", _lib.F64, device.ptr(self.D), self.code, device.ptr(self.Dt), N * N,
+                      device.stream())
+        _lib.call("pscf_xc", self.code, device.ptr(self.ao), N * self.g, self.g, 1, self.g, N, device.ptr(self.Dt),
+                  device.ptr(self.weights), EPSILON_B3LYP, device.ptr(self.exc), device.ptr(self.Vxc),
+                  device.ptr(self.ws_xc), self.ws_xc_b, device.stream())
+        _lib.call("pscf_jk_s8", self.code, device.ptr(self.eri), N, device.ptr(self.Dt), HYB_B3LYP,
+                  device.ptr(self.dJK), device.ptr(self.ws_jk), self.ws_jk_b, device.stream())
+
+    def energy_terms(self):
+        _lib.call("pscf_energy", self.code, self.N, device.ptr(self.D), device.ptr(self.Hcore), device.ptr(self.dJK),
+                  device.ptr(self.exc), float(self.E_nuc), device.ptr(self.E), device.stream())
+
+    def step(self, it: int, diis: bool = True):
+        _lib.call("pscf_scf_step", self.code, it, self.N, self.nocc, device.ptr(self.Hcore), device.ptr(self.S),
+                  device.ptr(self.L_inv), device.ptr(self.dJK), device.ptr(self.Vxc), device.ptr(self.D),
+                  device.ptr(self.H), device.ptr(self.mo_e), device.ptr(self.C), device.ptr(self.state), int(diis),
+                  device.stream())
+
+    def run(self, its: int = 20, conv_tol: float = 1e-8, diis: bool = True, dm0=None) -> SCFResult:
+        t0 = time.perf_counter()
+        if not self._setup_done:
+            self.setup(dm0)
+        t1 = time.perf_counter()
+        self.fock_parts()
+        self.energy_terms()
+        E_prev = float(self.E[0].item())
+        log, gaps = [], []
+        converged = False
+        for i in range(its):
+            self.step(i, diis)
+            self.fock_parts()
+            self.energy_terms()
+            e = self.E.cpu().numpy().copy()
+            mo = self.mo_e.cpu().numpy()
+            log.append(e)
+            gaps.append(abs(mo[self.nocc] - mo[self.nocc - 1]))
+            if abs(e[0] - E_prev) < conv_tol:
+                converged = True
+                break
+            E_prev = e[0]
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        log = np.array(log)
+        return SCFResult(float(log[-1, 0]), log, np.array(gaps), self.mo_e.cpu().numpy(), self.C.cpu().numpy(),
+                         self.D.cpu().numpy(), len(log), converged, {"setup_s": t1 - t0, "scf_s": t2 - t1})
+
+
+def nanoDFT(mol, opts):
+    """Reference-shaped entry point: returns (energies (n, 5) in eV,
+    (logged_energies eV, hlgaps eV, mo_energy, mo_coeff, grid_coords, grid_weights))."""
+    dtype = "fp32" if getattr(opts, "float32", False) else "fp64"
+    conv = getattr(opts, "conv_tol", 0.0)
+    rks = RKS(mol, level=opts.level, dtype=dtype)
+    res = rks.run(its=opts.its, conv_tol=conv if conv else -1.0, diis=opts.diis)
+    energies = res.energies * HARTREE_TO_EV
+    grid_coords = rks.coords.cpu().numpy()
+    grid_weights = rks.weights64.cpu().numpy()
+    return energies, (energies.copy(), res.hlgaps * HARTREE_TO_EV, res.mo_energy, res.mo_coeff, grid_coords,
+                      grid_weights)
+
+
+_OPTION_FIELDS = ["its", "mol_str", "float32", "basis", "xc", "backend", "level", "multv", "intv", "threads",
+                  "threads_int", "diis", "structure_optimization", "eri_threshold", "ao_threshold", "batches",
+                  "ndevices", "dense_ERI", "v", "profile", "vis_num_error", "molecule_name", "conv_tol"]
+
+
+def nanoDFT_options(its: int = 20, mol_str: str = "methane", float32: bool = False, basis: str = "sto-3g",
+                    xc: str = "b3lyp", backend: str = "mi355x", level: int = 0, multv: int = 2, intv: int = 1,
+                    threads: int = 1, threads_int: int = 1, diis: bool = True, structure_optimization: bool = False,
+                    eri_threshold: float = 0.0, ao_threshold: float = 0.0, batches: int = 32, ndevices: int = 1,
+                    dense_ERI: bool = False, v: bool = False, profile: bool = False, vis_num_error: bool = False,
+                    molecule_name: str = None, conv_tol: float = 0.0):
+    """Options namedtuple with the reference's field names (nanoDFT.py:548-571)
+    plus `conv_tol` (0 = run all `its` iterations like the reference).  Returns
+    (opts, mol_str); `mol_str` is an atom string ("C 0 0 0; H ...") or
+    "methane"; molecules are never fetched from the network."""
+    if molecule_name is None:
+        molecule_name = mol_str
+    if mol_str.strip().lower() == "methane":
+        from ..gto import METHANE
+        mol_str = METHANE
+    if xc.lower() != "b3lyp":
+        raise ValueError("only b3lyp is supported")
+    if ndevices != 1:
+        raise ValueError("one GPU per process: ndevices must be 1")
+    vals = dict(its=its, mol_str=mol_str, float32=float32, basis=basis, xc=xc, backend=backend, level=level,
+                multv=multv, intv=intv, threads=threads, threads_int=threads_int, diis=diis,
+                structure_optimization=structure_optimization, eri_threshold=eri_threshold,
+                ao_threshold=ao_threshold, batches=batches, ndevices=ndevices, dense_ERI=dense_ERI, v=v,
+                profile=profile, vis_num_error=vis_num_error, molecule_name=molecule_name, conv_tol=conv_tol)
+    opts = namedtuple("DFTOptionsImmutable", _OPTION_FIELDS)(**vals)
+    return opts, mol_str
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser(description="nanoDFT (MI355X): RKS B3LYP SCF")
+    p.add_argument("--its", type=int, default=20)
+    p.add_argument("--mol_str", default="methane")
+    p.add_argument("--float32", action="store_true")
+    p.add_argument("--basis", default="sto-3g")
+    p.add_argument("--level", type=int, default=0)
+    p.add_argument("--conv_tol", type=float, default=0.0)
+    p.add_argument("--no-diis", dest="diis", action="store_false")
+    a = p.parse_args(argv)
+    opts, mol_str = nanoDFT_options(its=a.its, mol_str=a.mol_str, float32=a.float32, basis=a.basis, level=a.level,
+                                    diis=a.diis, conv_tol=a.conv_tol)
+    mol = build_mol(mol_str, opts.basis)
+    print(f"Number of Atomic Orbitals\t{mol.nao_nr():15d}")
+    print(f"Number of electrons\t{mol.nelectron:15d}")
+    E, (logged, gaps, *_rest) = nanoDFT(mol, opts)
+    for i, (e, g) in enumerate(zip(E[:, 0], gaps)):
+        print(f"iter {i:3d}  E = {e / HARTREE_TO_EV:.10f} Ha   HL gap = {g:.6f} eV")
+    return E
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,256 @@
+"""GPU parity of each kernel against the FP64 CPU oracle, through the C-ABI.
+
+Tolerances: FP64 kernels to near machine precision (stated per test); FP32
+kernels to the relative accuracy FP32 arithmetic allows (stated per test).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ao as oracle_ao
+from oracle import cint, rks
+from oracle import xc as oracle_xc
+from pyscf_ipu_amd import device, grids, molgen
+from pyscf_ipu_amd.gto import METHANE, Mole, build_mol
+
+pytestmark = pytest.mark.gpu
+
+MOLS = {
+    "methane": METHANE,
+    "hcno": "N 0 0 0; C 1.3 0.2 0; O -1.1 0.5 0.3; H 0 1 -0.8; H 1.9 -0.7 0.1; H 1.6 1.1 0.4",
+    "gen3": molgen.plain_set(1, 3, seed=7)[0],
+}
+
+
+@pytest.fixture(scope="module", params=list(MOLS))
+def mol(request):
+    return build_mol(MOLS[request.param])
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_int1e(hip, mol):
+    from pyscf_ipu_amd import _lib
+    from pyscf_ipu_amd.electron_repulsion import direct
+    print(f"int1e nao={mol.nao_nr()}", flush=True)
+    atm, bas, env = direct._mol_device(mol)
+    N = mol.nao_nr()
+    S, T, V = (device.empty((N, N), torch.float64) for _ in range(3))
+    _lib.call("pscf_int1e", device.ptr(atm), mol.natm, device.ptr(bas), mol.nbas, device.ptr(env), N, device.ptr(S),
+              device.ptr(T), device.ptr(V), device.stream())
+    So, To, Vo = cint.int1e(mol)
+    assert np.abs(_np(S) - So).max() < 1e-12
+    assert np.abs(_np(T) - To).max() < 1e-11
+    assert np.abs(_np(V) - Vo).max() < 1e-10
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+def test_eri_s8(hip, mol, dtype, tol):
+    from pyscf_ipu_amd.electron_repulsion import direct
+    print(f"eri_s8 nao={mol.nao_nr()} {dtype}", flush=True)
+    got = _np(direct.eri_s8(mol, dtype)).astype(np.float64)
+    ref = cint.eri_s8(mol)
+    err = np.abs(got - ref)
+    # absolute error relative to the largest integral (FP32: ~1e-7 relative arithmetic)
+    assert err.max() < tol * max(1.0, np.abs(ref).max())
+
+
+def test_eri_quartets_block_layout(hip):
+    from pyscf_ipu_amd.electron_repulsion import direct
+    mol = build_mol(MOLS["hcno"])
+    q = np.array([[3, 1, 2, 0], [0, 3, 4, 2], [4, 4, 3, 3], [2, 2, 2, 2], [5, 0, 4, 1], [1, 4, 4, 0]],
+                 dtype=np.int32)
+    flat = _np(direct.eri_quartets(mol, q, "fp64"))
+    off = 0
+    for sh in q:
+        ref = cint.eri_quartet(mol, sh)  # (di, dj, dk, dl)
+        n = ref.size
+        blk = flat[off:off + n].reshape(ref.shape[::-1]).transpose(3, 2, 1, 0)
+        assert np.abs(blk - ref).max() < 1e-12, sh
+        off += n
+
+
+def test_electron_repulsion_integrals_classes(hip):
+    from pyscf_ipu_amd.electron_repulsion import direct
+    mol = build_mol(METHANE)
+    out = direct.electron_repulsion_integrals(None, None, mol)
+    _, _, tuple_ijkl, _, sizes, counts, _, _, _ = direct.prepare_electron_repulsion_integrals(mol)
+    assert [tuple(o.shape) for o in out] == [(c, s) for s, c in zip(sizes, counts)]
+    for cls, arr in zip(tuple_ijkl, out):
+        a = _np(arr)
+        for r in (0, len(cls) // 2, len(cls) - 1):
+            ref = cint.eri_quartet(mol, cls[r])
+            assert np.abs(a[r].reshape(ref.shape[::-1]).transpose(3, 2, 1, 0) - ref).max() < 2e-6
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+def test_jk_sparse_dropin(hip, dtype, tol):
+    from pyscf_ipu_amd.nanoDFT import sparse_symmetric_ERI as ssE
+    mol = build_mol(MOLS["gen3"])
+    N = mol.nao_nr()
+    e8 = cint.eri_s8(mol)
+    v, idx = ssE.prepare_sparse(e8)
+    rng = np.random.default_rng(1)
+    dm = rng.normal(size=(N, N))  # not symmetric: the drop-in must not assume it
+    ref = cint.sparse_jk(v, idx, dm)
+    npdt = np.float64 if dtype == "fp64" else np.float32
+    # reference shapes: (batches, n) values and (batches, n, 4) int16 viewed as float16
+    nb = 4
+    pad = (-len(v)) % nb
+    vv = np.pad(v, (0, pad)).reshape(nb, -1).astype(npdt)
+    ii = np.pad(idx, ((0, pad), (0, 0))).reshape(nb, -1, 4).view(np.float16)
+    got = ssE.sparse_symmetric_einsum(vv, ii, dm.reshape(-1).astype(npdt), "mi355x").reshape(N, N)
+    assert np.abs(got - ref).max() < tol * max(1.0, np.abs(ref).max())
+
+
+def test_jk_sparse_empty(hip):
+    from pyscf_ipu_amd.nanoDFT import sparse_symmetric_ERI as ssE
+    got = ssE.sparse_symmetric_einsum(np.zeros(0, np.float32), np.zeros((0, 4), np.int16), np.ones(16, np.float32))
+    assert np.all(got == 0)
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-11), ("fp32", 1e-5)])
+def test_jk_s8(hip, mol, dtype, tol):
+    from pyscf_ipu_amd.electron_repulsion import direct
+    from pyscf_ipu_amd.nanoDFT import sparse_symmetric_ERI as ssE
+    N = mol.nao_nr()
+    rng = np.random.default_rng(2)
+    dm = rng.normal(size=(N, N))
+    dm = dm + dm.T
+    e8 = cint.eri_s8(mol)
+    ref = rks.dense_jk(cint.s8_to_s1(e8, N), dm) if N <= 40 else cint.sparse_jk(*rks.s8_sparse(mol, e8), dm)
+    tdt = device.DTYPES[dtype][0]
+    eri = device.to_dev(e8, tdt)
+    got = _np(ssE.jk_s8(eri, device.to_dev(dm, tdt)))
+    assert np.abs(got - ref).max() < tol * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_ao_eval(hip, mol, dtype, tol):
+    from pyscf_ipu_amd import _lib
+    from pyscf_ipu_amd.electron_repulsion import direct
+    g = grids.build_grid(mol, 0)
+    pts = g.coords[::7]
+    tdt, code, _ = device.DTYPES[dtype]
+    atm, bas, env = direct._mol_device(mol)
+    N = mol.nao_nr()
+    out = device.empty((4, N, len(pts)), tdt)
+    c = device.to_dev(pts)
+    _lib.call("pscf_ao_eval", code, device.ptr(atm), mol.natm, device.ptr(bas), mol.nbas, device.ptr(env), N,
+              device.ptr(c), len(pts), device.ptr(out), device.stream())
+    ref = oracle_ao.eval_ao_deriv1(mol, pts)  # (4, g, N)
+    assert np.abs(_np(out).transpose(0, 2, 1) - ref).max() < tol * max(1.0, np.abs(ref).max())
+
+
+def test_becke_weights(hip, mol):
+    from pyscf_ipu_amd import _lib
+    g = grids.atomic_grids(mol, 0)
+    w = device.empty(g.size, torch.float64)
+    _lib.call("pscf_becke_weights", device.ptr(device.to_dev(g.coords)), device.ptr(device.to_dev(g.owner)),
+              device.ptr(device.to_dev(g.vol)), g.size, device.ptr(device.to_dev(mol.atom_coords())),
+              device.ptr(device.to_dev(grids.becke_adjust_table(mol.atom_charges()))), mol.natm, device.ptr(w),
+              device.stream())
+    ref = g.vol * grids.becke_partition(g.coords, mol.atom_coords(), mol.atom_charges(), g.owner)
+    assert np.abs(_np(w) - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+def test_b3lyp_dropin(hip, dtype, tol):
+    from pyscf_ipu_amd.exchange_correlation.b3lyp import b3lyp
+    rng = np.random.default_rng(3)
+    g = 4096
+    rho = np.zeros((4, g))
+    rho[0] = 10 ** rng.uniform(-10, 3, g)
+    rho[1:4] = rng.normal(size=(3, g)) * rho[0] * 10 ** rng.uniform(-2, 0.5, g)
+    npdt = np.float64 if dtype == "fp64" else np.float32
+    got = b3lyp(rho.astype(npdt), 1e-20)
+    ref = oracle_xc.b3lyp(rho.astype(npdt).astype(np.float64), 1e-20)
+    for a, b in zip(got, ref):
+        assert np.max(np.abs(a - b) / (np.abs(b) + 1e-30)) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+def test_exchange_correlation_dropin(hip, mol, dtype, tol):
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import exchange_correlation
+    g = grids.build_grid(mol, 0)
+    ao = oracle_ao.eval_ao_deriv1(mol, g.coords)
+    N = mol.nao_nr()
+    S, _, _ = cint.int1e(mol)
+    from pyscf_ipu_amd import guess
+    dm = guess.minao(mol, S)
+    E_ref, V_ref, _ = oracle_xc.exchange_correlation(dm, ao, g.weights)
+    npdt = np.float64 if dtype == "fp64" else np.float32
+    E, V = exchange_correlation(dm.astype(npdt), ao.astype(npdt), g.weights.astype(npdt))
+    assert abs(E - E_ref) < tol * abs(E_ref)
+    assert np.abs(V - V_ref).max() < tol * max(1.0, np.abs(V_ref).max())
+
+
+def test_scf_step_matches_oracle_iteration(hip):
+    """Two SCF steps (DIIS + eigh + density) from the same Fock inputs."""
+    from pyscf_ipu_amd import _lib
+    mol = build_mol(MOLS["hcno"])
+    N, nocc = mol.nao_nr(), mol.nelectron // 2
+    S, T, V = cint.int1e(mol)
+    Hc = T + V
+    Li = np.linalg.inv(np.linalg.cholesky(S))
+    rng = np.random.default_rng(4)
+    from pyscf_ipu_amd import guess
+    D = guess.minao(mol, S)
+    st = rks.diis_init(N)
+    dD = device.to_dev(D)
+    state = device.zeros(_lib.lib().pscf_scf_state_bytes(N) // 8, torch.float64)
+    Hd, Cd, mo = (device.empty((N, N), torch.float64), device.empty((N, N), torch.float64),
+                  device.empty(N, torch.float64))
+    for it in range(3):
+        dJK = rng.normal(size=(N, N)) * 0.1
+        dJK = dJK + dJK.T
+        Vxc = rng.normal(size=(N, N)) * 0.05
+        Vxc = Vxc + Vxc.T
+        # oracle
+        H = rks.diis(it, Hc + dJK + Vxc, D, S, st)
+        w, v = np.linalg.eigh(Li @ H @ Li.T)
+        C = Li.T @ v
+        D = 2 * C[:, :nocc] @ C[:, :nocc].T
+        # gpu
+        _lib.call("pscf_scf_step", _lib.F64, it, N, nocc, device.ptr(device.to_dev(Hc)), device.ptr(device.to_dev(S)),
+                  device.ptr(device.to_dev(Li)), device.ptr(device.to_dev(dJK)), device.ptr(device.to_dev(Vxc)),
+                  device.ptr(dD), device.ptr(Hd), device.ptr(mo), device.ptr(Cd), device.ptr(state), 1,
+                  device.stream())
+        assert np.abs(_np(Hd) - H).max() < 1e-10
+        assert np.abs(_np(mo) - w).max() < 1e-10
+        assert np.abs(_np(dD) - D).max() < 1e-9
+        dD = device.to_dev(D)  # continue both from the oracle density
+
+
+@pytest.mark.parametrize("name", ["methane", "gen3"])
+def test_full_scf_fp64_matches_oracle(hip, name):
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS
+    mol = build_mol(MOLS[name])
+    grid = grids.build_grid(mol, 0)
+    print(f"scf {name} nao={mol.nao_nr()} g={grid.size}", flush=True)
+    gpu = RKS(mol, level=0, dtype="fp64", grid=grid).run(its=30, conv_tol=1e-9)
+    ref = rks.scf(mol, grid=grid, its=30, conv_tol=1e-9)
+    assert gpu.converged and ref.converged
+    assert abs(gpu.energy - ref.energy) < 1e-7
+    assert abs(gpu.hlgaps[-1] - ref.hlgap) < 1e-5
+    assert np.abs(gpu.dm - ref.dm).max() < 1e-4
+
+
+def test_full_scf_fp32_within_1mha(hip):
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS
+    mol = build_mol(MOLS["gen3"])
+    grid = grids.build_grid(mol, 0)
+    gpu = RKS(mol, level=0, dtype="fp32", grid=grid).run(its=30, conv_tol=1e-7)
+    ref = rks.scf(mol, grid=grid, its=30, conv_tol=1e-9)
+    assert abs(gpu.energy - ref.energy) < 1e-3
+
+
+def test_gpu_becke_grid_scf(hip):
+    """Plain path with device Becke weights equals the host-partitioned grid."""
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS
+    mol = build_mol(METHANE)
+    a = RKS(mol, level=0, dtype="fp64").run(its=30, conv_tol=1e-9)
+    b = RKS(mol, level=0, dtype="fp64", grid=grids.build_grid(mol, 0)).run(its=30, conv_tol=1e-9)
+    assert abs(a.energy - b.energy) < 1e-9
