@@ -315,8 +315,9 @@ int oracle_eri_s8(const int *atm, int natm, const int *bas, int nbas, const doub
     load_shells(atm, bas, nbas, env, sh);
     for (int i = 0; i < nbas; i++)
         if (sh[i].l > 1) { free(sh); return -1; }
-    double buf[81];
-    for (int i = 0; i < nbas; i++)
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = nbas - 1; i >= 0; i--) {
+        double buf[81];
         for (int j = 0; j <= i; j++)
             for (int k = 0; k <= i; k++)
                 for (int l = 0; l <= (k == i ? j : k); l++) {
@@ -339,6 +340,7 @@ int oracle_eri_s8(const int *atm, int natm, const int *bas, int nbas, const doub
                                     eri[tri(hi) + lo] = buf[o];
                                 }
                 }
+    }
     free(sh);
     return 0;
 }

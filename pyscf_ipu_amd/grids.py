@@ -284,7 +284,7 @@ class Grid:
 
     @property
     def size(self) -> int:
-        return len(self.weights)
+        return len(self.coords)
 
 
 def atomic_grids(mol, level: int = 0) -> Grid:

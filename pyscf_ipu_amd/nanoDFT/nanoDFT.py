@@ -87,6 +87,46 @@ def get_JK(density_matrix, ERI, dense_ERI=False, backend="mi355x"):
 
 
 # ------------------------------------------------------------------ driver ---
+class KernelEvents:
+    """HIP-event timing of named launch groups on the current stream.
+
+    Records a start/stop event pair around each group; durations are read
+    after the caller synchronises (`summary()`), so timing adds no host sync."""
+
+    def __init__(self):
+        self.pairs = {}
+        self.meta = {}
+
+    def record(self, name):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        self.pairs.setdefault(name, []).append([a, b])
+        return b
+
+    def summary(self):
+        out = {}
+        for k, v in self.pairs.items():
+            ms = [a.elapsed_time(b) for a, b in v]
+            out[k] = {"calls": len(ms), "total_ms": float(np.sum(ms)), "mean_ms": float(np.mean(ms))}
+        return out
+
+
+class _timed:
+    def __init__(self, ev: Optional[KernelEvents], name: str):
+        self.ev, self.name, self.stop = ev, name, None
+
+    def __enter__(self):
+        if self.ev is not None:
+            self.stop = self.ev.record(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        if self.stop is not None:
+            self.stop.record()
+        return False
+
+
 @dataclass
 class SCFResult:
     energy: float                 # final total energy (Ha)
@@ -106,12 +146,14 @@ class RKS:
     dtype "fp64": every kernel in FP64.  dtype "fp32": ERIs, J/K and XC in FP32;
     SCF step (DIIS, eigh, density) and the energy reduction in FP64."""
 
-    def __init__(self, mol: Mole, level: int = 0, dtype: str = "fp64", grid: Optional[grids.Grid] = None):
+    def __init__(self, mol: Mole, level: int = 0, dtype: str = "fp64", grid: Optional[grids.Grid] = None,
+                 events: Optional["KernelEvents"] = None):
         self.mol = mol
         self.level = level
         self.dtype = dtype
         self.tdt, self.code, _ = device.DTYPES[dtype]
         self.grid = grid
+        self.events = events
         self._setup_done = False
 
     # -- set-up ----------------------------------------------------------------
@@ -152,7 +194,8 @@ class RKS:
         _lib.call("pscf_ao_eval", self.code, device.ptr(atm), mol.natm, device.ptr(bas), mol.nbas, device.ptr(env), N,
                   device.ptr(self.coords), g.size, device.ptr(self.ao), device.stream())
         # ERIs, packed s8
-        self.eri = direct.eri_s8(mol, self.dtype, dev_mol=self.dev_mol)
+        with _timed(self.events, "eri_s8"):
+            self.eri = direct.eri_s8(mol, self.dtype, dev_mol=self.dev_mol)
         # work buffers
         self.D = device.to_dev(guess.minao(mol, S_h) if dm0 is None else dm0, f64)
         self.Dt = self.D.to(self.tdt) if self.tdt != f64 else self.D
@@ -180,17 +223,23 @@ class RKS:
         if self.tdt != torch.float64:
             _lib.call("pscf_cast", _lib.F64, device.ptr(self.D), self.code, device.ptr(self.Dt), N * N,
                       device.stream())
-        _lib.call("pscf_xc", self.code, device.ptr(self.ao), N * self.g, self.g, 1, self.g, N, device.ptr(self.Dt),
-                  device.ptr(self.weights), EPSILON_B3LYP, device.ptr(self.exc), device.ptr(self.Vxc),
-                  device.ptr(self.ws_xc), self.ws_xc_b, device.stream())
-        _lib.call("pscf_jk_s8", self.code, device.ptr(self.eri), N, device.ptr(self.Dt), HYB_B3LYP,
-                  device.ptr(self.dJK), device.ptr(self.ws_jk), self.ws_jk_b, device.stream())
+        with _timed(self.events, "xc"):
+            _lib.call("pscf_xc", self.code, device.ptr(self.ao), N * self.g, self.g, 1, self.g, N,
+                      device.ptr(self.Dt), device.ptr(self.weights), EPSILON_B3LYP, device.ptr(self.exc),
+                      device.ptr(self.Vxc), device.ptr(self.ws_xc), self.ws_xc_b, device.stream())
+        with _timed(self.events, "jk_s8"):
+            _lib.call("pscf_jk_s8", self.code, device.ptr(self.eri), N, device.ptr(self.Dt), HYB_B3LYP,
+                      device.ptr(self.dJK), device.ptr(self.ws_jk), self.ws_jk_b, device.stream())
 
     def energy_terms(self):
         _lib.call("pscf_energy", self.code, self.N, device.ptr(self.D), device.ptr(self.Hcore), device.ptr(self.dJK),
                   device.ptr(self.exc), float(self.E_nuc), device.ptr(self.E), device.stream())
 
     def step(self, it: int, diis: bool = True):
+        with _timed(self.events, "scf_step"):
+            self._step(it, diis)
+
+    def _step(self, it: int, diis: bool = True):
         _lib.call("pscf_scf_step", self.code, it, self.N, self.nocc, device.ptr(self.Hcore), device.ptr(self.S),
                   device.ptr(self.L_inv), device.ptr(self.dJK), device.ptr(self.Vxc), device.ptr(self.D),
                   device.ptr(self.H), device.ptr(self.mo_e), device.ptr(self.C), device.ptr(self.state), int(diis),
