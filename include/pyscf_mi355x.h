@@ -145,6 +145,123 @@ int pscf_scf_step(int dtype, int iteration, int N, int nocc, const double* d_Hco
 int pscf_energy(int dtype, int N, const double* d_D, const double* d_Hcore, const void* d_dJK,
                 const double* d_exc, double e_nuc, double* d_out, void* stream);
 
+/* --------------------------------------------------------- batched form ----
+ * Many molecules of uneven size per launch.  d_mols is an int64 [nmol][16]
+ * descriptor array (natm, nbas, nao, nocc, atm0, bas0, ns, np, shl0, ngrid,
+ * grid0, eri0, mat0, vec0, ao0, state0) giving each molecule's offsets into
+ * the concatenated buffers: global atm/bas/env arrays (bas atom indices and
+ * env pointers absolute), d_shell_ao (each shell's first AO within its
+ * molecule), d_shl_sorted (per molecule: s shells then p shells), packed s8
+ * ERIs at eri0, N x N matrices at mat0, length-N vectors at vec0, AO blocks
+ * (4 x N x ngrid) at ao0 and SCF state at state0.  d_ctl is double [nmol][8]
+ * (iteration, active, converged (-1 = failed set-up), E_prev, E_last, gap,
+ * iterations used, spare); kernels skip molecules whose active flag is 0.
+ * d_order lists molecules largest first (load balance of one workgroup per
+ * molecule).  The per-molecule arithmetic is that of the single-molecule
+ * entry points above; the reference runs one molecule per process
+ * (pyscf_ipu/dft.py:240-245, generate.sh:11-14) through the same
+ * nanoDFT_iteration (nanoDFT.py:40-102).
+ */
+/* ERIs of all molecules: h_work[c] (device, [h_nwork[c]][2] int32 of
+ * (molecule, first quartet)) for the classes (ss|ss), (ps|ss), (ps|ps),
+ * (pp|ss), (pp|ps), (pp|pp); mol.intor("int2e_sph", aosym="s8") per molecule
+ * (nanoDFT.py:285). */
+int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const int* const* h_work, const int* h_nwork,
+                      const int* d_atm, const int* d_bas, const double* d_env, const int* d_shell_ao,
+                      const int* d_shl_sorted, void* d_out, void* stream);
+/* S and H_core = T + V of all molecules (nanoDFT.py:236-238). */
+int pscf_batch_int1e(const void* d_mols, int nmol, const int* d_atm, const int* d_bas, const double* d_env,
+                     const int* d_shell_ao, double* d_S, double* d_Hcore, void* stream);
+/* L^-1 = inv(cholesky(S)) per molecule (nanoDFT.py:239); a non-positive-definite
+ * S marks the molecule failed and inactive (the skip of pyscf_ipu/dft.py:457-464). */
+int pscf_batch_chol(const void* d_mols, int nmol, int max_nao, const double* d_S, double* d_Linv, double* d_ctl,
+                    void* stream);
+/* Atomic-density initial guess per molecule (role of init_guess_by_minao,
+ * nanoDFT.py:233); d_Dt receives the same density in dtype. */
+int pscf_batch_guess(int dtype, const void* d_mols, int nmol, const int* d_atm, const int* d_bas,
+                     const int* d_shell_ao, const double* d_S, double* d_D, void* d_Dt, void* stream);
+/* Grid points of all atoms from per-element templates (Grids.build atom grids,
+ * nanoDFT.py:215-217): d_work [nwork][2] = (atom, first template point). */
+int pscf_batch_grid(const int* d_work, int nwork, const int* d_atom_elem, const int64_t* d_atom_pt0,
+                    const int* d_atom_mol, const int* d_tmpl_off, const int* d_tmpl_n, const double* d_tmpl,
+                    const int* d_atm, const double* d_env, double* d_coords, double* d_vol, int* d_owner,
+                    int* d_ptmol, void* stream);
+/* Becke weights of all points (build_grid.py:34-100); d_w64 optional FP64 copy. */
+int pscf_batch_becke(int dtype, const void* d_mols, const int* d_ptmol, const int* d_owner, const double* d_coords,
+                     const double* d_vol, int64_t G, const int* d_atm, const double* d_env, void* d_w, double* d_w64,
+                     void* stream);
+/* AO values and gradients of all points (eval_gto, nanoDFT.py:219-220). */
+int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, const double* d_coords, int64_t G,
+                  const int* d_atm, const int* d_bas, const double* d_env, void* d_ao, void* stream);
+/* E_xc (d_exc[nmol]) and V_xc of all active molecules (nanoDFT.py:105-123). */
+int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
+                  const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
+                  void* stream);
+/* dJK of all active molecules from their packed s8 ERIs (sparse_symmetric_ERI.py:86-137).
+ * accumulate = 0: d_out (dtype) = dJK(d_dm); accumulate = 1: d_out (FP64) +=
+ * dJK(d_dm), the incremental build from density changes. */
+int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
+                     const void* d_eri, const void* d_dm, double hyb, void* d_out, int accumulate, void* stream);
+/* One SCF step of all active molecules (nanoDFT.py:64-72, 338-393); iteration
+ * index per molecule from d_ctl; dJK in FP64 (written or accumulated by
+ * pscf_batch_jk_s8), V_xc in dtype; writes D (FP64), D in dtype (d_Dt) and,
+ * if d_dDt is not NULL, the change of D in dtype (for the incremental J/K). */
+int pscf_batch_scf_step(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, double* d_ctl,
+                        const double* d_Hcore, const double* d_S, const double* d_Linv, const double* d_dJK,
+                        const void* d_Vxc, double* d_D, void* d_Dt, void* d_dDt, double* d_H, double* d_mo_energy,
+                        double* d_C, double* d_state, int use_diis, void* stream);
+/* Energies (d_E [nmol][5], nanoDFT.py:20-38) and convergence control: init
+ * records E_prev; otherwise a molecule converges when |E - E_prev| < tol and
+ * stops at max_its iterations.  dJK in FP64. */
+int pscf_batch_energy(int dtype, const void* d_mols, int nmol, const double* d_e_nuc, const double* d_D,
+                      const double* d_Hcore, const double* d_dJK, const double* d_exc, const double* d_mo_energy,
+                      double* d_ctl, double* d_E, int init, double tol, int max_its, void* stream);
+
+/* ---- learned initial guess --------------------------------------------------
+ * No counterpart in the reference, whose SCF always starts from PySCF's MINAO
+ * guess (nanoDFT.py:233); these replace that call when a trained network is
+ * given, and MINAO stays the fallback.  Per ordered atom pair an MLP
+ * 43 -> 256 -> 256 -> 256 -> 25 (SiLU) predicts the STO-3G density block. */
+
+/* Symmetrise and purify predicted densities: D <- 2 L^-T V_occ V_occ^T L^-1
+ * with V the nocc eigenvectors of largest occupation of L^T D L. */
+int pscf_batch_purify(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
+                      const double* d_S, const double* d_Linv, double* d_D, void* d_Dt, void* stream);
+/* Pair features: d_pairs [P][2] global atom indices; d_X16 [P][64] BF16 (43
+ * used: one-hot H/C/N/O of a and b, 32 Gaussian RBFs of |R_b - R_a| on 0..5
+ * Angstrom with sigma 0.15, unit direction a->b); optional d_X32 FP32 copy;
+ * d_valid[p] = distance < 5 Angstrom. */
+int pscf_pair_features(int64_t P, const int* d_pairs, const int* d_atm, const double* d_env, void* d_X16,
+                       float* d_X32, unsigned char* d_valid, void* stream);
+/* Training targets from converged densities: d_pinfo [P][5] = (molecule, AO
+ * offset of a, AO count of a, AO offset of b, AO count of b); d_Y, d_mask
+ * [P][32] (block entries r*5+c, masked beyond the atoms' AO counts). */
+int pscf_pair_targets(int64_t P, const int* d_pinfo, const unsigned char* d_valid, const void* d_mols,
+                      const double* d_D, float* d_Y, float* d_mask, void* stream);
+/* Write predicted blocks d_pred [P][32] into d_D (caller zeroes d_D). */
+int pscf_pair_assemble(int64_t P, const int* d_pinfo, const unsigned char* d_valid, const void* d_mols,
+                       const float* d_pred, double* d_D, void* stream);
+/* BF16 MFMA GEMM with fused epilogues (row-major, FP32 accumulation):
+ *   op 0: out = A B + bias (FP32), out2 = silu(out) (BF16)       [hidden layer]
+ *   op 1: out = A B + bias (FP32)                                 [output layer]
+ *   op 2: out = (A B^T) * silu'(aux) (FP32 and BF16 out2)         [backward data]
+ *   op 3: out += A^T B (FP32 atomics; ksplit slices of K)         [weight gradient] */
+int pscf_mlp_gemm(int op, int M, int N, int K, const void* d_A, int lda, const void* d_B, int ldb,
+                  const float* d_bias, const float* d_aux, float* d_out, void* d_out2, int ldc, int ksplit,
+                  void* stream);
+/* Masked mean-squared-error gradient on [B][32] outputs: g = 2 (pred - Y) mask
+ * inv_n (FP32 and BF16); the loss sum is added to *d_loss. */
+int pscf_mlp_mse_grad(int B, const float* d_pred, const float* d_Y, const float* d_mask, float inv_n, float* d_g32,
+                      void* d_g16, double* d_loss, void* stream);
+/* Bias gradient: d_db[n] = sum_m dZ[m][n]. */
+int pscf_mlp_colsum(int M, int N, const float* d_dz, int ld, float* d_db, void* stream);
+/* Adam (bias-corrected, step >= 1) on FP32 master weights; refreshes the BF16
+ * copy d_w16 when given. */
+int pscf_mlp_adam(int64_t n, float* d_w, void* d_w16, const float* d_g, float* d_m, float* d_v, float lr, float b1,
+                  float b2, float eps, int step, void* stream);
+/* FP32 -> BF16 (round to nearest even). */
+int pscf_mlp_to_bf16(int64_t n, const float* d_x, void* d_y, void* stream);
+
 /* pscf_cast: elementwise FP64 <-> FP32 conversion helper (n elements). */
 int pscf_cast(int src_dtype, const void* d_src, int dst_dtype, void* d_dst, int64_t n, void* stream);
 

@@ -1,0 +1,331 @@
+"""Batched RKS B3LYP SCF: thousands of molecules of uneven size per launch.
+
+Every SCF phase is one launch over all molecules (per angular class for the
+ERIs), driven by per-molecule descriptors (csrc/batch.h) into concatenated
+device buffers:
+
+  set-up     grid expansion from element templates -> Becke weights -> AO values
+             -> S and H_core -> Cholesky / L^-1 -> atomic-density guess -> s8 ERIs
+  iteration  SCF step (DIIS + eigh + D) -> fused XC -> s8 J/K -> energy and
+             per-molecule convergence control (converged molecules drop out
+             of every later launch)
+
+One iteration can be captured in a HIP graph and replayed.  The host
+assembly is vectorised numpy over all atoms/shells (no per-molecule Python
+loops beyond input parsing).  Precision: dtype "fp32" runs ERIs, J/K, XC and
+the Becke weights in FP32 with FP64 SCF step, DIIS and energies; "fp64" runs
+everything in FP64.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib, device, grids, gto
+
+DIIS_N = 9
+CTL = 8
+CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+
+
+def state_doubles(N: np.ndarray) -> np.ndarray:
+    return 2 * DIIS_N * N * N + 100 + N * N + 2
+
+
+def class_pairs(X: int, ns: np.ndarray, npp: np.ndarray) -> np.ndarray:
+    return ns * (ns + 1) // 2 if X == 0 else (ns * npp if X == 1 else npp * (npp + 1) // 2)
+
+
+def _chunks(counts: np.ndarray, size: int) -> np.ndarray:
+    """(owner, start) pairs covering range(counts[i]) in steps of `size`."""
+    n = (counts + size - 1) // size
+    owner = np.repeat(np.arange(len(counts)), n)
+    first = np.repeat(np.cumsum(n) - n, n)
+    start = (np.arange(n.sum()) - first) * size
+    return np.stack([owner, start], axis=1).astype(np.int32)
+
+
+def parse(mol_strs: Sequence[str]) -> List[Tuple[np.ndarray, np.ndarray]]:
+    """Atom strings (Angstrom) -> [(Z, R Bohr)]."""
+    out = []
+    for s in mol_strs:
+        atoms = gto.parse_atom_string(s)
+        out.append((np.array([gto.ELEMENTS[a] for a, _ in atoms], dtype=np.int32),
+                    np.array([x for _, x in atoms], dtype=np.float64) / gto.BOHR))
+    return out
+
+
+@dataclass
+class BatchResult:
+    energy: np.ndarray       # (M,) Ha
+    terms: np.ndarray        # (M, 5)
+    iterations: np.ndarray   # (M,)
+    converged: np.ndarray    # (M,) bool
+    failed: np.ndarray       # (M,) bool (overlap not positive definite)
+    hlgap: np.ndarray        # (M,) Ha
+    n_iter_launches: int
+
+
+class BatchRKS:
+    def __init__(self, molecules: Sequence[Tuple[np.ndarray, np.ndarray]], basis: str = "sto-3g", level: int = 0,
+                 dtype: str = "fp32"):
+        self.molecules = molecules
+        self.basis = basis
+        self.level = level
+        self.dtype = dtype
+        self.tdt, self.code, _ = device.DTYPES[dtype]
+        self._assemble()
+
+    # ------------------------------------------------------------ host assembly ---
+    def _assemble(self):
+        mols = self.molecules
+        M = len(mols)
+        Z = np.concatenate([z for z, _ in mols]).astype(np.int32)
+        R = np.concatenate([r for _, r in mols]).astype(np.float64).reshape(-1, 3)
+        natm = np.array([len(z) for z, _ in mols], dtype=np.int64)
+        atom_mol = np.repeat(np.arange(M), natm).astype(np.int32)
+        A = len(Z)
+        # element templates: shells and grids
+        elems = sorted(set(Z.tolist()))
+        slot = {z: i for i, z in enumerate(elems)}
+        env = [0.0] * gto.PTR_ENV_START
+        tmpl_rows, tmpl_cnt = [], []
+        for z in elems:
+            rows = []
+            for l, exps, coefs in gto.element_shells(gto.SYMBOLS[z], self.basis):
+                pe = len(env)
+                env.extend(exps.tolist())
+                pc = len(env)
+                env.extend(gto.normalized_coefficients(l, exps, coefs).tolist())
+                rows.append([0, l, len(exps), 1, 0, pe, pc, 0])
+            tmpl_rows.append(rows)
+            tmpl_cnt.append(len(rows))
+        maxsh = max(tmpl_cnt)
+        tmpl = np.zeros((len(elems), maxsh, 8), dtype=np.int32)
+        for i, rows in enumerate(tmpl_rows):
+            tmpl[i, :len(rows)] = rows
+        coord0 = len(env)
+        env = np.concatenate([np.array(env), R.reshape(-1)])
+        eslot = np.array([slot[z] for z in Z.tolist()], dtype=np.int32) if A else np.zeros(0, np.int32)
+        atm = np.zeros((A, 6), dtype=np.int32)
+        atm[:, 0] = Z
+        atm[:, 1] = coord0 + 3 * np.arange(A)
+        atm[:, 2] = 1
+        nsh_atom = np.array(tmpl_cnt, dtype=np.int64)[eslot]
+        sh_atom = np.repeat(np.arange(A), nsh_atom)
+        sh_first = np.repeat(np.cumsum(nsh_atom) - nsh_atom, nsh_atom)
+        sh_in_atom = np.arange(len(sh_atom)) - sh_first
+        bas = tmpl[eslot[sh_atom], sh_in_atom].copy()
+        bas[:, 0] = sh_atom
+        l = bas[:, 1].astype(np.int64)
+        width = 2 * l + 1
+        sh_mol = atom_mol[sh_atom]
+        nbas = np.bincount(sh_mol, minlength=M).astype(np.int64)
+        nao = np.bincount(sh_mol, weights=width, minlength=M).astype(np.int64)
+        bas0 = np.cumsum(nbas) - nbas
+        atm0 = np.cumsum(natm) - natm
+        ao_start = np.cumsum(nao) - nao
+        shell_ao = (np.cumsum(width) - width - ao_start[sh_mol]).astype(np.int32)
+        nelec = np.bincount(atom_mol, weights=Z, minlength=M).astype(np.int64)
+        if np.any(nelec % 2):
+            raise ValueError("closed-shell molecules only (even electron counts)")
+        nocc = nelec // 2
+        order_sh = np.lexsort((np.arange(len(l)), l, sh_mol))
+        ns = np.bincount(sh_mol, weights=(l == 0), minlength=M).astype(np.int64)
+        npp = np.bincount(sh_mol, weights=(l == 1), minlength=M).astype(np.int64)
+        if np.any(l > 1):
+            raise ValueError("only s and p shells are supported")
+        P = nao * (nao + 1) // 2
+        n_s8 = P * (P + 1) // 2
+        eri0 = np.cumsum(n_s8) - n_s8
+        mat0 = np.cumsum(nao * nao) - nao * nao
+        vec0 = np.cumsum(nao) - nao
+        st = state_doubles(nao)
+        state0 = np.cumsum(st) - st
+        # grid from element templates
+        tm, toff, tn = [], [], []
+        off = 0
+        for z in elems:
+            c, v = grids.atomic_grid(int(z), self.level)
+            tm.append(np.hstack([c, v[:, None]]))
+            toff.append(off)
+            tn.append(len(v))
+            off += len(v)
+        self.tmpl_grid = np.vstack(tm)
+        self.tmpl_off = np.array(toff, dtype=np.int32)
+        self.tmpl_n = np.array(tn, dtype=np.int32)
+        apts = self.tmpl_n[eslot].astype(np.int64)
+        atom_pt0 = np.cumsum(apts) - apts
+        ngrid = np.bincount(atom_mol, weights=apts, minlength=M).astype(np.int64)
+        grid0 = np.cumsum(ngrid) - ngrid
+        ao_sz = 4 * nao * ngrid
+        ao0 = np.cumsum(ao_sz) - ao_sz
+        desc = np.stack([natm, nbas, nao, nocc, atm0, bas0, ns, npp, bas0, ngrid, grid0, eri0, mat0, vec0, ao0, state0],
+                        axis=1).astype(np.int64)
+        self.M, self.A = M, A
+        self.nao, self.ngrid, self.natm, self.nocc = nao, ngrid, natm, nocc
+        self.atm, self.bas, self.env = atm, bas, env
+        self.shell_ao, self.shl_sorted = shell_ao, order_sh.astype(np.int32)
+        # first AO of each atom within its molecule, and its AO count
+        self.atom_ao0 = shell_ao[np.cumsum(nsh_atom) - nsh_atom].astype(np.int32) if A else np.zeros(0, np.int32)
+        self.atom_nao = np.bincount(sh_atom, weights=width, minlength=A).astype(np.int32)
+        self.atom_elem, self.atom_pt0, self.atom_mol = eslot, atom_pt0, atom_mol
+        self.desc = desc
+        self.sizes = {"eri": int(n_s8.sum()), "mat": int((nao * nao).sum()), "vec": int(nao.sum()),
+                      "ao": int(ao_sz.sum()), "state": int(st.sum()), "grid": int(ngrid.sum())}
+        self.grid_work = _chunks(apts, 256)
+        self.eri_work = []
+        for X, Y in CLASSES:
+            nx, ny = class_pairs(X, ns, npp), class_pairs(Y, ns, npp)
+            nq = nx * (nx + 1) // 2 if X == Y else nx * ny
+            self.eri_work.append(_chunks(nq, 256))
+        self.order = np.argsort(-nao, kind="stable").astype(np.int32)
+        self.max_nao = int(nao.max())
+        # nuclear repulsion, vectorised over molecules of equal atom count
+        e_nuc = np.zeros(M)
+        for na in np.unique(natm):
+            ms = np.where(natm == na)[0]
+            if na < 2:
+                continue
+            idx = atm0[ms][:, None] + np.arange(na)[None, :]
+            r, z = R[idx], Z[idx].astype(np.float64)           # (k, na, 3), (k, na)
+            d = np.linalg.norm(r[:, :, None] - r[:, None, :], axis=-1)
+            iu = np.triu_indices(na, 1)
+            e_nuc[ms] = np.sum((z[:, :, None] * z[:, None, :])[:, iu[0], iu[1]] / d[:, iu[0], iu[1]], axis=1)
+        self.e_nuc = e_nuc
+
+    # ------------------------------------------------------------ device set-up ---
+    def setup(self, dm0: torch.Tensor = None, guess=None):
+        """Device set-up.  The initial density is `dm0` (concatenated FP64
+        matrices) if given, else what `guess(self)` writes into self.D and
+        self.Dt (called once S, L^-1 and ctl exist), else the atomic guess."""
+        dev = device.require_gpu()
+        f64, T = torch.float64, self.tdt
+        S = self.sizes
+        self.d_desc = device.to_dev(self.desc)
+        self.d_atm, self.d_bas, self.d_env = (device.to_dev(self.atm), device.to_dev(self.bas), device.to_dev(self.env))
+        self.d_shell_ao, self.d_shl = device.to_dev(self.shell_ao), device.to_dev(self.shl_sorted)
+        self.d_order = device.to_dev(self.order)
+        self.d_enuc = device.to_dev(self.e_nuc)
+        # grid
+        G = S["grid"]
+        self.G = G
+        self.coords = device.empty((G, 3), f64)
+        self.vol = device.empty(G, f64)
+        self.owner = device.empty(G, torch.int32)
+        self.ptmol = device.empty(G, torch.int32)
+        gw = device.to_dev(self.grid_work)
+        ae, ap0, am = (device.to_dev(self.atom_elem), device.to_dev(self.atom_pt0), device.to_dev(self.atom_mol))
+        toff, tn, tg = device.to_dev(self.tmpl_off), device.to_dev(self.tmpl_n), device.to_dev(self.tmpl_grid)
+        P = device.ptr
+        st = device.stream()
+        _lib.call("pscf_batch_grid", P(gw), len(self.grid_work), P(ae), P(ap0), P(am), P(toff), P(tn), P(tg),
+                  P(self.d_atm), P(self.d_env), P(self.coords), P(self.vol), P(self.owner), P(self.ptmol), st)
+        self.w = device.empty(G, T)
+        _lib.call("pscf_batch_becke", self.code, P(self.d_desc), P(self.ptmol), P(self.owner), P(self.coords),
+                  P(self.vol), G, P(self.d_atm), P(self.d_env), P(self.w), None, st)
+        self.ao = device.empty(S["ao"], T)
+        _lib.call("pscf_batch_ao", self.code, P(self.d_desc), P(self.ptmol), P(self.coords), G, P(self.d_atm),
+                  P(self.d_bas), P(self.d_env), P(self.ao), st)
+        # one-electron part, orthogonaliser, guess
+        nm = S["mat"]
+        self.S, self.Hc, self.Linv = device.empty(nm, f64), device.empty(nm, f64), device.empty(nm, f64)
+        self.ctl = device.zeros((self.M, CTL), f64)
+        self.ctl[:, 1] = 1.0
+        _lib.call("pscf_batch_int1e", P(self.d_desc), self.M, P(self.d_atm), P(self.d_bas), P(self.d_env),
+                  P(self.d_shell_ao), P(self.S), P(self.Hc), st)
+        _lib.call("pscf_batch_chol", P(self.d_desc), self.M, self.max_nao, P(self.S), P(self.Linv), P(self.ctl), st)
+        self.D = device.empty(nm, f64)
+        self.Dt = device.empty(nm, T)
+        if guess is not None:
+            guess(self)
+        elif dm0 is None:
+            _lib.call("pscf_batch_guess", self.code, P(self.d_desc), self.M, P(self.d_atm), P(self.d_bas),
+                      P(self.d_shell_ao), P(self.S), P(self.D), P(self.Dt), st)
+        else:
+            self.D.copy_(dm0)
+            self.Dt.copy_(dm0)
+        # ERIs
+        self.eri = device.empty(S["eri"], T)
+        works = [device.to_dev(w) for w in self.eri_work]
+        arr = (ctypes.c_void_p * 6)(*[w.data_ptr() for w in works])
+        cnt = (ctypes.c_int * 6)(*[len(w) for w in self.eri_work])
+        _lib.call("pscf_batch_eri_s8", self.code, P(self.d_desc), self.M, arr, cnt, P(self.d_atm), P(self.d_bas),
+                  P(self.d_env), P(self.d_shell_ao), P(self.d_shl), P(self.eri), st)
+        # iteration buffers.  dJK is FP64; in FP32 mode it is built
+        # incrementally, dJK += JK(D_new - D_old) with the change in FP32, so the
+        # FP32 rounding shrinks with the density change and the energy can
+        # converge below the FP32 noise of a full build.
+        self.incremental = T != f64
+        self.dJK, self.Vxc = device.zeros(nm, f64), device.empty(nm, T)
+        self.dDt = device.empty(nm, T) if self.incremental else None
+        self.exc = device.empty(self.M, f64)
+        self.H, self.C = device.empty(nm, f64), device.empty(nm, f64)
+        self.mo_e = device.empty(S["vec"], f64)
+        self.E = device.empty((self.M, 5), f64)
+        self.state = device.zeros(S["state"], f64)
+        self._keep = [gw, ae, ap0, am, toff, tn, tg] + works
+        torch.cuda.synchronize()
+
+    # ---------------------------------------------------------------- iterations ---
+    def _fock(self, ctl_ptr, first=False):
+        P, st = device.ptr, device.stream()
+        _lib.call("pscf_batch_xc", self.code, P(self.d_desc), P(self.d_order), self.M, self.max_nao, ctl_ptr,
+                  P(self.ao), P(self.Dt), P(self.w), 1e-20, P(self.Vxc), P(self.exc), st)
+        dm = self.Dt if (first or not self.incremental) else self.dDt
+        _lib.call("pscf_batch_jk_s8", self.code, P(self.d_desc), P(self.d_order), self.M, self.max_nao, ctl_ptr,
+                  P(self.eri), P(dm), 0.2, P(self.dJK), int(self.incremental), st)
+
+    def _energy(self, init, tol, max_its):
+        P, st = device.ptr, device.stream()
+        _lib.call("pscf_batch_energy", self.code, P(self.d_desc), self.M, P(self.d_enuc), P(self.D), P(self.Hc),
+                  P(self.dJK), P(self.exc), P(self.mo_e), P(self.ctl), P(self.E), int(init), float(tol), int(max_its),
+                  st)
+
+    def _iteration(self, tol, max_its, diis=True):
+        P, st = device.ptr, device.stream()
+        _lib.call("pscf_batch_scf_step", self.code, P(self.d_desc), P(self.d_order), self.M, self.max_nao, P(self.ctl),
+                  P(self.Hc), P(self.S), P(self.Linv), P(self.dJK), P(self.Vxc), P(self.D), P(self.Dt), P(self.dDt),
+                  P(self.H), P(self.mo_e), P(self.C), P(self.state), int(diis), st)
+        self._fock(P(self.ctl))
+        self._energy(False, tol, max_its)
+
+    def run(self, max_its: int = 20, conv_tol: float = 1e-8, graph: bool = True, check_every: int = 2,
+            diis: bool = True) -> BatchResult:
+        if not hasattr(self, "eri"):
+            self.setup()
+        self._fock(device.ptr(self.ctl), first=True)
+        self._energy(True, conv_tol, max_its)
+        launches = 0
+        g = None
+        if graph:
+            # first iteration eagerly (sets kernel attributes), then capture one iteration
+            self._iteration(conv_tol, max_its, diis)
+            launches += 1
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._iteration(conv_tol, max_its, diis)
+        while launches < max_its:
+            if g is not None:
+                g.replay()
+            else:
+                self._iteration(conv_tol, max_its, diis)
+            launches += 1
+            if launches % check_every == 0 or launches == max_its:
+                if float(self.ctl[:, 1].sum().item()) == 0.0:
+                    break
+        torch.cuda.synchronize()
+        ctl = self.ctl.cpu().numpy()
+        E = self.E.cpu().numpy()
+        return BatchResult(E[:, 0].copy(), E, ctl[:, 6].astype(int), ctl[:, 2] > 0, ctl[:, 2] < 0, ctl[:, 5].copy(),
+                           launches)
+
+    # ---------------------------------------------------------------- accessors ---
+    def matrix(self, buf: torch.Tensor, m: int) -> np.ndarray:
+        N, o = int(self.nao[m]), int(self.desc[m, 12])
+        return buf[o:o + N * N].reshape(N, N).cpu().numpy()

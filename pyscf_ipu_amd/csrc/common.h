@@ -15,6 +15,20 @@ int check_hip(hipError_t e, const char* what);
     hipError_t _e = (call);                                         \
     if (_e != hipSuccess) return ::pscf::check_hip(_e, #call);      \
   } while (0)
+// Raise a kernel's dynamic-LDS limit once per process (not a stream operation,
+// kept out of graph-captured call sequences after the first call).
+#define PSCF_SET_LDS(kernel, bytes)                                                                        \
+  do {                                                                                                     \
+    static int _pscf_lds_set = 0;                                                                          \
+    if ((int)(bytes) > 65536 && (int)(bytes) > _pscf_lds_set) {                                            \
+      PSCF_HIP(hipFuncSetAttribute((const void*)(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                   (int)(bytes)));                                                         \
+      _pscf_lds_set = (int)(bytes);                                                                        \
+    }                                                                                                      \
+  } while (0)
+
+int device_max_lds();
+
 #define PSCF_REQUIRE(cond, ...)                                     \
   do {                                                              \
     if (!(cond)) {                                                  \

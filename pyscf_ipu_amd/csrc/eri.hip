@@ -18,6 +18,7 @@
 #include <mutex>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 
 namespace pscf {
@@ -409,7 +410,138 @@ __global__ void __launch_bounds__(256) eri_quartets_kernel(const int* __restrict
 }
 
 // ------------------------------------------------------------------- int1e ---
-// One lane per shell pair (i >= j): S, T (Obara-Saika) and V (Rys, all nuclei).
+// S, T (Obara-Saika) and V (Rys over nuclei atom0..atom0+natm-1) blocks of one
+// shell pair computed by one 64-lane wave: lanes take (primitive pair, nucleus)
+// combinations, a fixed shuffle tree sums them, lane 0 writes both triangles
+// (row stride nao).  hcore: write T + V into Tk instead of T and V.
+__device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, int natm, const int* __restrict__ bas,
+                                       const double* __restrict__ env, const double* __restrict__ tab, int ia, int ib,
+                                       int ao_a, int ao_b, int nao, double* __restrict__ S, double* __restrict__ Tk,
+                                       double* __restrict__ V, bool hcore) {
+  const int lane = threadIdx.x & 63;
+  const int* ba = bas + ia * BAS_SLOTS;
+  const int* bb = bas + ib * BAS_SLOTS;
+  const double* RA = env + atm[ba[0] * ATM_SLOTS + 1];
+  const double* RB = env + atm[bb[0] * ATM_SLOTS + 1];
+  const int la = ba[1], lb = bb[1], npa = ba[2], npb = bb[2];
+  const double* ea = env + ba[5];
+  const double* ca = env + ba[6];
+  const double* eb = env + bb[5];
+  const double* cb = env + bb[6];
+  const int na = la ? 3 : 1, nb = lb ? 3 : 1;
+  const double fac = (la ? ANG_P : ANG_S) * (lb ? ANG_P : ANG_S);
+  double s_acc[9] = {0}, t_acc[9] = {0}, v_acc[9] = {0};
+  double AB[3] = {RA[0] - RB[0], RA[1] - RB[1], RA[2] - RB[2]};
+  const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+  const int ncombo = npa * npb * natm;
+  for (int cidx = lane; cidx < ncombo; cidx += 64) {
+    const int iat = cidx % natm, pp = cidx / natm;
+    const int pa = pp / npb, pb = pp % npb;
+    const double a = ea[pa], b = eb[pb], p = a + b, rp = 1.0 / p, mu = a * b * rp;
+    const double cc = ca[pa] * cb[pb] * fac;
+    const double ex = exp(-mu * rab2);
+    double PA[3], PB[3], P[3];
+    for (int d = 0; d < 3; ++d) {
+      P[d] = (a * RA[d] + b * RB[d]) * rp;
+      PA[d] = P[d] - RA[d];
+      PB[d] = P[d] - RB[d];
+    }
+    if (iat == 0) {  // overlap and kinetic once per primitive pair
+      const double s00 = pow(PI * rp, 1.5) * ex;
+      const double h = 0.5 * rp;
+      double s1[3][2][4];
+      for (int d = 0; d < 3; ++d) {
+        s1[d][0][0] = 1.0;
+        s1[d][1][0] = PA[d];
+        for (int j = 0; j < 3; ++j)
+          for (int i = 0; i <= 1; ++i)
+            s1[d][i][j + 1] = PB[d] * s1[d][i][j] + h * ((i ? s1[d][i - 1][j] : 0.0) + (j ? j * s1[d][i][j - 1] : 0.0));
+      }
+      for (int fa = 0; fa < na; ++fa)
+        for (int fb = 0; fb < nb; ++fb) {
+          int e1[3], e2[3];
+          for (int d = 0; d < 3; ++d) {
+            e1[d] = la && fa == d;
+            e2[d] = lb && fb == d;
+          }
+          double tt = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            const int i = e1[d], j = e2[d];
+            const double t1 = b * (2 * j + 1) * s1[d][i][j] - 2.0 * b * b * s1[d][i][j + 2];
+            double o = 1.0;
+            for (int q = 0; q < 3; ++q)
+              if (q != d) o *= s1[q][e1[q]][e2[q]];
+            tt += t1 * o;
+          }
+          s_acc[fa * 3 + fb] += cc * s00 * s1[0][e1[0]][e2[0]] * s1[1][e1[1]][e2[1]] * s1[2][e1[2]][e2[2]];
+          t_acc[fa * 3 + fb] += cc * s00 * tt;
+        }
+    }
+    const int c = atom0 + iat;
+    const double* Cn = env + atm[c * ATM_SLOTS + 1];
+    const double Z = (double)atm[c * ATM_SLOTS];
+    double PC[3] = {P[0] - Cn[0], P[1] - Cn[1], P[2] - Cn[2]};
+    const double tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+    const int nr = (la + lb) / 2 + 1;
+    double u[2], w[2];
+    if (nr == 1) {
+      double f[2];
+      boys<double, 1, BOYS_TAYLOR>(tn, tab, f);
+      rys_from_moments<1>(f, u, w);
+    } else {
+      double f[4];
+      boys<double, 3, BOYS_TAYLOR>(tn, tab, f);
+      rys_from_moments<2>(f, u, w);
+    }
+    const double vpre = 2.0 * PI * rp * ex * cc;
+    for (int r = 0; r < nr; ++r) {
+      double J[3][2][2];
+      const double hr = 0.5 * (1.0 - u[r]) * rp;
+      for (int d = 0; d < 3; ++d) {
+        const double pa_ = PA[d] - u[r] * PC[d], pb_ = PB[d] - u[r] * PC[d];
+        J[d][0][0] = 1.0;
+        J[d][1][0] = pa_;
+        J[d][0][1] = pb_;
+        J[d][1][1] = pb_ * pa_ + hr;
+      }
+      for (int fa = 0; fa < na; ++fa)
+        for (int fb = 0; fb < nb; ++fb) {
+          double v = -Z * w[r] * vpre;
+          for (int d = 0; d < 3; ++d) v *= J[d][la && fa == d][lb && fb == d];
+          v_acc[fa * 3 + fb] += v;
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      s_acc[k] += __shfl_xor(s_acc[k], off, 64);
+      t_acc[k] += __shfl_xor(t_acc[k], off, 64);
+      v_acc[k] += __shfl_xor(v_acc[k], off, 64);
+    }
+  if (lane == 0)
+    for (int fa = 0; fa < na; ++fa)
+      for (int fb = 0; fb < nb; ++fb) {
+        const int64_t r = ao_a + fa, c = ao_b + fb;
+        S[r * nao + c] = S[c * nao + r] = s_acc[fa * 3 + fb];
+        if (hcore) {
+          Tk[r * nao + c] = Tk[c * nao + r] = t_acc[fa * 3 + fb] + v_acc[fa * 3 + fb];
+        } else {
+          Tk[r * nao + c] = Tk[c * nao + r] = t_acc[fa * 3 + fb];
+          V[r * nao + c] = V[c * nao + r] = v_acc[fa * 3 + fb];
+        }
+      }
+}
+
+__device__ inline int64_t tri_root(int64_t q) {
+  int64_t ia = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while (tri(ia + 1) <= q) ++ia;
+  while (tri(ia) > q) --ia;
+  return ia;
+}
+
+// One wave per shell pair (i >= j).
 __global__ void __launch_bounds__(256) int1e_kernel(const int* __restrict__ atm, int natm,
                                                     const int* __restrict__ bas, int nbas,
                                                     const double* __restrict__ env, int nao, double* __restrict__ S,
@@ -420,107 +552,119 @@ __global__ void __launch_bounds__(256) int1e_kernel(const int* __restrict__ atm,
   for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) tab[i] = g_boys[i];
   stage_ao_loc(ao_loc, bas, nbas);
   __syncthreads();
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (q >= tri((int64_t)nbas)) return;
-  int64_t ia = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
-  while (tri(ia + 1) <= q) ++ia;
-  while (tri(ia) > q) --ia;
+  const int64_t ia = tri_root(q);
   const int ib = (int)(q - tri(ia));
-  const ShellD A = load_shell(atm, bas, env, ao_loc, (int)ia), B = load_shell(atm, bas, env, ao_loc, ib);
-  const int la = bas[ia * BAS_SLOTS + 1], lb = bas[ib * BAS_SLOTS + 1];
-  const int na = la ? 3 : 1, nb = lb ? 3 : 1;
-  const double fac = (la ? ANG_P : ANG_S) * (lb ? ANG_P : ANG_S);
-  double s_acc[9] = {0}, t_acc[9] = {0}, v_acc[9] = {0};
-  double AB[3] = {A.r[0] - B.r[0], A.r[1] - B.r[1], A.r[2] - B.r[2]};
-  const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
-  for (int pa = 0; pa < A.np; ++pa)
-    for (int pb = 0; pb < B.np; ++pb) {
-      const double a = A.e[pa], b = B.e[pb], p = a + b, rp = 1.0 / p, mu = a * b * rp;
-      const double cc = A.c[pa] * B.c[pb] * fac;
-      const double ex = exp(-mu * rab2);
-      const double s00 = pow(PI * rp, 1.5) * ex;
-      double PA[3], PB[3], P[3];
-      for (int d = 0; d < 3; ++d) {
-        P[d] = (a * A.r[d] + b * B.r[d]) * rp;
-        PA[d] = P[d] - A.r[d];
-        PB[d] = P[d] - B.r[d];
-      }
-      const double h = 0.5 * rp;
-      // 1D overlaps s[d][i][j], i <= 1, j <= 3
-      double s1[3][2][4];
-      for (int d = 0; d < 3; ++d) {
-        s1[d][0][0] = 1.0;
-        s1[d][1][0] = PA[d];
-        for (int j = 0; j < 3; ++j)
-          for (int i = 0; i <= 1; ++i)
-            s1[d][i][j + 1] = PB[d] * s1[d][i][j] + h * ((i ? s1[d][i - 1][j] : 0.0) + (j ? j * s1[d][i][j - 1] : 0.0));
-      }
-      // nuclear attraction
-      double vb[3][3] = {{0}};
-      for (int c = 0; c < natm; ++c) {
-        const double* Cn = env + atm[c * ATM_SLOTS + 1];
-        const double Z = (double)atm[c * ATM_SLOTS];
-        double PC[3] = {P[0] - Cn[0], P[1] - Cn[1], P[2] - Cn[2]};
-        const double tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-        const int nr = (la + lb) / 2 + 1;
-        double u[2], w[2];
-        if (nr == 1) {
-          double f[2];
-          boys<double, 1, BOYS_TAYLOR>(tn, tab, f);
-          rys_from_moments<1>(f, u, w);
-        } else {
-          double f[4];
-          boys<double, 3, BOYS_TAYLOR>(tn, tab, f);
-          rys_from_moments<2>(f, u, w);
+  int1e_pair_wave(atm, 0, natm, bas, env, tab, (int)ia, ib, ao_loc[ia], ao_loc[ib], nao, S, Tk, V, false);
+}
+
+// ------------------------------------------------------------ batched form ---
+// Shell pair a of class X from a molecule's class-sorted shell list
+// (s shells then p shells): ss = tri over s, ps = (p, s) rectangle, pp = tri over p.
+template <int X>
+__device__ inline void pair_shells(int64_t a, int ns, int np, const int* __restrict__ sl, int& i, int& j) {
+  if (X == 1) {
+    i = sl[ns + (int)(a / ns)];
+    j = sl[(int)(a % ns)];
+  } else {
+    int64_t t = (int64_t)((sqrt(8.0 * (double)a + 1.0) - 1.0) * 0.5);
+    while (tri(t + 1) <= a) ++t;
+    while (tri(t) > a) --t;
+    const int base = X == 0 ? 0 : ns;
+    i = sl[base + (int)t];
+    j = sl[base + (int)(a - tri(t))];
+  }
+}
+
+__host__ __device__ inline int64_t class_pairs(int X, int64_t ns, int64_t np) {
+  return X == 0 ? tri(ns) : (X == 1 ? ns * np : tri(np));
+}
+
+// One chunk of <= 256 quartets of class (X, Y) of one molecule per workgroup.
+template <typename T, int X, int Y>
+__global__ void __launch_bounds__(256) eri_s8_batch_kernel(const BatchMol* __restrict__ mols,
+                                                           const int* __restrict__ work,
+                                                           const int* __restrict__ atm, const int* __restrict__ bas,
+                                                           const double* __restrict__ env,
+                                                           const int* __restrict__ shell_ao,
+                                                           const int* __restrict__ shl_sorted, T* __restrict__ out) {
+  constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
+  constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
+  using BT = typename BoysType<T, NR>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  BT* tab = reinterpret_cast<BT*>(smem);
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) tab[i] = (BT)g_boys[i];
+  __syncthreads();
+  const int m = work[2 * blockIdx.x];
+  const int64_t q = (int64_t)work[2 * blockIdx.x + 1] + threadIdx.x;
+  const BatchMol M = mols[m];
+  const int ns = (int)M.ns, np = (int)M.np;
+  const int64_t nx = class_pairs(X, ns, np), ny = class_pairs(Y, ns, np);
+  const bool tri_launch = (X == Y);
+  const int64_t total = tri_launch ? tri(nx) : nx * ny;
+  if (q >= total) return;
+  int64_t a, b;
+  if (tri_launch) {
+    a = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+    while (tri(a + 1) <= q) ++a;
+    while (tri(a) > q) --a;
+    b = q - tri(a);
+  } else {
+    a = q / ny;
+    b = q - a * ny;
+  }
+  const int* sl = shl_sorted + M.shl0;
+  int i, j, k, l;
+  pair_shells<X>(a, ns, np, sl, i, j);
+  pair_shells<Y>(b, ns, np, sl, k, l);
+  // shell_ao holds each shell's first AO within its molecule
+  const ShellD A = load_shell(atm, bas, env, shell_ao, i), B = load_shell(atm, bas, env, shell_ao, j);
+  const ShellD C = load_shell(atm, bas, env, shell_ao, k), D = load_shell(atm, bas, env, shell_ao, l);
+  constexpr int NI = NComp<LI>::v, NJ = NComp<LJ>::v, NK = NComp<LK>::v, NL = NComp<LL>::v;
+  T acc[NI * NJ * NK * NL];
+  shell_quartet<T, BT, BoysTerms<BT>::v, LI, LJ, LK, LL>(A, B, C, D, tab, acc);
+  const bool diag_bra = (i == j), diag_ket = (k == l), same_pair = tri_launch && (a == b);
+  T* dst = out + M.eri0;
+#pragma unroll
+  for (int fi = 0; fi < NI; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < NJ; ++fj)
+#pragma unroll
+      for (int fk = 0; fk < NK; ++fk)
+#pragma unroll
+        for (int fl = 0; fl < NL; ++fl) {
+          const int64_t p = A.ao0 + fi, qq = B.ao0 + fj, r = C.ao0 + fk, s = D.ao0 + fl;
+          if (diag_bra && qq > p) continue;
+          if (diag_ket && s > r) continue;
+          const int64_t pq = p >= qq ? tri(p) + qq : tri(qq) + p;
+          const int64_t rs = r >= s ? tri(r) + s : tri(s) + r;
+          if (same_pair && rs > pq) continue;
+          const int64_t idx = pq >= rs ? tri(pq) + rs : tri(rs) + pq;
+          dst[idx] = acc[((fi * NJ + fj) * NK + fk) * NL + fl];
         }
-        for (int r = 0; r < nr; ++r) {
-          double J[3][2][2];
-          const double hr = 0.5 * (1.0 - u[r]) * rp;
-          for (int d = 0; d < 3; ++d) {
-            const double pa_ = PA[d] - u[r] * PC[d], pb_ = PB[d] - u[r] * PC[d];
-            J[d][0][0] = 1.0;
-            J[d][1][0] = pa_;
-            J[d][0][1] = pb_;
-            J[d][1][1] = pb_ * pa_ + hr;
-          }
-          for (int fa = 0; fa < na; ++fa)
-            for (int fb = 0; fb < nb; ++fb) {
-              double v = -Z * w[r];
-              for (int d = 0; d < 3; ++d) v *= J[d][la && fa == d][lb && fb == d];
-              vb[fa][fb] += v;
-            }
-        }
-      }
-      const double vpre = 2.0 * PI * rp * ex;
-      for (int fa = 0; fa < na; ++fa)
-        for (int fb = 0; fb < nb; ++fb) {
-          int ea[3], eb[3];
-          for (int d = 0; d < 3; ++d) {
-            ea[d] = la && fa == d;
-            eb[d] = lb && fb == d;
-          }
-          double sx = s1[0][ea[0]][eb[0]], sy = s1[1][ea[1]][eb[1]], sz = s1[2][ea[2]][eb[2]];
-          double tt = 0.0;
-          for (int d = 0; d < 3; ++d) {
-            const int i = ea[d], j = eb[d];
-            double t1 = b * (2 * j + 1) * s1[d][i][j] - 2.0 * b * b * s1[d][i][j + 2];
-            double o = 1.0;
-            for (int e2 = 0; e2 < 3; ++e2)
-              if (e2 != d) o *= s1[e2][ea[e2]][eb[e2]];
-            tt += t1 * o;
-          }
-          s_acc[fa * 3 + fb] += cc * s00 * sx * sy * sz;
-          t_acc[fa * 3 + fb] += cc * s00 * tt;
-          v_acc[fa * 3 + fb] += cc * vpre * vb[fa][fb];
-        }
-    }
-  for (int fa = 0; fa < na; ++fa)
-    for (int fb = 0; fb < nb; ++fb) {
-      const int64_t r = A.ao0 + fa, c = B.ao0 + fb;
-      S[r * nao + c] = S[c * nao + r] = s_acc[fa * 3 + fb];
-      Tk[r * nao + c] = Tk[c * nao + r] = t_acc[fa * 3 + fb];
-      V[r * nao + c] = V[c * nao + r] = v_acc[fa * 3 + fb];
-    }
+}
+
+// One workgroup per molecule, one wave per shell pair: S and H_core = T + V.
+__global__ void __launch_bounds__(256) int1e_batch_kernel(const BatchMol* __restrict__ mols,
+                                                          const int* __restrict__ atm, const int* __restrict__ bas,
+                                                          const double* __restrict__ env,
+                                                          const int* __restrict__ shell_ao, double* __restrict__ S,
+                                                          double* __restrict__ H) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* tab = reinterpret_cast<double*>(smem);
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) tab[i] = g_boys[i];
+  __syncthreads();
+  const BatchMol M = mols[blockIdx.x];
+  const int nb = (int)M.nbas, b0 = (int)M.bas0, N = (int)M.nao;
+  const int nw = blockDim.x >> 6;
+  for (int64_t q = threadIdx.x >> 6; q < tri((int64_t)nb); q += nw) {
+    const int64_t ia = tri_root(q);
+    const int ib = (int)(q - tri(ia));
+    const int ga = b0 + (int)ia, gb = b0 + ib;
+    int1e_pair_wave(atm, (int)M.atm0, (int)M.natm, bas, env, tab, ga, gb, shell_ao[ga], shell_ao[gb], N,
+                    S + M.mat0, H + M.mat0, nullptr, true);
+  }
 }
 
 template <typename T, int X, int Y>
@@ -588,12 +732,8 @@ extern "C" int pscf_eri_quartets(int dtype, const int* d_atm, int natm, const in
   hipStream_t st = (hipStream_t)stream;
   const size_t shm = (sizeof(double) + sizeof(float)) * BOYS_NT * BOYS_NM + sizeof(int) * (nbas + 1);
   const int blocks = (nq + 255) / 256;
-  if (shm > 65536) {
-    PSCF_HIP(hipFuncSetAttribute((const void*)eri_quartets_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
-    PSCF_HIP(hipFuncSetAttribute((const void*)eri_quartets_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
-  }
+  PSCF_SET_LDS(eri_quartets_kernel<double>, shm);
+  PSCF_SET_LDS(eri_quartets_kernel<float>, shm);
   if (dtype == PSCF_F64)
     eri_quartets_kernel<double><<<blocks, 256, shm, st>>>(d_atm, d_bas, d_env, nbas, d_quartets, d_offsets, nq,
                                                           (double*)d_out);
@@ -613,8 +753,64 @@ extern "C" int pscf_int1e(const int* d_atm, int natm, const int* d_bas, int nbas
   hipStream_t st = (hipStream_t)stream;
   const int64_t npair = tri((int64_t)nbas);
   const size_t shm = sizeof(double) * BOYS_NT * BOYS_NM + sizeof(int) * (nbas + 1);
-  int1e_kernel<<<(unsigned)((npair + 255) / 256), 256, shm, st>>>(d_atm, natm, d_bas, nbas, d_env, nao, d_S, d_T,
-                                                                  d_V);
+  int1e_kernel<<<(unsigned)((npair + 3) / 4), 256, shm, st>>>(d_atm, natm, d_bas, nbas, d_env, nao, d_S, d_T, d_V);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+template <typename T, int X, int Y>
+static int launch_batch_s8(const BatchMol* mols, const int* work, int nwork, const int* atm, const int* bas,
+                           const double* env, const int* shell_ao, const int* shl, T* out, hipStream_t st) {
+  if (nwork == 0) return PSCF_OK;
+  constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
+  constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
+  using BT = typename BoysType<T, NR>::type;
+  const size_t shm = sizeof(BT) * BOYS_NT * BOYS_NM;
+  eri_s8_batch_kernel<T, X, Y><<<nwork, 256, shm, st>>>(mols, work, atm, bas, env, shell_ao, shl, out);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+template <typename T>
+static int batch_s8_all(const BatchMol* mols, const int* const* work, const int* nwork, const int* atm, const int* bas,
+                        const double* env, const int* shell_ao, const int* shl, T* out, hipStream_t st) {
+  int rc;
+  if ((rc = launch_batch_s8<T, 0, 0>(mols, work[0], nwork[0], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  if ((rc = launch_batch_s8<T, 1, 0>(mols, work[1], nwork[1], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  if ((rc = launch_batch_s8<T, 1, 1>(mols, work[2], nwork[2], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  if ((rc = launch_batch_s8<T, 2, 0>(mols, work[3], nwork[3], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  if ((rc = launch_batch_s8<T, 2, 1>(mols, work[4], nwork[4], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  if ((rc = launch_batch_s8<T, 2, 2>(mols, work[5], nwork[5], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const int* const* h_work,
+                                 const int* h_nwork, const int* d_atm, const int* d_bas, const double* d_env,
+                                 const int* d_shell_ao, const int* d_shl_sorted, void* d_out, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_eri_s8: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && d_mols && h_work && h_nwork && d_atm && d_bas && d_env && d_shell_ao && d_shl_sorted &&
+                   d_out,
+               "pscf_batch_eri_s8: bad arguments");
+  for (int c = 0; c < 6; ++c) PSCF_REQUIRE(h_nwork[c] >= 0 && (h_nwork[c] == 0 || h_work[c]), "pscf_batch_eri_s8: bad work list");
+  int rc = ensure_boys_table();
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    return batch_s8_all<double>((const BatchMol*)d_mols, h_work, h_nwork, d_atm, d_bas, d_env, d_shell_ao,
+                                d_shl_sorted, (double*)d_out, st);
+  return batch_s8_all<float>((const BatchMol*)d_mols, h_work, h_nwork, d_atm, d_bas, d_env, d_shell_ao, d_shl_sorted,
+                             (float*)d_out, st);
+}
+
+extern "C" int pscf_batch_int1e(const void* d_mols, int nmol, const int* d_atm, const int* d_bas, const double* d_env,
+                                const int* d_shell_ao, double* d_S, double* d_Hcore, void* stream) {
+  PSCF_REQUIRE(nmol > 0 && d_mols && d_atm && d_bas && d_env && d_shell_ao && d_S && d_Hcore,
+               "pscf_batch_int1e: bad arguments");
+  int rc = ensure_boys_table();
+  if (rc) return rc;
+  const size_t shm = sizeof(double) * BOYS_NT * BOYS_NM;
+  int1e_batch_kernel<<<nmol, 256, shm, (hipStream_t)stream>>>((const BatchMol*)d_mols, d_atm, d_bas, d_env,
+                                                              d_shell_ao, d_S, d_Hcore);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }

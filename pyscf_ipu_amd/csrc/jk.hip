@@ -11,22 +11,12 @@
 //   * jk_s8: the packed s8 array of pscf_eri_s8, indices regenerated from the
 //     element position (4 B of HBM traffic per ERI), dm symmetric, so J and K
 //     fold into one accumulator M with dJK = M + M^T (6 scatters per ERI).
+#include "batch.h"
 #include "common.h"
 
 namespace pscf {
 
-static int max_lds_bytes() {
-  static int cached = -1;
-  if (cached < 0) {
-    int dev = 0, v = 65536;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
-      cached = v;
-    else
-      cached = 65536;
-  }
-  return cached;
-}
+static int max_lds_bytes() { return device_max_lds(); }
 
 constexpr int JK_THREADS = 256;
 constexpr int JK_MAX_BLOCKS = 240;
@@ -87,20 +77,25 @@ __global__ void __launch_bounds__(JK_THREADS) jk_sparse_kernel(const T* __restri
   }
 }
 
-// out[x] = sum_b part[b][x] (+ part[b][x^T] when SYM), fixed order.
+// out[x] = sum_b part[b][x] (+ part[b][x^T] when SYM), fixed order.  One
+// 64-lane wave per output element: lanes stride over the partials, then a
+// fixed shuffle tree combines them.
 template <typename T, bool SYM>
 __global__ void jk_reduce_kernel(const T* __restrict__ part, int nb, int N, T* __restrict__ out) {
   const int NN = N * N;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (x >= NN) return;
   const int a = x / N, b = x - a * N;
   const int xt = b * N + a;
   T s = T(0);
-  for (int k = 0; k < nb; ++k) {
+  for (int k = lane; k < nb; k += 64) {
     s += part[(int64_t)k * NN + x];
     if (SYM) s += part[(int64_t)k * NN + xt];
   }
-  out[x] = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[x] = s;
 }
 
 template <typename T>
@@ -168,6 +163,77 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
   for (int x = threadIdx.x; x < NN; x += blockDim.x) dst[x] = acc[x];
 }
 
+// ------------------------------------------------------------- batched ----
+// One workgroup per molecule (molecule order[blockIdx.x]); converged molecules
+// (ctl active == 0) are skipped.  dJK = M + M^T is written directly, or, with
+// ACC, added to an FP64 accumulator (incremental build from density changes:
+// the FP32 rounding then scales with |dD| instead of |D|).
+template <typename T, typename TO, bool ACC>
+__global__ void __launch_bounds__(JK_THREADS) jk_s8_batch_kernel(const BatchMol* __restrict__ mols,
+                                                                 const int* __restrict__ order,
+                                                                 const double* __restrict__ ctl,
+                                                                 const T* __restrict__ eri_all,
+                                                                 const T* __restrict__ dm_all, T hhalf,
+                                                                 TO* __restrict__ out_all) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = order[blockIdx.x];
+  if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
+  const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao, NN = N * N, P = N * (N + 1) / 2;
+  const T* eri = eri_all + Mo.eri0;
+  const T* dm_g = dm_all + Mo.mat0;
+  T* dm = reinterpret_cast<T*>(smem);
+  T* acc = dm + NN;
+  unsigned* pair = reinterpret_cast<unsigned*>(acc + NN);
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+    dm[x] = dm_g[x];
+    acc[x] = T(0);
+  }
+  for (int c = threadIdx.x; c < P; c += blockDim.x) {
+    int a = (int)((sqrtf(8.0f * (float)c + 1.0f) - 1.0f) * 0.5f);
+    while (tri(a + 1) <= c) ++a;
+    while (tri(a) > c) --a;
+    pair[c] = (unsigned)a | ((unsigned)(c - tri(a)) << 16);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  for (int pq = wave; pq < P; pq += nwaves) {
+    const unsigned pp = pair[pq];
+    const int p = pp & 0xffff, q = pp >> 16;
+    const T dpq = dm[p * N + q];
+    const T* row = eri + tri(pq);
+    T jrow = T(0);
+    for (int rs = lane; rs <= pq; rs += 64) {
+      const unsigned rr = pair[rs];
+      const int r = rr & 0xffff, s = rr >> 16;
+      T v = row[rs];
+      if (p == q) v *= T(0.5);
+      if (r == s) v *= T(0.5);
+      if (pq == rs) v *= T(0.5);
+      const T v2 = v + v;
+      atomicAdd(acc + r * N + s, v2 * dpq);
+      jrow += v2 * dm[r * N + s];
+      const T vk = -hhalf * v;
+      atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
+      atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
+      atomicAdd(acc + p * N + r, vk * dm[q * N + s]);
+      atomicAdd(acc + q * N + r, vk * dm[p * N + s]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) jrow += __shfl_xor(jrow, off, 64);
+    if (lane == 0) atomicAdd(acc + p * N + q, jrow);
+  }
+  __syncthreads();
+  TO* out = out_all + Mo.mat0;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    const TO v = (TO)(acc[x] + acc[j * N + i]);
+    out[x] = ACC ? out[x] + v : v;
+  }
+}
+
 template <typename T>
 static int jk_sparse_impl(const T* eri, const int16_t* idx, int64_t nnz, const T* dm, int N, double hyb, T* out,
                           void* ws, size_t ws_bytes, hipStream_t st) {
@@ -178,14 +244,13 @@ static int jk_sparse_impl(const T* eri, const int16_t* idx, int64_t nnz, const T
   int nb = (int)(want < 1 ? 1 : (want > JK_MAX_BLOCKS ? JK_MAX_BLOCKS : want));
   if ((int)lds <= max_lds_bytes()) {
     PSCF_REQUIRE(ws && ws_bytes >= sizeof(T) * (size_t)nb * NN, "pscf_jk_sparse: workspace too small");
-    if (lds > 65536) PSCF_HIP(hipFuncSetAttribute((const void*)jk_sparse_kernel<T, true>,
-                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    PSCF_SET_LDS((jk_sparse_kernel<T, true>), lds);
     if (nnz > 0)
       jk_sparse_kernel<T, true><<<nb, JK_THREADS, lds, st>>>(eri, idx, nnz, dm, N, kscale, (T*)ws);
     else
       fill_kernel<T><<<64, 256, 0, st>>>((T*)ws, (int64_t)NN, T(0)), nb = 1;
     PSCF_HIP(hipGetLastError());
-    jk_reduce_kernel<T, false><<<(NN + 255) / 256, 256, 0, st>>>((const T*)ws, nb, N, out);
+    jk_reduce_kernel<T, false><<<(NN + 3) / 4, 256, 0, st>>>((const T*)ws, nb, N, out);
   } else {
     fill_kernel<T><<<64, 256, 0, st>>>(out, (int64_t)NN, T(0));
     if (nnz > 0) jk_sparse_kernel<T, false><<<nb, JK_THREADS, 0, st>>>(eri, idx, nnz, dm, N, kscale, out);
@@ -214,11 +279,10 @@ static int jk_s8_impl(const T* eri, int N, const T* dm, double hyb, T* out, void
                max_lds_bytes());
   const int nb = jk_s8_blocks(N);
   PSCF_REQUIRE(ws && ws_bytes >= sizeof(T) * (size_t)nb * N * N, "pscf_jk_s8: workspace too small");
-  if (lds > 65536)
-    PSCF_HIP(hipFuncSetAttribute((const void*)jk_s8_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  PSCF_SET_LDS(jk_s8_kernel<T>, lds);
   jk_s8_kernel<T><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
   PSCF_HIP(hipGetLastError());
-  jk_reduce_kernel<T, true><<<(N * N + 255) / 256, 256, 0, st>>>((const T*)ws, nb, N, out);
+  jk_reduce_kernel<T, true><<<(N * N + 3) / 4, 256, 0, st>>>((const T*)ws, nb, N, out);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
@@ -261,4 +325,44 @@ extern "C" int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_
   if (dtype == PSCF_F64)
     return jk_s8_impl<double>((const double*)d_eri_s8, N, (const double*)d_dm, hyb, (double*)d_out, d_ws, ws_bytes, st);
   return jk_s8_impl<float>((const float*)d_eri_s8, N, (const float*)d_dm, hyb, (float*)d_out, d_ws, ws_bytes, st);
+}
+
+extern "C" int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                                const double* d_ctl, const void* d_eri, const void* d_dm, double hyb, void* d_out,
+                                int accumulate, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_jk_s8: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && max_nao > 0 && max_nao < 65536 && d_mols && d_order && d_eri && d_dm && d_out,
+               "pscf_batch_jk_s8: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64) {
+    const size_t lds = jk_s8_lds<double>(max_nao);
+    PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
+    if (accumulate) {
+      PSCF_SET_LDS((jk_s8_batch_kernel<double, double, true>), lds);
+      jk_s8_batch_kernel<double, double, true><<<nmol, JK_THREADS, lds, st>>>(
+          (const BatchMol*)d_mols, d_order, d_ctl, (const double*)d_eri, (const double*)d_dm, hyb / 2.0,
+          (double*)d_out);
+    } else {
+      PSCF_SET_LDS((jk_s8_batch_kernel<double, double, false>), lds);
+      jk_s8_batch_kernel<double, double, false><<<nmol, JK_THREADS, lds, st>>>(
+          (const BatchMol*)d_mols, d_order, d_ctl, (const double*)d_eri, (const double*)d_dm, hyb / 2.0,
+          (double*)d_out);
+    }
+  } else {
+    const size_t lds = jk_s8_lds<float>(max_nao);
+    PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
+    if (accumulate) {
+      PSCF_SET_LDS((jk_s8_batch_kernel<float, double, true>), lds);
+      jk_s8_batch_kernel<float, double, true><<<nmol, JK_THREADS, lds, st>>>(
+          (const BatchMol*)d_mols, d_order, d_ctl, (const float*)d_eri, (const float*)d_dm, (float)(hyb / 2.0),
+          (double*)d_out);
+    } else {
+      PSCF_SET_LDS((jk_s8_batch_kernel<float, float, false>), lds);
+      jk_s8_batch_kernel<float, float, false><<<nmol, JK_THREADS, lds, st>>>(
+          (const BatchMol*)d_mols, d_order, d_ctl, (const float*)d_eri, (const float*)d_dm, (float)(hyb / 2.0),
+          (float*)d_out);
+    }
+  }
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
 }

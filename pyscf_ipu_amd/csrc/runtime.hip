@@ -16,6 +16,18 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+int device_max_lds() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (cached[dev] == 0) {
+    int v = 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
 int check_hip(hipError_t e, const char* what) {
   set_error("%s: %s", what, hipGetErrorString(e));
   return PSCF_EHIP;

@@ -10,11 +10,12 @@
 // disjoint rotations per round), warm-started from the previous step's
 // eigenvectors: A' = U_prev^T A U_prev is nearly diagonal after the first SCF
 // iterations, so a few sweeps suffice.
+#include "batch.h"
 #include "common.h"
 
 namespace pscf {
 
-constexpr int SCF_THREADS = 256;
+constexpr int SCF_THREADS = 1024;
 constexpr int DIIS_N = 9;
 constexpr int MAX_SWEEPS = 30;
 
@@ -23,22 +24,37 @@ __host__ __device__ inline size_t state_doubles(int N) {
   return (size_t)2 * DIIS_N * N * N + 100 + (size_t)N * N + 2;
 }
 
-// C (LDS or global, ldc = N) = op(A) * op(B), N x N, all FP64, block-cooperative.
-// TA / TB: use the transpose.
-template <bool TA, bool TB>
-__device__ inline void matmul(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
-                              int N) {
-  const int NN = N * N;
-  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    double s = 0.0;
+// C = op(X) op(Y), N x N, all operands in LDS (C may be LDS or global, never
+// aliasing X or Y).  2x2 register blocks per thread.
+template <bool TX, bool TY>
+__device__ inline void lds_matmul(const double* __restrict__ X, const double* __restrict__ Y, double* __restrict__ C,
+                                  int N) {
+  const int nb = (N + 1) >> 1;
+  for (int b = threadIdx.x; b < nb * nb; b += blockDim.x) {
+    const int i = (b / nb) * 2, j = (b % nb) * 2;
+    const bool i1 = i + 1 < N, j1 = j + 1 < N;
+    double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
     for (int k = 0; k < N; ++k) {
-      const double a = TA ? A[k * N + i] : A[i * N + k];
-      const double b = TB ? B[j * N + k] : B[k * N + j];
-      s += a * b;
+      const double x0 = TX ? X[k * N + i] : X[i * N + k];
+      const double x1 = i1 ? (TX ? X[k * N + i + 1] : X[(i + 1) * N + k]) : 0.0;
+      const double y0 = TY ? Y[j * N + k] : Y[k * N + j];
+      const double y1 = j1 ? (TY ? Y[(j + 1) * N + k] : Y[k * N + j + 1]) : 0.0;
+      c00 += x0 * y0;
+      c01 += x0 * y1;
+      c10 += x1 * y0;
+      c11 += x1 * y1;
     }
-    C[x] = s;
+    C[i * N + j] = c00;
+    if (j1) C[i * N + j + 1] = c01;
+    if (i1) {
+      C[(i + 1) * N + j] = c10;
+      if (j1) C[(i + 1) * N + j + 1] = c11;
+    }
   }
+}
+
+__device__ inline void lds_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
+  for (int x = threadIdx.x; x < n; x += blockDim.x) dst[x] = src[x];
 }
 
 __device__ inline double block_sum(double v, double* red) {
@@ -90,14 +106,17 @@ __device__ inline void small_jacobi(double* a, double* v, int n) {
   }
 }
 
-// Parallel cyclic Jacobi on A (LDS, N x N, destroyed: diagonal = eigenvalues),
-// accumulating rotations into V (LDS, N x N; caller initialises it).
-__device__ inline void parallel_jacobi(double* A, double* V, int N, int* rp, int* rq, double* rc, double* rs,
-                                       double* red) {
-  const int m = N + (N & 1);  // even number of players, index N is a dummy
+// Parallel cyclic Jacobi on A (LDS, N x N; on exit its diagonal holds the
+// eigenvalues), accumulating rotations into V (LDS, caller initialised).
+// Round-robin pairing gives N/2 disjoint rotations per round; the two-sided
+// update A <- J^T A J is applied as one pass over the 2x2 blocks (pair k rows,
+// pair l columns): Y = R_k^T X R_l with R = [[c, s], [-s, c]].
+__device__ inline int parallel_jacobi(double* A, double* V, int N, int* rp, int* rq, double* rc, double* rs,
+                                      double* red) {
+  const int m = N + (N & 1);  // even number of players; index N is a dummy
   const int half = m / 2;
-  for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
-    // convergence test: off-diagonal vs total Frobenius norm
+  int sweep = 0;
+  for (; sweep < MAX_SWEEPS; ++sweep) {
     double off = 0.0, tot = 0.0;
     for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
@@ -141,32 +160,225 @@ __device__ inline void parallel_jacobi(double* A, double* V, int N, int* rp, int
         rs[k] = s;
       }
       __syncthreads();
-      // rows: A <- J^T A
-      for (int x = threadIdx.x; x < half * N; x += blockDim.x) {
-        const int k = x / N, j = x - k * N;
-        const int p = rp[k], q = rq[k];
-        if (q >= N || rs[k] == 0.0) continue;
-        const double c = rc[k], s = rs[k];
-        const double apj = A[p * N + j], aqj = A[q * N + j];
-        A[p * N + j] = c * apj - s * aqj;
-        A[q * N + j] = s * apj + c * aqj;
+      // 2x2 blocks of A
+      for (int b = threadIdx.x; b < half * half; b += blockDim.x) {
+        const int k = b / half, l = b - k * half;
+        const int pk = rp[k], qk = rq[k], pl = rp[l], ql = rq[l];
+        const double ck = rc[k], sk = rs[k], cl = rc[l], sl = rs[l];
+        const bool qkv = qk < N, qlv = ql < N;
+        const double x00 = A[pk * N + pl];
+        const double x01 = qlv ? A[pk * N + ql] : 0.0;
+        const double x10 = qkv ? A[qk * N + pl] : 0.0;
+        const double x11 = (qkv && qlv) ? A[qk * N + ql] : 0.0;
+        // columns: [x.0, x.1] R_l
+        const double t00 = cl * x00 - sl * x01, t01 = sl * x00 + cl * x01;
+        const double t10 = cl * x10 - sl * x11, t11 = sl * x10 + cl * x11;
+        // rows: R_k^T [t0.; t1.]
+        A[pk * N + pl] = ck * t00 - sk * t10;
+        if (qlv) A[pk * N + ql] = ck * t01 - sk * t11;
+        if (qkv) A[qk * N + pl] = sk * t00 + ck * t10;
+        if (qkv && qlv) A[qk * N + ql] = sk * t01 + ck * t11;
       }
-      __syncthreads();
-      // columns: A <- A J, V <- V J
+      // columns of V
       for (int x = threadIdx.x; x < half * N; x += blockDim.x) {
-        const int k = x / N, i = x - k * N;
-        const int p = rp[k], q = rq[k];
-        if (q >= N || rs[k] == 0.0) continue;
-        const double c = rc[k], s = rs[k];
-        const double aip = A[i * N + p], aiq = A[i * N + q];
-        A[i * N + p] = c * aip - s * aiq;
-        A[i * N + q] = s * aip + c * aiq;
+        const int l = x / N, i = x - l * N;
+        const int p = rp[l], q = rq[l];
+        if (q >= N) continue;
+        const double c = rc[l], s = rs[l];
         const double vip = V[i * N + p], viq = V[i * N + q];
         V[i * N + p] = c * vip - s * viq;
         V[i * N + q] = s * vip + c * viq;
       }
       __syncthreads();
     }
+  }
+  return sweep;
+}
+
+struct ScfLds {
+  double *A, *U, *B, *Bm, *Bv, *cvec, *red, *rc, *rs, *eig;
+  int *rp, *rq, *rank;
+};
+
+__host__ __device__ inline size_t scf_lds_bytes(int N) {
+  const int h = N / 2 + 1;
+  return sizeof(double) * (3 * (size_t)N * N + 200 + 16 + 16 + 2 * h + N) + sizeof(int) * (2 * h + N);
+}
+
+__device__ inline ScfLds scf_lds_carve(unsigned char* smem, int N) {
+  ScfLds L;
+  const int NN = N * N, h = N / 2 + 1;
+  L.A = reinterpret_cast<double*>(smem);
+  L.U = L.A + NN;
+  L.B = L.U + NN;
+  L.Bm = L.B + NN;
+  L.Bv = L.Bm + 100;
+  L.cvec = L.Bv + 100;
+  L.red = L.cvec + 16;
+  L.rc = L.red + 16;
+  L.rs = L.rc + h;
+  L.eig = L.rs + h;
+  L.rp = reinterpret_cast<int*>(L.eig + N);
+  L.rq = L.rp + h;
+  L.rank = L.rq + h;
+  return L;
+}
+
+// One SCF step for one molecule by the whole workgroup (see pscf_scf_step).
+// Optional Dt receives D converted to T (the precision of the next J/K / XC),
+// optional dDt the change of D in T (for an incremental J/K build).
+template <typename T, typename TJ>
+__device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, const double* __restrict__ Hcore,
+                                     const double* __restrict__ S, const double* __restrict__ Linv,
+                                     const TJ* __restrict__ dJK, const T* __restrict__ Vxc, double* __restrict__ D,
+                                     T* __restrict__ Dt, T* __restrict__ dDt, double* __restrict__ Hout,
+                                     double* __restrict__ mo_e, double* __restrict__ Cout,
+                                     double* __restrict__ state, int use_diis) {
+  const int NN = N * N;
+  double *A = L.A, *U = L.U, *B = L.B;
+  double* Vh = state;
+  double* Hh = state + (size_t)DIIS_N * NN;
+  double* Bst = Hh + (size_t)DIIS_N * NN;
+  double* Uprev = Bst + 100;
+  double* flags = Uprev + NN;
+  const int tid = threadIdx.x;
+
+  // H = H_core + dJK + V_xc
+  for (int x = tid; x < NN; x += blockDim.x) A[x] = Hcore[x] + (double)dJK[x] + (double)Vxc[x];
+  if (use_diis) lds_copy(B, D, NN);
+  __syncthreads();
+
+  if (use_diis) {
+    const int head = it % DIIS_N;
+    if (it == 0)
+      for (int x = tid; x < 100; x += blockDim.x) {
+        const int i = x / 10, j = x % 10;
+        Bst[x] = ((i == 0) != (j == 0)) ? 1.0 : 0.0;
+      }
+    lds_matmul<false, false>(B, A, U, N);  // U = D H
+    __syncthreads();
+    lds_copy(B, S, NN);
+    __syncthreads();
+    // errvec = S U - (S U)^T into Vh[head]; H into Hh[head]
+    double* err = Vh + (size_t)head * NN;
+    for (int x = tid; x < NN; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      double sij = 0.0, sji = 0.0;
+      for (int k = 0; k < N; ++k) {
+        sij += B[i * N + k] * U[k * N + j];
+        sji += B[j * N + k] * U[k * N + i];
+      }
+      err[x] = sij - sji;
+      Hh[(size_t)head * NN + x] = A[x];
+    }
+    __threadfence_block();
+    __syncthreads();
+    const int nvalid = it + 1 < DIIS_N ? it + 1 : DIIS_N;
+    for (int k = 0; k < DIIS_N; ++k) {
+      double part = 0.0;
+      if (k < nvalid)
+        for (int x = tid; x < NN; x += blockDim.x) part += Vh[(size_t)k * NN + x] * err[x];
+      const double dot = block_sum(part, L.red);
+      if (tid == 0) {
+        Bst[(head + 1) * 10 + (k + 1)] = dot;
+        Bst[(k + 1) * 10 + (head + 1)] = dot;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double mask[10];
+      mask[0] = 1.0;
+      for (int k = 0; k < DIIS_N; ++k) mask[k + 1] = k < nvalid ? 1.0 : 0.0;
+      for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j) L.Bm[i * 10 + j] = Bst[i * 10 + j] * mask[i] * mask[j];
+      small_jacobi(L.Bm, L.Bv, 10);
+      double wmax = 0.0;
+      for (int i = 0; i < 10; ++i) wmax = fmax(wmax, fabs(L.Bm[i * 10 + i]));
+      const double cut = 10.0 * 10.0 * 2.220446049250313e-16 * wmax;
+      // pinv(B)[0, j] = sum_k V[0,k] V[j,k] / w_k over |w_k| > cut
+      for (int j = 0; j < 10; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < 10; ++k) {
+          const double wk = L.Bm[k * 10 + k];
+          if (fabs(wk) > cut) acc += L.Bv[k] * L.Bv[j * 10 + k] / wk;
+        }
+        L.cvec[j] = acc;
+      }
+    }
+    __syncthreads();
+    for (int x = tid; x < NN; x += blockDim.x) {
+      double h = 0.0;
+      for (int k = 0; k < DIIS_N; ++k) h += L.cvec[k + 1] * Hh[(size_t)k * NN + x];
+      A[x] = h;
+    }
+  }
+  lds_copy(B, Linv, NN);
+  __syncthreads();
+  for (int x = tid; x < NN; x += blockDim.x) Hout[x] = A[x];
+  lds_matmul<false, true>(A, B, U, N);  // U = H L^T
+  __syncthreads();
+  lds_matmul<false, false>(B, U, A, N);  // A = L U
+  __syncthreads();
+  // exact symmetry (rounding of the FP32 Fock terms and of the transform
+  // leaves O(eps) asymmetry that Jacobi rotations cannot remove)
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    if (i < j) {
+      const double v = 0.5 * (A[x] + A[j * N + i]);
+      A[x] = v;
+      A[j * N + i] = v;
+    }
+  }
+  __syncthreads();
+  const bool warm = flags[0] != 0.0;
+  if (warm) {
+    lds_copy(B, Uprev, NN);
+    __syncthreads();
+    lds_matmul<false, false>(A, B, U, N);  // U = A Uprev
+    __syncthreads();
+    lds_matmul<true, false>(B, U, A, N);  // A = Uprev^T U
+    __syncthreads();
+    lds_copy(U, B, NN);
+  } else {
+    for (int x = tid; x < NN; x += blockDim.x) U[x] = (x / N == x % N) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int sweeps = parallel_jacobi(A, U, N, L.rp, L.rq, L.rc, L.rs, L.red);
+  if (tid == 0) flags[1] = (double)sweeps;  // diagnostic: Jacobi sweeps of this step
+  for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * N + j];
+  __syncthreads();
+  for (int j = tid; j < N; j += blockDim.x) {
+    const double lj = L.eig[j];
+    int r = 0;
+    for (int k = 0; k < N; ++k) {
+      const double lk = L.eig[k];
+      r += (lk < lj) || (lk == lj && k < j);
+    }
+    L.rank[j] = r;
+    mo_e[r] = lj;
+  }
+  // previous eigenvectors (orthogonalised basis) for the next warm start
+  for (int x = tid; x < NN; x += blockDim.x) Uprev[x] = U[x];
+  if (tid == 0) flags[0] = 1.0;
+  lds_copy(B, Linv, NN);
+  __syncthreads();
+  // C = L^-T U, columns in ascending eigenvalue order, into A
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    double acc = 0.0;
+    for (int k = 0; k < N; ++k) acc += B[k * N + i] * U[k * N + j];
+    A[i * N + L.rank[j]] = acc;
+  }
+  __syncthreads();
+  for (int x = tid; x < NN; x += blockDim.x) Cout[x] = A[x];
+  // D = 2 C_occ C_occ^T
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    double acc = 0.0;
+    for (int k = 0; k < nocc; ++k) acc += A[i * N + k] * A[j * N + k];
+    if (dDt) dDt[x] = (T)(2.0 * acc - D[x]);
+    D[x] = 2.0 * acc;
+    if (Dt) Dt[x] = (T)(2.0 * acc);
   }
 }
 
@@ -179,145 +391,9 @@ __global__ void __launch_bounds__(SCF_THREADS) scf_step_kernel(int it, int N, in
                                                                double* __restrict__ mo_e, double* __restrict__ Cout,
                                                                double* __restrict__ state, int use_diis) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int NN = N * N;
-  double* A = reinterpret_cast<double*>(smem);  // N*N
-  double* U = A + NN;                            // N*N
-  double* Bm = U + NN;                           // 10*10 work
-  double* Bv = Bm + 100;                         // 10*10 eigenvectors
-  double* cvec = Bv + 100;                       // 10
-  double* red = cvec + 16;                       // 8
-  double* rc = red + 8;                          // N/2+1
-  double* rs = rc + (N / 2 + 1);
-  int* rp = reinterpret_cast<int*>(rs + (N / 2 + 1));
-  int* rq = rp + (N / 2 + 1);
-  int* rank = rq + (N / 2 + 1);                  // N
-
-  double* Vh = state;
-  double* Hh = state + (size_t)DIIS_N * NN;
-  double* Bst = Hh + (size_t)DIIS_N * NN;
-  double* Uprev = Bst + 100;
-  double* flags = Uprev + NN;
-  const int tid = threadIdx.x;
-
-  // H = H_core + dJK + V_xc
-  for (int x = tid; x < NN; x += blockDim.x) A[x] = Hcore[x] + (double)dJK[x] + (double)Vxc[x];
-  __syncthreads();
-
-  if (use_diis) {
-    const int head = it % DIIS_N;
-    if (it == 0) {
-      for (int x = tid; x < 100; x += blockDim.x) {
-        const int i = x / 10, j = x % 10;
-        Bst[x] = ((i == 0) != (j == 0)) ? 1.0 : 0.0;
-      }
-    }
-    // U = D H ; then errvec = S U - (S U)^T written to Vh[head]; H to Hh[head]
-    matmul<false, false>(D, A, U, N);
-    __syncthreads();
-    double* err = Vh + (size_t)head * NN;
-    for (int x = tid; x < NN; x += blockDim.x) {
-      const int i = x / N, j = x - i * N;
-      double sij = 0.0, sji = 0.0;
-      for (int k = 0; k < N; ++k) {
-        sij += S[i * N + k] * U[k * N + j];
-        sji += S[j * N + k] * U[k * N + i];
-      }
-      err[x] = sij - sji;
-      Hh[(size_t)head * NN + x] = A[x];
-    }
-    __threadfence_block();
-    __syncthreads();
-    const int nvalid = it + 1 < DIIS_N ? it + 1 : DIIS_N;
-    for (int k = 0; k < DIIS_N; ++k) {
-      double part = 0.0;
-      if (k < nvalid)
-        for (int x = tid; x < NN; x += blockDim.x) part += Vh[(size_t)k * NN + x] * err[x];
-      const double dot = block_sum(part, red);
-      if (tid == 0) {
-        Bst[(head + 1) * 10 + (k + 1)] = dot;
-        Bst[(k + 1) * 10 + (head + 1)] = dot;
-      }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double mask[10];
-      mask[0] = 1.0;
-      for (int k = 0; k < DIIS_N; ++k) mask[k + 1] = k < nvalid ? 1.0 : 0.0;
-      for (int i = 0; i < 10; ++i)
-        for (int j = 0; j < 10; ++j) Bm[i * 10 + j] = Bst[i * 10 + j] * mask[i] * mask[j];
-      small_jacobi(Bm, Bv, 10);
-      double wmax = 0.0;
-      for (int i = 0; i < 10; ++i) wmax = fmax(wmax, fabs(Bm[i * 10 + i]));
-      const double cut = 10.0 * 10.0 * 2.220446049250313e-16 * wmax;
-      // pinv row 0: c_j = sum_k V[0,k] V[j,k] / w_k over |w_k| > cut
-      for (int j = 0; j < 10; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < 10; ++k) {
-          const double wk = Bm[k * 10 + k];
-          if (fabs(wk) > cut) s += Bv[0 * 10 + k] * Bv[j * 10 + k] / wk;
-        }
-        cvec[j] = s;
-      }
-    }
-    __syncthreads();
-    for (int x = tid; x < NN; x += blockDim.x) {
-      double h = 0.0;
-      for (int k = 0; k < DIIS_N; ++k) h += cvec[k + 1] * Hh[(size_t)k * NN + x];
-      A[x] = h;
-    }
-    __syncthreads();
-  }
-  for (int x = tid; x < NN; x += blockDim.x) Hout[x] = A[x];
-  // M = L_inv H L_inv^T
-  matmul<false, true>(A, Linv, U, N);  // U = H L^T
-  __syncthreads();
-  matmul<false, false>(Linv, U, A, N);  // A = L U
-  __syncthreads();
-  const bool warm = flags[0] != 0.0;
-  if (warm) {
-    // A <- Uprev^T A Uprev ; V starts at Uprev
-    matmul<false, false>(A, Uprev, U, N);  // U = A Uprev
-    __syncthreads();
-    matmul<true, false>(Uprev, U, A, N);  // A = Uprev^T U
-    __syncthreads();
-    for (int x = tid; x < NN; x += blockDim.x) U[x] = Uprev[x];
-  } else {
-    for (int x = tid; x < NN; x += blockDim.x) U[x] = (x / N == x % N) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  parallel_jacobi(A, U, N, rp, rq, rc, rs, red);
-  // ranks of eigenvalues (ascending, ties by index)
-  for (int j = tid; j < N; j += blockDim.x) {
-    const double lj = A[j * N + j];
-    int r = 0;
-    for (int k = 0; k < N; ++k) {
-      const double lk = A[k * N + k];
-      r += (lk < lj) || (lk == lj && k < j);
-    }
-    rank[j] = r;
-  }
-  __syncthreads();
-  for (int j = tid; j < N; j += blockDim.x) mo_e[rank[j]] = A[j * N + j];
-  // save eigenvectors of the orthogonalised problem for the next warm start
-  for (int x = tid; x < NN; x += blockDim.x) Uprev[x] = U[x];
-  if (tid == 0) flags[0] = 1.0;
-  __syncthreads();
-  // C = L_inv^T U with columns permuted to ascending eigenvalue order (into A)
-  for (int x = tid; x < NN; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    double s = 0.0;
-    for (int k = 0; k < N; ++k) s += Linv[k * N + i] * U[k * N + j];
-    A[i * N + rank[j]] = s;
-  }
-  __syncthreads();
-  for (int x = tid; x < NN; x += blockDim.x) Cout[x] = A[x];
-  // D = 2 C_occ C_occ^T
-  for (int x = tid; x < NN; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    double s = 0.0;
-    for (int k = 0; k < nocc; ++k) s += A[i * N + k] * A[j * N + k];
-    D[x] = 2.0 * s;
-  }
+  const ScfLds L = scf_lds_carve(smem, N);
+  scf_step_body<T, T>(L, it, N, nocc, Hcore, S, Linv, dJK, Vxc, D, (T*)nullptr, (T*)nullptr, Hout, mo_e, Cout, state,
+                      use_diis);
 }
 
 template <typename T>
@@ -348,10 +424,284 @@ __global__ void energy_kernel(int N, const double* __restrict__ D, const double*
   }
 }
 
-static size_t scf_lds(int N) {
-  const int h = N / 2 + 1;
-  return sizeof(double) * (2 * (size_t)N * N + 200 + 16 + 8 + 2 * h) + sizeof(int) * (2 * h + N);
+// ------------------------------------------------------------- batched ----
+// Overlap Cholesky S = L L^T and L^-1 per molecule (FP64, LDS).  A molecule
+// whose S is not positive definite is marked failed and inactive (ctl[2] = -1).
+__global__ void __launch_bounds__(256) chol_batch_kernel(const BatchMol* __restrict__ mols,
+                                                         const double* __restrict__ S, double* __restrict__ Linv,
+                                                         double* __restrict__ ctl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BatchMol Mo = mols[blockIdx.x];
+  const int N = (int)Mo.nao, NN = N * N;
+  double* A = reinterpret_cast<double*>(smem);
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) A[x] = S[Mo.mat0 + x];
+  __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    if (threadIdx.x == 0) {
+      const double d = A[k * N + k];
+      if (!(d > 0.0)) bad = 1;
+      A[k * N + k] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    const double lkk = A[k * N + k];
+    for (int i = k + 1 + threadIdx.x; i < N; i += blockDim.x) A[i * N + k] /= lkk;
+    __syncthreads();
+    const int rem = N - k - 1;
+    for (int t = threadIdx.x; t < rem * rem; t += blockDim.x) {
+      const int i = k + 1 + t / rem, j = k + 1 + t % rem;
+      if (j <= i) A[i * N + j] -= A[i * N + k] * A[j * N + k];
+    }
+    __syncthreads();
+  }
+  // L^-1 column by column (forward substitution), one thread per column
+  double* out = Linv + Mo.mat0;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    for (int i = 0; i < N; ++i) {
+      if (i < j) {
+        out[i * N + j] = 0.0;
+        continue;
+      }
+      double acc = (i == j) ? 1.0 : 0.0;
+      for (int k = j; k < i; ++k) acc -= A[i * N + k] * out[k * N + j];
+      out[i * N + j] = acc / A[i * N + i];
+    }
+  }
+  if (threadIdx.x == 0 && bad) {
+    ctl[blockIdx.x * CTL + 1] = 0.0;
+    ctl[blockIdx.x * CTL + 2] = -1.0;
+  }
 }
+
+// Atomic-density guess (same rule as the host `guess.minao`): per atom and
+// angular momentum, Gram-Schmidt the atom's shells (Cholesky of the S block)
+// and occupy them with the element's spherical-average occupations.
+__constant__ double c_occ[11][2][4] = {
+    // {s occupations}, {p occupations (per component)} for Z = 0..10
+    {{0, 0, 0, 0}, {0, 0, 0, 0}},
+    {{1, 0, 0, 0}, {0, 0, 0, 0}},           // H  1s1
+    {{2, 0, 0, 0}, {0, 0, 0, 0}},           // He
+    {{2, 1, 0, 0}, {0, 0, 0, 0}},           // Li
+    {{2, 2, 0, 0}, {0, 0, 0, 0}},           // Be
+    {{2, 2, 0, 0}, {1.0 / 3, 0, 0, 0}},     // B
+    {{2, 2, 0, 0}, {2.0 / 3, 0, 0, 0}},     // C
+    {{2, 2, 0, 0}, {1.0, 0, 0, 0}},         // N
+    {{2, 2, 0, 0}, {4.0 / 3, 0, 0, 0}},     // O
+    {{2, 2, 0, 0}, {5.0 / 3, 0, 0, 0}},     // F
+    {{2, 2, 0, 0}, {2.0, 0, 0, 0}}};        // Ne
+
+template <typename T>
+__global__ void __launch_bounds__(256) guess_batch_kernel(const BatchMol* __restrict__ mols,
+                                                          const int* __restrict__ atm, const int* __restrict__ bas,
+                                                          const int* __restrict__ shell_ao,
+                                                          const double* __restrict__ S, double* __restrict__ D,
+                                                          T* __restrict__ Dt) {
+  const BatchMol Mo = mols[blockIdx.x];
+  const int N = (int)Mo.nao, NN = N * N;
+  double* Dm = D + Mo.mat0;
+  const double* Sm = S + Mo.mat0;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) Dm[x] = 0.0;
+  __syncthreads();
+  for (int ia = threadIdx.x; ia < (int)Mo.natm; ia += blockDim.x) {
+    const int a = (int)Mo.atm0 + ia;
+    const int z = atm[a * ATM_SLOTS];
+    for (int l = 0; l < 2; ++l) {
+      int ao[4], n = 0;
+      for (int ib = (int)Mo.bas0; ib < (int)(Mo.bas0 + Mo.nbas) && n < 4; ++ib)
+        if (bas[ib * BAS_SLOTS] == a && bas[ib * BAS_SLOTS + 1] == l) ao[n++] = shell_ao[ib];
+      if (n == 0) continue;
+      for (int m = 0; m < 2 * l + 1; ++m) {
+        // S block -> L (Cholesky) -> C = L^-T (columns orthonormal)
+        double Lb[4][4] = {{0}}, Ci[4][4] = {{0}};
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j <= i; ++j) {
+            double acc = Sm[(ao[i] + m) * N + ao[j] + m];
+            for (int k = 0; k < j; ++k) acc -= Lb[i][k] * Lb[j][k];
+            Lb[i][j] = (i == j) ? sqrt(fmax(acc, 1e-300)) : acc / Lb[j][j];
+          }
+        for (int j = 0; j < n; ++j)  // Linv column j
+          for (int i = j; i < n; ++i) {
+            double acc = (i == j) ? 1.0 : 0.0;
+            for (int k = j; k < i; ++k) acc -= Lb[i][k] * Ci[k][j];
+            Ci[i][j] = acc / Lb[i][i];
+          }
+        // orbital k has coefficients C[:, k] = (L^-T)[:, k] = Linv[k][:]
+        for (int k = 0; k < n; ++k) {
+          const double o = c_occ[z > 10 ? 0 : z][l][k];
+          if (o == 0.0) continue;
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) Dm[(ao[i] + m) * N + ao[j] + m] += o * Ci[k][i] * Ci[k][j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (Dt != nullptr)
+    for (int x = threadIdx.x; x < NN; x += blockDim.x) Dt[Mo.mat0 + x] = (T)Dm[x];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCF_THREADS) scf_step_batch_kernel(const BatchMol* __restrict__ mols,
+                                                                     const int* __restrict__ order,
+                                                                     double* __restrict__ ctl,
+                                                                     const double* __restrict__ Hcore,
+                                                                     const double* __restrict__ S,
+                                                                     const double* __restrict__ Linv,
+                                                                     const double* __restrict__ dJK,
+                                                                     const T* __restrict__ Vxc, double* __restrict__ D,
+                                                                     T* __restrict__ Dt, T* __restrict__ dDt,
+                                                                     double* __restrict__ H,
+                                                                     double* __restrict__ mo_e,
+                                                                     double* __restrict__ C, double* __restrict__ state,
+                                                                     int use_diis) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = order[blockIdx.x];
+  if (ctl[m * CTL + 1] == 0.0) return;
+  const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao;
+  const ScfLds L = scf_lds_carve(smem, N);
+  const int it = (int)ctl[m * CTL + 0];
+  const int64_t o = Mo.mat0;
+  scf_step_body<T, double>(L, it, N, (int)Mo.nocc, Hcore + o, S + o, Linv + o, dJK + o, Vxc + o, D + o, Dt + o,
+                           dDt ? dDt + o : nullptr, H + o, mo_e + Mo.vec0, C + o, state + Mo.state0, use_diis);
+}
+
+// Purification of a predicted density (one workgroup per molecule):
+// symmetrise, map to the orthonormal basis P = L^T D L = L^-1 (S D S) L^-T,
+// diagonalise, and keep the nocc eigenvectors of largest occupation:
+// D = 2 L^-T V_occ V_occ^T L^-1, so tr(D S) = 2 nocc and D S D = 2 D exactly.
+template <typename T>
+__global__ void __launch_bounds__(SCF_THREADS) purify_batch_kernel(const BatchMol* __restrict__ mols,
+                                                                   const int* __restrict__ order,
+                                                                   const double* __restrict__ ctl,
+                                                                   const double* __restrict__ S,
+                                                                   const double* __restrict__ Linv,
+                                                                   double* __restrict__ D, T* __restrict__ Dt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = order[blockIdx.x];
+  if (ctl[m * CTL + 2] < 0.0) return;  // failed set-up
+  const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao, NN = N * N, nocc = (int)Mo.nocc;
+  const ScfLds L = scf_lds_carve(smem, N);
+  double *A = L.A, *U = L.U, *B = L.B;
+  const int64_t o = Mo.mat0;
+  const int tid = threadIdx.x;
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    A[x] = 0.5 * (D[o + x] + D[o + j * N + i]);
+  }
+  lds_copy(B, S + o, NN);
+  __syncthreads();
+  lds_matmul<false, false>(B, A, U, N);  // U = S D
+  __syncthreads();
+  lds_matmul<false, false>(U, B, A, N);  // A = S D S
+  __syncthreads();
+  lds_copy(B, Linv + o, NN);
+  __syncthreads();
+  lds_matmul<false, false>(B, A, U, N);  // U = L^-1 S D S
+  __syncthreads();
+  lds_matmul<false, true>(U, B, A, N);  // A = U L^-T
+  __syncthreads();
+  for (int x = tid; x < NN; x += blockDim.x) {
+    A[x] = -A[x];  // ascending eigenvalues of -P = descending occupations
+    U[x] = (x / N == x % N) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  parallel_jacobi(A, U, N, L.rp, L.rq, L.rc, L.rs, L.red);
+  for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * N + j];
+  __syncthreads();
+  for (int j = tid; j < N; j += blockDim.x) {
+    const double lj = L.eig[j];
+    int r = 0;
+    for (int k = 0; k < N; ++k) {
+      const double lk = L.eig[k];
+      r += (lk < lj) || (lk == lj && k < j);
+    }
+    L.rank[j] = r;
+  }
+  __syncthreads();
+  // C = L^-T U with columns in rank order (B still holds L^-1)
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    double acc = 0.0;
+    for (int k = 0; k < N; ++k) acc += B[k * N + i] * U[k * N + j];
+    A[i * N + L.rank[j]] = acc;
+  }
+  __syncthreads();
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    double acc = 0.0;
+    for (int k = 0; k < nocc; ++k) acc += A[i * N + k] * A[j * N + k];
+    D[o + x] = 2.0 * acc;
+    Dt[o + x] = (T)(2.0 * acc);
+  }
+}
+
+// Energy terms and convergence control per molecule.  init != 0: record E as
+// E_prev only.  Otherwise: E_last = E, converged when |E - E_prev| < tol;
+// iteration += 1; inactive once converged or at max_its.
+template <typename T>
+__global__ void __launch_bounds__(256) energy_batch_kernel(const BatchMol* __restrict__ mols,
+                                                           const double* __restrict__ e_nuc,
+                                                           const double* __restrict__ D,
+                                                           const double* __restrict__ Hcore,
+                                                           const double* __restrict__ dJK,
+                                                           const double* __restrict__ exc,
+                                                           const double* __restrict__ mo_e, double* __restrict__ ctl,
+                                                           double* __restrict__ E, int init, double tol, int max_its) {
+  __shared__ double red[16];
+  const int m = blockIdx.x;
+  if (!init && ctl[m * CTL + 1] == 0.0) return;
+  if (init && ctl[m * CTL + 2] < 0.0) return;  // failed set-up
+  const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao;
+  const int64_t o = Mo.mat0;
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
+  for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+    const double a = D[o + x] * Hcore[o + x];
+    double t = s1 + a;
+    c1 += fabs(s1) >= fabs(a) ? (s1 - t) + a : (a - t) + s1;
+    s1 = t;
+    const double b = D[o + x] * (double)dJK[o + x];
+    t = s2 + b;
+    c2 += fabs(s2) >= fabs(b) ? (s2 - t) + b : (b - t) + s2;
+    s2 = t;
+  }
+  const double e_core = block_sum(s1 + c1, red);
+  const double e_jk = 0.5 * block_sum(s2 + c2, red);
+  if (threadIdx.x == 0) {
+    const double ex = exc[m];
+    const double tot = e_core + e_jk + ex + e_nuc[m];
+    double* Em = E + (int64_t)m * 5;
+    Em[0] = tot;
+    Em[1] = e_core;
+    Em[2] = e_jk;
+    Em[3] = ex;
+    Em[4] = e_nuc[m];
+    double* c = ctl + m * CTL;
+    if (init) {
+      c[3] = tot;
+      c[4] = tot;
+    } else {
+      const int nocc = (int)Mo.nocc;
+      if (nocc > 0 && nocc < N) c[5] = mo_e[Mo.vec0 + nocc] - mo_e[Mo.vec0 + nocc - 1];
+      const double prev = c[3];
+      c[4] = tot;
+      c[3] = tot;
+      c[0] += 1.0;
+      c[6] = c[0];
+      if (fabs(tot - prev) < tol) {
+        c[2] = 1.0;
+        c[1] = 0.0;
+      } else if ((int)c[0] >= max_its) {
+        c[1] = 0.0;
+      }
+    }
+  }
+}
+
+static size_t scf_lds(int N) { return scf_lds_bytes(N); }
 
 }  // namespace pscf
 
@@ -367,22 +717,16 @@ extern "C" int pscf_scf_step(int dtype, int iteration, int N, int nocc, const do
   PSCF_REQUIRE(d_Hcore && d_S && d_Linv && d_dJK && d_Vxc && d_D && d_H && d_mo_energy && d_C && d_state,
                "pscf_scf_step: null pointer");
   const size_t lds = scf_lds(N);
-  int dev = 0, maxlds = 65536;
-  PSCF_HIP(hipGetDevice(&dev));
-  PSCF_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const int maxlds = device_max_lds();
   PSCF_REQUIRE((int)lds <= maxlds, "pscf_scf_step: N=%d needs %zu B LDS (max %d)", N, lds, maxlds);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == PSCF_F64) {
-    if (lds > 65536)
-      PSCF_HIP(hipFuncSetAttribute((const void*)scf_step_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
+    PSCF_SET_LDS(scf_step_kernel<double>, lds);
     scf_step_kernel<double><<<1, SCF_THREADS, lds, st>>>(iteration, N, nocc, d_Hcore, d_S, d_Linv,
                                                          (const double*)d_dJK, (const double*)d_Vxc, d_D, d_H,
                                                          d_mo_energy, d_C, (double*)d_state, use_diis);
   } else {
-    if (lds > 65536)
-      PSCF_HIP(hipFuncSetAttribute((const void*)scf_step_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
+    PSCF_SET_LDS(scf_step_kernel<float>, lds);
     scf_step_kernel<float><<<1, SCF_THREADS, lds, st>>>(iteration, N, nocc, d_Hcore, d_S, d_Linv, (const float*)d_dJK,
                                                         (const float*)d_Vxc, d_D, d_H, d_mo_energy, d_C,
                                                         (double*)d_state, use_diis);
@@ -400,6 +744,102 @@ extern "C" int pscf_energy(int dtype, int N, const double* d_D, const double* d_
     energy_kernel<double><<<1, 256, 0, st>>>(N, d_D, d_Hcore, (const double*)d_dJK, d_exc, e_nuc, d_out);
   else
     energy_kernel<float><<<1, 256, 0, st>>>(N, d_D, d_Hcore, (const float*)d_dJK, d_exc, e_nuc, d_out);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_chol(const void* d_mols, int nmol, int max_nao, const double* d_S, double* d_Linv,
+                               double* d_ctl, void* stream) {
+  PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_S && d_Linv && d_ctl, "pscf_batch_chol: bad arguments");
+  const size_t lds = sizeof(double) * (size_t)max_nao * max_nao;
+  const int maxlds = device_max_lds();
+  PSCF_REQUIRE((int)lds <= maxlds, "pscf_batch_chol: nao %d too large", max_nao);
+  PSCF_SET_LDS(chol_batch_kernel, lds);
+  chol_batch_kernel<<<nmol, 256, lds, (hipStream_t)stream>>>((const BatchMol*)d_mols, d_S, d_Linv, d_ctl);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_guess(int dtype, const void* d_mols, int nmol, const int* d_atm, const int* d_bas,
+                                const int* d_shell_ao, const double* d_S, double* d_D, void* d_Dt, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_guess: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && d_mols && d_atm && d_bas && d_shell_ao && d_S && d_D, "pscf_batch_guess: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    guess_batch_kernel<double><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_atm, d_bas, d_shell_ao, d_S, d_D,
+                                                     (double*)d_Dt);
+  else
+    guess_batch_kernel<float><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_atm, d_bas, d_shell_ao, d_S, d_D,
+                                                    (float*)d_Dt);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_scf_step(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                                   double* d_ctl, const double* d_Hcore, const double* d_S, const double* d_Linv,
+                                   const double* d_dJK, const void* d_Vxc, double* d_D, void* d_Dt, void* d_dDt,
+                                   double* d_H, double* d_mo_energy, double* d_C, double* d_state, int use_diis,
+                                   void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_scf_step: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && max_nao > 1 && d_mols && d_order && d_ctl && d_Hcore && d_S && d_Linv && d_dJK && d_Vxc &&
+                   d_D && d_Dt && d_H && d_mo_energy && d_C && d_state,
+               "pscf_batch_scf_step: bad arguments");
+  const size_t lds = scf_lds(max_nao);
+  const int maxlds = device_max_lds();
+  PSCF_REQUIRE((int)lds <= maxlds, "pscf_batch_scf_step: nao %d needs %zu B LDS", max_nao, lds);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64) {
+    PSCF_SET_LDS(scf_step_batch_kernel<double>, lds);
+    scf_step_batch_kernel<double><<<nmol, SCF_THREADS, lds, st>>>(
+        (const BatchMol*)d_mols, d_order, d_ctl, d_Hcore, d_S, d_Linv, d_dJK, (const double*)d_Vxc, d_D,
+        (double*)d_Dt, (double*)d_dDt, d_H, d_mo_energy, d_C, d_state, use_diis);
+  } else {
+    PSCF_SET_LDS(scf_step_batch_kernel<float>, lds);
+    scf_step_batch_kernel<float><<<nmol, SCF_THREADS, lds, st>>>(
+        (const BatchMol*)d_mols, d_order, d_ctl, d_Hcore, d_S, d_Linv, d_dJK, (const float*)d_Vxc, d_D,
+        (float*)d_Dt, (float*)d_dDt, d_H, d_mo_energy, d_C, d_state, use_diis);
+  }
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_purify(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                                 const double* d_ctl, const double* d_S, const double* d_Linv, double* d_D, void* d_Dt,
+                                 void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_purify: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && max_nao > 1 && d_mols && d_order && d_ctl && d_S && d_Linv && d_D && d_Dt,
+               "pscf_batch_purify: bad arguments");
+  const size_t lds = scf_lds(max_nao);
+  const int maxlds = device_max_lds();
+  PSCF_REQUIRE((int)lds <= maxlds, "pscf_batch_purify: nao %d needs %zu B LDS", max_nao, lds);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64) {
+    PSCF_SET_LDS(purify_batch_kernel<double>, lds);
+    purify_batch_kernel<double><<<nmol, SCF_THREADS, lds, st>>>((const BatchMol*)d_mols, d_order, d_ctl, d_S, d_Linv,
+                                                                d_D, (double*)d_Dt);
+  } else {
+    PSCF_SET_LDS(purify_batch_kernel<float>, lds);
+    purify_batch_kernel<float><<<nmol, SCF_THREADS, lds, st>>>((const BatchMol*)d_mols, d_order, d_ctl, d_S, d_Linv,
+                                                               d_D, (float*)d_Dt);
+  }
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_energy(int dtype, const void* d_mols, int nmol, const double* d_e_nuc, const double* d_D,
+                                 const double* d_Hcore, const double* d_dJK, const double* d_exc,
+                                 const double* d_mo_energy, double* d_ctl, double* d_E, int init, double tol,
+                                 int max_its, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_energy: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && d_mols && d_e_nuc && d_D && d_Hcore && d_dJK && d_exc && d_mo_energy && d_ctl && d_E,
+               "pscf_batch_energy: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    energy_batch_kernel<double><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_e_nuc, d_D, d_Hcore, d_dJK, d_exc,
+                                                      d_mo_energy, d_ctl, d_E, init, tol, max_its);
+  else
+    energy_batch_kernel<float><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_e_nuc, d_D, d_Hcore, d_dJK, d_exc,
+                                                     d_mo_energy, d_ctl, d_E, init, tol, max_its);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }

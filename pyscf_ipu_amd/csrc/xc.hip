@@ -11,22 +11,12 @@
 // V = phi0^T Y in registers (4x4 register blocks); one partial V per
 // workgroup is reduced in a fixed order by a second kernel.
 #include "b3lyp.h"
+#include "batch.h"
 #include "common.h"
 
 namespace pscf {
 
-static int xc_max_lds() {
-  static int cached = -1;
-  if (cached < 0) {
-    int dev = 0, v = 65536;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
-      cached = v;
-    else
-      cached = 65536;
-  }
-  return cached;
-}
+static int xc_max_lds() { return device_max_lds(); }
 
 // ---------------------------------------------------------------- AO eval ---
 template <typename T>
@@ -155,36 +145,35 @@ struct XCArgs {
 };
 
 // LDS layout: D[N][N] | phi0[N][TILE+1] | XY[N][TILE+1] | coef[4][TILE] | red[8]
+constexpr int XC_MAXT = 2;  // register blocks per thread: NB*NB <= 512 (N <= 88)
+
+// Walks tiles tile0, tile0 + tstride, ... of one molecule's grid, accumulating
+// this thread's 4x4 blocks of V = phi0^T Y in vacc and its share of E_xc in e_loc.
 template <typename T>
-__global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
+                                T (&vacc)[XC_MAXT][16], double& e_loc) {
   const int N = a.N, TP = XC_TILE + 1;
   T* D = reinterpret_cast<T*>(smem);
   T* phi0 = D + N * N;
   T* XY = phi0 + N * TP;
   T* coef = XY + N * TP;
-  double* red = reinterpret_cast<double*>(coef + 4 * XC_TILE);
+  __syncthreads();
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) D[x] = a.dm[x];
   const int tid = threadIdx.x;
   const int gl = tid & (XC_TILE - 1);  // point within tile
   const int sl = tid / XC_TILE;        // slice 0..3
-  // V register blocks: 4x4 tiles over padded N, tiles t = tid, tid + 256, ...
   const int NB = (N + 3) / 4;
   const int ntile = NB * NB;
-  constexpr int MAXT = 2;  // supports NB*NB <= 512 (N <= 88)
-  T vacc[MAXT][16];
 #pragma unroll
-  for (int u = 0; u < MAXT; ++u)
+  for (int u = 0; u < XC_MAXT; ++u)
 #pragma unroll
     for (int k = 0; k < 16; ++k) vacc[u][k] = T(0);
-  double e_loc = 0.0;
   const int ntiles = (a.g + XC_TILE - 1) / XC_TILE;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int tile = tile0; tile < ntiles; tile += tstride) {
     const int p0 = tile * XC_TILE;
     const int pt = p0 + gl;
     const bool valid = pt < a.g;
     __syncthreads();
-    // stage phi0 tile
     for (int mu = sl; mu < N; mu += 4) phi0[mu * TP + gl] = valid ? a.ao[mu * a.smu + (int64_t)pt * a.sp] : T(0);
     __syncthreads();
     // X[nu][p] = sum_mu phi0[mu][p] D[mu][nu]  (slice sl owns nu = sl, sl+4, ...)
@@ -243,7 +232,7 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
     __syncthreads();
     // V[mu][nu] += sum_p phi0[mu][p] Y[nu][p]
 #pragma unroll
-    for (int u = 0; u < MAXT; ++u) {
+    for (int u = 0; u < XC_MAXT; ++u) {
       const int t = tid + u * XC_THREADS;
       if (t < ntile) {
         const int bm = (t / NB) * 4, bn = (t % NB) * 4;
@@ -262,11 +251,15 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
       }
     }
   }
-  // write this block's partial V and E
-  T* dst = a.part_v + (int64_t)blockIdx.x * N * N;
+}
+
+// Stores this thread's register blocks of V into dst (row stride N).
+template <typename T>
+__device__ inline void xc_store_v(const T (&vacc)[XC_MAXT][16], int N, T* dst) {
+  const int NB = (N + 3) / 4, ntile = NB * NB;
 #pragma unroll
-  for (int u = 0; u < MAXT; ++u) {
-    const int t = tid + u * XC_THREADS;
+  for (int u = 0; u < XC_MAXT; ++u) {
+    const int t = threadIdx.x + u * XC_THREADS;
     if (t < ntile) {
       const int bm = (t / NB) * 4, bn = (t % NB) * 4;
 #pragma unroll
@@ -276,29 +269,220 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
           if (bm + i < N && bn + j < N) dst[(bm + i) * N + bn + j] = vacc[u][i * 4 + j];
     }
   }
-  // block reduction of E (only slice-0 lanes carry values)
+}
+
+__device__ inline double xc_block_energy(double e_loc, double* red) {
   for (int off = 32; off > 0; off >>= 1) e_loc += __shfl_xor(e_loc, off, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = e_loc;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e_loc;
   __syncthreads();
-  if (tid == 0) {
-    double s = 0.0;
-    for (int k = 0; k < XC_THREADS / 64; ++k) s += red[k];
-    a.part_e[blockIdx.x] = s;
-  }
+  double s = 0.0;
+  for (int k = 0; k < XC_THREADS / 64; ++k) s += red[k];
+  return s;
 }
 
 template <typename T>
+__global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T vacc[XC_MAXT][16];
+  double e_loc = 0.0;
+  xc_tiles<T>(a, smem, blockIdx.x, gridDim.x, vacc, e_loc);
+  xc_store_v<T>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N);
+  double* red = reinterpret_cast<double*>(smem);
+  __syncthreads();
+  const double e = xc_block_energy(e_loc, red);
+  if (threadIdx.x == 0) a.part_e[blockIdx.x] = e;
+}
+
+// Batched: one workgroup per molecule (order[blockIdx.x]); writes V + V^T and
+// E_xc of that molecule; skips converged molecules.
+template <typename T>
+__global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __restrict__ mols,
+                                                              const int* __restrict__ order,
+                                                              const double* __restrict__ ctl, const T* __restrict__ ao,
+                                                              const T* __restrict__ dm, const T* __restrict__ w,
+                                                              T eps, T* __restrict__ vxc, double* __restrict__ exc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = order[blockIdx.x];
+  if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
+  const BatchMol Mo = mols[m];
+  XCArgs<T> a;
+  a.N = (int)Mo.nao;
+  a.g = (int)Mo.ngrid;
+  a.ao = ao + Mo.ao0;
+  a.sc = (int64_t)a.N * a.g;
+  a.smu = a.g;
+  a.sp = 1;
+  a.dm = dm + Mo.mat0;
+  a.w = w + Mo.grid0;
+  a.eps = eps;
+  T vacc[XC_MAXT][16];
+  double e_loc = 0.0;
+  xc_tiles<T>(a, smem, 0, 1, vacc, e_loc);
+  const int N = a.N;
+  T* Vl = reinterpret_cast<T*>(smem);
+  __syncthreads();
+  xc_store_v<T>(vacc, N, Vl);
+  __syncthreads();
+  T* out = vxc + Mo.mat0;
+  for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    out[x] = Vl[x] + Vl[j * N + i];
+  }
+  double* red = reinterpret_cast<double*>(smem + sizeof(T) * N * N);
+  __syncthreads();
+  const double e = xc_block_energy(e_loc, red);
+  if (threadIdx.x == 0) exc[m] = e;
+}
+
+// Grid points of one atom's template: chunk (atom, first point) per workgroup.
+__global__ void __launch_bounds__(256) grid_expand_kernel(const int* __restrict__ work, const int* __restrict__ atom_elem,
+                                                          const int64_t* __restrict__ atom_pt0,
+                                                          const int* __restrict__ atom_mol,
+                                                          const int* __restrict__ tmpl_off,
+                                                          const int* __restrict__ tmpl_n,
+                                                          const double* __restrict__ tmpl, const int* __restrict__ atm,
+                                                          const double* __restrict__ env, double* __restrict__ coords,
+                                                          double* __restrict__ vol, int* __restrict__ owner,
+                                                          int* __restrict__ ptmol) {
+  const int a = work[2 * blockIdx.x];
+  const int i = work[2 * blockIdx.x + 1] + threadIdx.x;
+  const int e = atom_elem[a];
+  if (i >= tmpl_n[e]) return;
+  const double* t = tmpl + 4 * ((int64_t)tmpl_off[e] + i);
+  const double* R = env + atm[a * ATM_SLOTS + 1];
+  const int64_t g = atom_pt0[a] + i;
+  coords[3 * g] = R[0] + t[0];
+  coords[3 * g + 1] = R[1] + t[1];
+  coords[3 * g + 2] = R[2] + t[2];
+  vol[g] = t[3];
+  owner[g] = a;
+  ptmol[g] = atom_mol[a];
+}
+
+// Bragg-Slater radii (Bohr), H..Ne, for the Treutler adjustment.
+__constant__ double c_bragg[11] = {0.0,
+                                   0.35 / 0.52917721092, 1.40 / 0.52917721092, 1.45 / 0.52917721092,
+                                   1.05 / 0.52917721092, 0.85 / 0.52917721092, 0.70 / 0.52917721092,
+                                   0.65 / 0.52917721092, 0.60 / 0.52917721092, 0.50 / 0.52917721092,
+                                   1.50 / 0.52917721092};
+
+// Becke weights of all points of all molecules; each point's cell function runs
+// over its own molecule's atoms only.  T: arithmetic (FP64 or FP32).
+template <typename T>
+__global__ void __launch_bounds__(256) becke_batch_kernel(const BatchMol* __restrict__ mols,
+                                                          const int* __restrict__ ptmol,
+                                                          const int* __restrict__ owner,
+                                                          const double* __restrict__ coords,
+                                                          const double* __restrict__ vol, int64_t G,
+                                                          const int* __restrict__ atm, const double* __restrict__ env,
+                                                          T* __restrict__ w, double* __restrict__ w64) {
+  const int64_t pt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= G) return;
+  const BatchMol Mo = mols[ptmol[pt]];
+  const int a0 = (int)Mo.atm0, na = (int)Mo.natm;
+  const T x = (T)coords[3 * pt], y = (T)coords[3 * pt + 1], z = (T)coords[3 * pt + 2];
+  const int own = owner[pt];
+  T psum = T(0), pown = T(0);
+  for (int i = a0; i < a0 + na; ++i) {
+    const double* Ri = env + atm[i * ATM_SLOTS + 1];
+    const T dxi = x - (T)Ri[0], dyi = y - (T)Ri[1], dzi = z - (T)Ri[2];
+    const T ri = sqrt(dxi * dxi + dyi * dyi + dzi * dzi);
+    const T radi = sqrt((T)c_bragg[atm[i * ATM_SLOTS]]);
+    T pi = T(1);
+    for (int j = a0; j < a0 + na; ++j) {
+      if (j == i) continue;
+      const double* Rj = env + atm[j * ATM_SLOTS + 1];
+      const T dxj = x - (T)Rj[0], dyj = y - (T)Rj[1], dzj = z - (T)Rj[2];
+      const T rj = sqrt(dxj * dxj + dyj * dyj + dzj * dzj);
+      const T ex = (T)(Ri[0] - Rj[0]), ey = (T)(Ri[1] - Rj[1]), ez = (T)(Ri[2] - Rj[2]);
+      const T radj = sqrt((T)c_bragg[atm[j * ATM_SLOTS]]);
+      T aij = T(0.25) * (radj / radi - radi / radj);
+      aij = aij < T(-0.5) ? T(-0.5) : (aij > T(0.5) ? T(0.5) : aij);
+      T gg = (ri - rj) / sqrt(ex * ex + ey * ey + ez * ez);
+      gg = gg + aij * (T(1) - gg * gg);
+      gg = (T(3) - gg * gg) * gg * T(0.5);
+      gg = (T(3) - gg * gg) * gg * T(0.5);
+      gg = (T(3) - gg * gg) * gg * T(0.5);
+      pi *= T(0.5) * (T(1) - gg);
+    }
+    psum += pi;
+    if (i == own) pown = pi;
+  }
+  const double wt = vol[pt] * (double)(pown / psum);
+  w[pt] = (T)wt;
+  if (w64) w64[pt] = wt;
+}
+
+// AO values/gradients of all points; molecule m's block is ao[ao0 + (c*N + mu)*g + p].
+template <typename T>
+__global__ void __launch_bounds__(256) ao_batch_kernel(const BatchMol* __restrict__ mols, const int* __restrict__ ptmol,
+                                                       const double* __restrict__ coords, int64_t G,
+                                                       const int* __restrict__ atm, const int* __restrict__ bas,
+                                                       const double* __restrict__ env, T* __restrict__ ao) {
+  const int64_t pt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= G) return;
+  const BatchMol Mo = mols[ptmol[pt]];
+  const int64_t g = Mo.ngrid, lp = pt - Mo.grid0;
+  const int64_t cs = Mo.nao * g;
+  T* base = ao + Mo.ao0;
+  const double x = coords[3 * pt], y = coords[3 * pt + 1], z = coords[3 * pt + 2];
+  int mu = 0;
+  for (int ib = (int)Mo.bas0; ib < (int)(Mo.bas0 + Mo.nbas); ++ib) {
+    const int* b = bas + ib * BAS_SLOTS;
+    const double* R = env + atm[b[0] * ATM_SLOTS + 1];
+    const int l = b[1], np = b[2];
+    const double* e = env + b[5];
+    const double* c = env + b[6];
+    const double dx = x - R[0], dy = y - R[1], dz = z - R[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double rad = 0.0, drad = 0.0;
+    for (int k = 0; k < np; ++k) {
+      const double ek = c[k] * exp(-e[k] * r2);
+      rad += ek;
+      drad += e[k] * ek;
+    }
+    drad *= -2.0;
+    if (l == 0) {
+      rad *= ANG_S;
+      drad *= ANG_S;
+      base[mu * g + lp] = (T)rad;
+      base[cs + mu * g + lp] = (T)(drad * dx);
+      base[2 * cs + mu * g + lp] = (T)(drad * dy);
+      base[3 * cs + mu * g + lp] = (T)(drad * dz);
+      mu += 1;
+    } else {
+      rad *= ANG_P;
+      drad *= ANG_P;
+      const double d[3] = {dx, dy, dz};
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa) {
+        const int64_t o = (int64_t)(mu + aa) * g + lp;
+        base[o] = (T)(d[aa] * rad);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) base[(1 + q) * cs + o] = (T)((aa == q ? rad : 0.0) + d[aa] * d[q] * drad);
+      }
+      mu += 3;
+    }
+  }
+}
+
+// V_xc = sum_b (part[b] + part[b]^T), one wave per element (lanes over the
+// partials, fixed shuffle tree); the last block also sums the energy partials.
+template <typename T>
 __global__ void xc_reduce_kernel(const T* __restrict__ part_v, const double* __restrict__ part_e, int nb, int N,
                                  T* __restrict__ vxc, double* __restrict__ exc) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int NN = N * N;
+  const int x = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (x < NN) {
     const int i = x / N, j = x - i * N;
     T s = T(0);
-    for (int b = 0; b < nb; ++b) s += part_v[(int64_t)b * NN + x] + part_v[(int64_t)b * NN + j * N + i];
-    vxc[x] = s;
+    for (int b = lane; b < nb; b += 64) s += part_v[(int64_t)b * NN + x] + part_v[(int64_t)b * NN + j * N + i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) vxc[x] = s;
   }
-  if (x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     double s = 0.0, c = 0.0;  // Neumaier
     for (int b = 0; b < nb; ++b) {
       const double v = part_e[b], t = s + v;
@@ -341,11 +525,10 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
   a.eps = (T)eps;
   a.part_v = (T*)ws;
   a.part_e = (double*)((char*)ws + sizeof(T) * (size_t)nb * N * N);
-  if (lds > 65536)
-    PSCF_HIP(hipFuncSetAttribute((const void*)xc_fused_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  PSCF_SET_LDS(xc_fused_kernel<T>, lds);
   xc_fused_kernel<T><<<nb, XC_THREADS, lds, st>>>(a);
   PSCF_HIP(hipGetLastError());
-  xc_reduce_kernel<T><<<(N * N + 255) / 256, 256, 0, st>>>(a.part_v, a.part_e, nb, N, vxc, exc);
+  xc_reduce_kernel<T><<<(N * N + 3) / 4, 256, 0, st>>>(a.part_v, a.part_e, nb, N, vxc, exc);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
@@ -428,4 +611,80 @@ extern "C" int pscf_xc(int dtype, const void* d_ao, int64_t ao_sc, int64_t ao_sm
                            (const double*)d_weights, eps, d_exc, (double*)d_vxc, d_ws, ws_bytes, st);
   return xc_impl<float>((const float*)d_ao, ao_sc, ao_smu, ao_sp, g, N, (const float*)d_dm, (const float*)d_weights,
                         eps, d_exc, (float*)d_vxc, d_ws, ws_bytes, st);
+}
+
+extern "C" int pscf_batch_grid(const int* d_work, int nwork, const int* d_atom_elem, const int64_t* d_atom_pt0,
+                               const int* d_atom_mol, const int* d_tmpl_off, const int* d_tmpl_n, const double* d_tmpl,
+                               const int* d_atm, const double* d_env, double* d_coords, double* d_vol, int* d_owner,
+                               int* d_ptmol, void* stream) {
+  PSCF_REQUIRE(nwork >= 0, "pscf_batch_grid: nwork < 0");
+  if (nwork == 0) return PSCF_OK;
+  grid_expand_kernel<<<nwork, 256, 0, (hipStream_t)stream>>>(d_work, d_atom_elem, d_atom_pt0, d_atom_mol, d_tmpl_off,
+                                                            d_tmpl_n, d_tmpl, d_atm, d_env, d_coords, d_vol, d_owner,
+                                                            d_ptmol);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_becke(int dtype, const void* d_mols, const int* d_ptmol, const int* d_owner,
+                                const double* d_coords, const double* d_vol, int64_t G, const int* d_atm,
+                                const double* d_env, void* d_w, double* d_w64, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_becke: bad dtype");
+  PSCF_REQUIRE(G >= 0, "pscf_batch_becke: G < 0");
+  if (G == 0) return PSCF_OK;
+  const int64_t blocks = (G + 255) / 256;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    becke_batch_kernel<double><<<(unsigned)blocks, 256, 0, st>>>((const BatchMol*)d_mols, d_ptmol, d_owner, d_coords,
+                                                                 d_vol, G, d_atm, d_env, (double*)d_w, d_w64);
+  else
+    becke_batch_kernel<float><<<(unsigned)blocks, 256, 0, st>>>((const BatchMol*)d_mols, d_ptmol, d_owner, d_coords,
+                                                                d_vol, G, d_atm, d_env, (float*)d_w, d_w64);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, const double* d_coords, int64_t G,
+                             const int* d_atm, const int* d_bas, const double* d_env, void* d_ao, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_ao: bad dtype");
+  PSCF_REQUIRE(G >= 0, "pscf_batch_ao: G < 0");
+  if (G == 0) return PSCF_OK;
+  const int64_t blocks = (G + 255) / 256;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    ao_batch_kernel<double><<<(unsigned)blocks, 256, 0, st>>>((const BatchMol*)d_mols, d_ptmol, d_coords, G, d_atm,
+                                                              d_bas, d_env, (double*)d_ao);
+  else
+    ao_batch_kernel<float><<<(unsigned)blocks, 256, 0, st>>>((const BatchMol*)d_mols, d_ptmol, d_coords, G, d_atm,
+                                                             d_bas, d_env, (float*)d_ao);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                             const double* d_ctl, const void* d_ao, const void* d_dm, const void* d_w, double eps,
+                             void* d_vxc, double* d_exc, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_xc: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && d_ao && d_dm && d_w && d_vxc && d_exc,
+               "pscf_batch_xc: bad arguments");
+  const int NB = (max_nao + 3) / 4;
+  PSCF_REQUIRE(NB * NB <= XC_MAXT * XC_THREADS, "pscf_batch_xc: nao %d > 88", max_nao);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64) {
+    const size_t lds = xc_lds<double>(max_nao);
+    PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
+    PSCF_SET_LDS(xc_batch_kernel<double>, lds);
+    xc_batch_kernel<double><<<nmol, XC_THREADS, lds, st>>>((const BatchMol*)d_mols, d_order, d_ctl,
+                                                           (const double*)d_ao, (const double*)d_dm,
+                                                           (const double*)d_w, eps, (double*)d_vxc, d_exc);
+  } else {
+    const size_t lds = xc_lds<float>(max_nao);
+    PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
+    PSCF_SET_LDS(xc_batch_kernel<float>, lds);
+    xc_batch_kernel<float><<<nmol, XC_THREADS, lds, st>>>((const BatchMol*)d_mols, d_order, d_ctl,
+                                                          (const float*)d_ao, (const float*)d_dm, (const float*)d_w,
+                                                          (float)eps, (float*)d_vxc, d_exc);
+  }
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
 }

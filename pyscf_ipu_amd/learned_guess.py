@@ -1,0 +1,291 @@
+"""Learned initial guess for the batched STO-3G SCF.
+
+The reference always starts from PySCF's MINAO guess
+(pyscf_ipu/nanoDFT/nanoDFT.py:233, pyscf_ipu/pyscf_utils/minao.py:5-123).
+Here a small network predicts the converged density block of every atom pair
+within 5 Angstrom:
+
+  features  one-hot(Z_a) ++ one-hot(Z_b) over H/C/N/O, 32 Gaussian RBFs of the
+            distance (centres 0..5 Angstrom, sigma 0.15 Angstrom), unit vector a->b
+  network   43 -> 256 -> 256 -> 256 -> 25, SiLU, BF16 MFMA with FP32
+            accumulation, FP32 master weights, Adam
+  output    the 5x5 block D[AOs of a, AOs of b] (rows/cols beyond an atom's
+            AO count unused)
+
+The assembled matrix is symmetrised and purified on the GPU
+(`pscf_batch_purify`: exactly nocc doubly occupied natural orbitals, D S D = 2D).
+Molecules that do not converge within 15 iterations from the learned guess
+are re-run from the MINAO guess (`solve`).  Every kernel -- features, the four
+GEMMs forward and backward, loss, bias gradients, Adam, target extraction,
+assembly, purification -- is the library's own HIP code; PyTorch only holds
+the buffers.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+from typing import Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib, device
+from .batch import BatchRKS, BatchResult
+
+NIN, NIN_PAD, NHID, NOUT, NOUT_PAD = 43, 64, 256, 25, 32
+RCUT_BOHR = 5.0 / 0.52917721092
+AO_PER_ATOM = {1: 1, 6: 5, 7: 5, 8: 5}
+BF16 = torch.bfloat16
+# flat FP32 parameter layout: (name, rows, cols); W0 rows >= 43 and W3/b3
+# columns >= 25 are padding and stay exactly zero (their gradients are zero)
+LAYOUT = [("W0", NIN_PAD, NHID), ("b0", 1, NHID), ("W1", NHID, NHID), ("b1", 1, NHID),
+          ("W2", NHID, NHID), ("b2", 1, NHID), ("W3", NHID, NOUT_PAD), ("b3", 1, NOUT_PAD)]
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "pair_mlp_sto3g.npz")
+
+
+def _offsets():
+    off, out = 0, {}
+    for name, r, c in LAYOUT:
+        out[name] = (off, r, c)
+        off += r * c
+    return out, off
+
+
+OFFSETS, NPARAM = _offsets()
+
+
+def pair_lists(b: BatchRKS, molecules: np.ndarray = None) -> Tuple[np.ndarray, np.ndarray]:
+    """Ordered atom pairs (a, b) of each molecule (a == b included) closer than
+    5 Angstrom, as global atom indices of the batch, and their block info
+    (molecule, AO offset of a, AO count of a, AO offset of b, AO count of b).
+    `molecules` restricts the list to those molecule indices."""
+    R = b.env[int(b.atm[0, 1]):].reshape(-1, 3) if b.A else np.zeros((0, 3))
+    Z = b.atm[:, 0]
+    pairs, info = [], []
+    mol_sel = np.ones(b.M, bool) if molecules is None else np.isin(np.arange(b.M), molecules)
+    atm0 = np.cumsum(b.natm) - b.natm
+    for na in np.unique(b.natm):
+        ms = np.where((b.natm == na) & mol_sel)[0]
+        if len(ms) == 0:
+            continue
+        idx = atm0[ms][:, None] + np.arange(na)[None, :]               # (k, na)
+        r = R[idx]
+        d = np.linalg.norm(r[:, :, None] - r[:, None, :], axis=-1)      # (k, na, na)
+        k, ia, ib = np.nonzero(d < RCUT_BOHR)
+        ga, gb = idx[k, ia], idx[k, ib]
+        pairs.append(np.stack([ga, gb], axis=1))
+        info.append(np.stack([ms[k], b.atom_ao0[ga], b.atom_nao[ga], b.atom_ao0[gb], b.atom_nao[gb]], axis=1))
+    if not pairs:
+        return np.zeros((0, 2), np.int32), np.zeros((0, 5), np.int32)
+    assert set(np.unique(Z).tolist()) <= set(AO_PER_ATOM), "learned guess covers H, C, N, O"
+    return np.concatenate(pairs).astype(np.int32), np.concatenate(info).astype(np.int32)
+
+
+def features(b: BatchRKS, pairs: np.ndarray, fp32_copy: bool = False):
+    """Device features for `pairs` of batch `b` (whose set-up has run)."""
+    P = len(pairs)
+    X16 = device.empty((max(P, 1), NIN_PAD), BF16)
+    X32 = device.empty((max(P, 1), NIN_PAD), torch.float32) if fp32_copy else None
+    valid = device.empty(max(P, 1), torch.uint8)
+    d_pairs = device.to_dev(pairs)
+    _lib.call("pscf_pair_features", P, device.ptr(d_pairs), device.ptr(b.d_atm), device.ptr(b.d_env),
+              device.ptr(X16), device.ptr(X32), device.ptr(valid), device.stream())
+    return X16, X32, valid
+
+
+class PairMLP:
+    """The pair network: FP32 master weights + BF16 working copy on the GPU."""
+
+    def __init__(self, params: np.ndarray = None, seed: int = 0):
+        if params is None:
+            params = init_params(seed)
+        assert params.shape == (NPARAM,)
+        self.w = device.to_dev(params.astype(np.float32))
+        self.w16 = device.empty(NPARAM, BF16)
+        _lib.call("pscf_mlp_to_bf16", NPARAM, device.ptr(self.w), device.ptr(self.w16), device.stream())
+        self.m = torch.zeros_like(self.w)
+        self.v = torch.zeros_like(self.w)
+        self.g = torch.zeros_like(self.w)
+        self.step = 0
+
+    # ---------------------------------------------------------------- access --
+    def _p(self, name, t=None):
+        off, _, _ = OFFSETS[name]
+        t = self.w if t is None else t
+        return ctypes_ptr(t, off)
+
+    def numpy(self) -> np.ndarray:
+        return self.w.cpu().numpy().copy()
+
+    def save(self, path: str, meta: dict = None):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez(path, params=self.numpy(), **{k: np.asarray(v) for k, v in (meta or {}).items()})
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_WEIGHTS) -> "PairMLP":
+        with np.load(path, allow_pickle=False) as f:
+            return cls(f["params"].astype(np.float32))
+
+    # --------------------------------------------------------------- compute --
+    def _buffers(self, B: int):
+        if getattr(self, "_bufB", 0) < B:
+            f32 = torch.float32
+            self._bufB = B
+            self.Z = [device.empty((B, NHID), f32) for _ in range(3)]
+            self.H = [device.empty((B, NHID), BF16) for _ in range(3)]
+            self.out = device.empty((B, NOUT_PAD), f32)
+            self.g32 = device.empty((B, NOUT_PAD), f32)
+            self.g16 = device.empty((B, NOUT_PAD), BF16)
+            self.dZ = device.empty((B, NHID), f32)
+            self.dZ16 = [device.empty((B, NHID), BF16) for _ in range(2)]
+            self.loss = device.zeros(1, torch.float64)
+
+    def forward(self, X16: torch.Tensor, B: int, row0: int = 0) -> torch.Tensor:
+        """Predictions [B][32] (FP32) for feature rows row0..row0+B of X16."""
+        self._buffers(B)
+        st = device.stream()
+        x = ctypes_ptr(X16, row0 * NIN_PAD)
+        G = lambda *a: _lib.call("pscf_mlp_gemm", *a)
+        G(0, B, NHID, NIN, x, NIN_PAD, self._p("W0", self.w16), NHID, self._p("b0"), None, device.ptr(self.Z[0]),
+          device.ptr(self.H[0]), NHID, 1, st)
+        for i in (1, 2):
+            G(0, B, NHID, NHID, device.ptr(self.H[i - 1]), NHID, self._p(f"W{i}", self.w16), NHID, self._p(f"b{i}"),
+              None, device.ptr(self.Z[i]), device.ptr(self.H[i]), NHID, 1, st)
+        G(1, B, NOUT_PAD, NHID, device.ptr(self.H[2]), NHID, self._p("W3", self.w16), NOUT_PAD, self._p("b3"), None,
+          device.ptr(self.out), None, NOUT_PAD, 1, st)
+        return self.out[:B]
+
+    def loss_and_grads(self, X16, Y, mask, B: int, row0: int = 0, nvalid: float = None):
+        """Forward + backward on rows row0..row0+B; gradients into self.g
+        (flat, same layout as the weights); returns the loss as a device scalar."""
+        self.forward(X16, B, row0)
+        st = device.stream()
+        P = device.ptr
+        if nvalid is None:
+            nvalid = float(mask[row0:row0 + B].sum().item())
+        self.g.zero_()
+        self.loss.zero_()
+        ksplit = max(1, min(64, B // 512))
+        G = lambda *a: _lib.call("pscf_mlp_gemm", *a)
+        _lib.call("pscf_mlp_mse_grad", B, P(self.out), ctypes_ptr(Y, row0 * NOUT_PAD), ctypes_ptr(mask, row0 * NOUT_PAD),
+                  1.0 / max(nvalid, 1.0), P(self.g32), P(self.g16), P(self.loss), st)
+        # output layer
+        G(3, NHID, NOUT_PAD, B, P(self.H[2]), NHID, P(self.g16), NOUT_PAD, None, None, self._p("W3", self.g), None,
+          NOUT_PAD, ksplit, st)
+        _lib.call("pscf_mlp_colsum", B, NOUT_PAD, P(self.g32), NOUT_PAD, self._p("b3", self.g), st)
+        up16, up_ld, Wname = self.g16, NOUT_PAD, "W3"
+        for i in (2, 1, 0):
+            dz16 = self.dZ16[i % 2]
+            # dZ_i = (dZ_{i+1} W_{i+1}^T) * silu'(Z_i)
+            G(2, B, NHID, up_ld, P(up16), up_ld, self._p(Wname, self.w16), up_ld, None, P(self.Z[i]), P(self.dZ),
+              P(dz16), NHID, 1, st)
+            _lib.call("pscf_mlp_colsum", B, NHID, P(self.dZ), NHID, self._p(f"b{i}", self.g), st)
+            if i > 0:
+                G(3, NHID, NHID, B, P(self.H[i - 1]), NHID, P(dz16), NHID, None, None, self._p(f"W{i}", self.g), None,
+                  NHID, ksplit, st)
+            else:
+                G(3, NIN, NHID, B, ctypes_ptr(X16, row0 * NIN_PAD), NIN_PAD, P(dz16), NHID, None, None,
+                  self._p("W0", self.g), None, NHID, ksplit, st)
+            up16, up_ld, Wname = dz16, NHID, f"W{i}"
+        return self.loss
+
+    def adam(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+        self.step += 1
+        _lib.call("pscf_mlp_adam", NPARAM, device.ptr(self.w), device.ptr(self.w16), device.ptr(self.g),
+                  device.ptr(self.m), device.ptr(self.v), lr, b1, b2, eps, self.step, device.stream())
+
+
+def ctypes_ptr(t: torch.Tensor, elem_offset: int = 0):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr() + elem_offset * t.element_size())
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights, zero biases, zero padding (flat FP32 layout)."""
+    rng = np.random.default_rng(seed)
+    flat = np.zeros(NPARAM, np.float32)
+    fan = {"W0": NIN, "W1": NHID, "W2": NHID, "W3": NHID}
+    for name, (off, r, c) in OFFSETS.items():
+        if name.startswith("W"):
+            w = np.zeros((r, c))
+            rr = NIN if name == "W0" else r
+            cc = NOUT if name == "W3" else c
+            w[:rr, :cc] = rng.normal(size=(rr, cc)) * np.sqrt(2.0 / fan[name])
+            flat[off:off + r * c] = w.reshape(-1)
+    return flat
+
+
+def unflatten(flat: np.ndarray):
+    """Flat layout -> [W0 (43,256), b0, W1, b1, W2, b2, W3 (256,25), b3 (25,)]."""
+    out = []
+    for name, (off, r, c) in OFFSETS.items():
+        a = flat[off:off + r * c].reshape(r, c).astype(np.float64)
+        if name == "W0":
+            a = a[:NIN]
+        if name in ("W3", "b3"):
+            a = a[:, :NOUT]
+        out.append(a[0] if name.startswith("b") else a)
+    return out
+
+
+# ------------------------------------------------------------------ guess ----
+def learned_guess(model: PairMLP):
+    """A guess function for BatchRKS.setup: predict, assemble, purify."""
+
+    def fill(b: BatchRKS):
+        pairs, info = pair_lists(b)
+        P = len(pairs)
+        X16, _, valid = features(b, pairs)
+        d_info = device.to_dev(info)
+        b.D.zero_()
+        st = device.stream()
+        chunk = 1 << 18
+        for r0 in range(0, P, chunk):
+            n = min(chunk, P - r0)
+            pred = model.forward(X16, n, r0)
+            _lib.call("pscf_pair_assemble", n, ctypes_ptr(d_info, 5 * r0), ctypes_ptr(valid, r0),
+                      device.ptr(b.d_desc), device.ptr(pred), device.ptr(b.D), st)
+        _lib.call("pscf_batch_purify", b.code, device.ptr(b.d_desc), device.ptr(b.d_order), b.M, b.max_nao,
+                  device.ptr(b.ctl), device.ptr(b.S), device.ptr(b.Linv), device.ptr(b.D), device.ptr(b.Dt), st)
+        b._guess_keep = (X16, valid, d_info)
+
+    return fill
+
+
+@dataclass
+class GuessedResult:
+    result: BatchResult          # merged per-molecule result
+    fallback: np.ndarray         # (M,) bool: re-run from MINAO
+    iterations_total: np.ndarray  # learned-guess iterations + fallback iterations
+
+
+def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_its: int = 20,
+          learned_its: int = 15, conv_tol: float = 1e-7, graph: bool = True) -> GuessedResult:
+    """Batched SCF from the learned guess; molecules not converged after
+    `learned_its` iterations are re-run from MINAO with `max_its` iterations."""
+    b = BatchRKS(molecules, dtype=dtype)
+    if model is not None:
+        b.setup(guess=learned_guess(model))
+        r = b.run(max_its=learned_its, conv_tol=conv_tol, graph=graph)
+    else:
+        b.setup()
+        r = b.run(max_its=max_its, conv_tol=conv_tol, graph=graph)
+        return GuessedResult(r, np.zeros(len(molecules), bool), r.iterations.copy())
+    redo = np.where(~r.converged & ~r.failed)[0]
+    total = r.iterations.copy()
+    fb = np.zeros(len(molecules), bool)
+    if len(redo):
+        del b
+        b2 = BatchRKS([molecules[i] for i in redo], dtype=dtype)
+        r2 = b2.run(max_its=max_its, conv_tol=conv_tol, graph=graph)
+        for k, i in enumerate(redo):
+            r.energy[i] = r2.energy[k]
+            r.terms[i] = r2.terms[k]
+            r.converged[i] = r2.converged[k]
+            r.failed[i] = r2.failed[k]
+            r.hlgap[i] = r2.hlgap[k]
+            r.iterations[i] = r2.iterations[k]
+            total[i] += r2.iterations[k]
+        fb[redo] = True
+        r.n_iter_launches += r2.n_iter_launches
+    return GuessedResult(r, fb, total)

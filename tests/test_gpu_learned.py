@@ -1,0 +1,203 @@
+"""Learned initial guess on the GPU: kernels against the FP64 oracle
+(oracle/mlp.py, itself pinned by finite differences) and against a numpy
+model of the same BF16 roundings; purification invariants; the trained
+network's gain over the MINAO guess."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as omlp
+from pyscf_ipu_amd import _lib, batch, device, learned_guess as lg, molgen
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def emulated_forward(parts, X):
+    """Forward pass with the kernels' roundings: BF16 inputs/weights/hidden
+    activations, wide accumulation."""
+    h = bf16(X)
+    pre = []
+    for i in range(4):
+        z = h @ bf16(parts[2 * i]) + np.asarray(parts[2 * i + 1], np.float32)
+        pre.append(z)
+        h = bf16(omlp.silu(z)) if i < 3 else z
+    return h, pre
+
+
+def _problem(nmol=6, seed=5, dtype="fp32"):
+    mols = batch.parse(molgen.batched_set(nmol, (1, 6), seed=seed))
+    b = batch.BatchRKS(mols, dtype=dtype)
+    b.setup()
+    pairs, info = lg.pair_lists(b)
+    return mols, b, pairs, info
+
+
+def test_pair_features_match_oracle(hip):
+    mols, b, pairs, info = _problem()
+    X16, X32, valid = lg.features(b, pairs, fp32_copy=True)
+    torch.cuda.synchronize()
+    X = X32.cpu().numpy()
+    atm0 = np.cumsum(b.natm) - b.natm
+    row = 0
+    for m, (Z, R) in enumerate(mols):
+        sel = np.where(info[:, 0] == m)[0]
+        op, OX, _ = omlp.pair_features(Z, R)
+        key = {tuple(p): k for k, p in enumerate(op.tolist())}
+        for r in sel:
+            a, c = pairs[r] - atm0[m]
+            np.testing.assert_allclose(X[r, :43], OX[key[(a, c)]], atol=2e-6)
+            assert np.all(X[r, 43:] == 0)
+    assert valid.cpu().numpy()[:len(pairs)].all()
+    np.testing.assert_array_equal(X16.float().cpu().numpy()[:len(pairs)], bf16(X[:len(pairs)]))
+
+
+@pytest.fixture(scope="module")
+def random_batch():
+    rng = np.random.default_rng(11)
+    B = 1000
+    X = np.zeros((B, lg.NIN_PAD), np.float32)
+    X[:, :lg.NIN] = rng.normal(size=(B, lg.NIN))
+    Y = np.zeros((B, lg.NOUT_PAD), np.float32)
+    mask = np.zeros((B, lg.NOUT_PAD), np.float32)
+    Y[:, :25] = rng.normal(size=(B, 25)) * 0.5
+    mask[:, :25] = rng.random((B, 25)) < 0.7
+    flat = lg.init_params(2)
+    for name in ("b0", "b1", "b2", "b3"):
+        off, r, c = lg.OFFSETS[name]
+        n = 25 if name == "b3" else c
+        flat[off:off + n] = rng.normal(size=n) * 0.1
+    return X, Y, mask, flat
+
+
+def test_mlp_forward_matches_oracle(hip, random_batch):
+    X, Y, mask, flat = random_batch
+    model = lg.PairMLP(flat)
+    X16 = device.to_dev(X).to(torch.bfloat16)
+    out = model.forward(X16, len(X)).cpu().numpy()
+    parts = lg.unflatten(flat)
+    ref, _ = omlp.forward(parts, X[:, :lg.NIN].astype(np.float64))
+    emu, _ = emulated_forward(parts, X[:, :lg.NIN])
+    scale = np.abs(ref).max()
+    assert np.abs(out[:, :25] - emu).max() < 2e-3 * scale
+    assert np.abs(out[:, :25] - ref).max() < 3e-2 * scale
+    assert np.all(out[:, 25:] == 0)
+
+
+def test_mlp_gradients_match_oracle(hip, random_batch):
+    X, Y, mask, flat = random_batch
+    model = lg.PairMLP(flat)
+    d = lambda a: device.to_dev(a)
+    X16 = d(X).to(torch.bfloat16)
+    loss = model.loss_and_grads(X16, d(Y), d(mask), len(X), nvalid=float(mask.sum()))
+    torch.cuda.synchronize()
+    g = lg.unflatten(model.g.cpu().numpy())
+    parts = lg.unflatten(flat)
+    ref_loss, ref = omlp.loss_and_grads(parts, X[:, :lg.NIN].astype(np.float64), Y[:, :25].astype(np.float64),
+                                        mask[:, :25].astype(np.float64))
+    assert abs(float(loss.item()) - ref_loss) < 1e-2 * ref_loss
+    for k in range(8):
+        err = np.linalg.norm(g[k] - ref[k]) / np.linalg.norm(ref[k])
+        assert err < 3e-2, (k, err)
+    # padding gradients are exactly zero
+    gf = model.g.cpu().numpy()
+    off, r, c = lg.OFFSETS["W0"]
+    assert np.all(gf[off + 43 * c:off + r * c] == 0)
+    off, r, c = lg.OFFSETS["W3"]
+    assert np.all(gf[off:off + r * c].reshape(r, c)[:, 25:] == 0)
+
+
+def test_adam_matches_oracle(hip, random_batch):
+    _, _, _, flat = random_batch
+    model = lg.PairMLP(flat)
+    rng = np.random.default_rng(3)
+    m = [np.zeros(lg.NPARAM)]
+    v = [np.zeros(lg.NPARAM)]
+    p = [flat.astype(np.float64)]
+    for t in (1, 2, 3):
+        g = rng.normal(size=lg.NPARAM).astype(np.float32)
+        model.g.copy_(torch.from_numpy(g))
+        model.adam(1e-3)
+        p = omlp.adam_step(p, [g.astype(np.float64)], m, v, t, lr=1e-3)
+    w = model.w.cpu().numpy()
+    np.testing.assert_allclose(w, p[0], atol=2e-6)
+    np.testing.assert_array_equal(model.w16.float().cpu().numpy(), bf16(w))
+
+
+def test_targets_assemble_round_trip_and_purify(hip):
+    mols, b, pairs, info = _problem(5, seed=9, dtype="fp64")
+    r = b.run(max_its=40, conv_tol=1e-9, graph=False)
+    assert r.converged.all()
+    X16, _, valid = lg.features(b, pairs)
+    P = len(pairs)
+    d_info = device.to_dev(info)
+    Y = device.empty((P, 32), torch.float32)
+    mask = device.empty((P, 32), torch.float32)
+    st = device.stream()
+    _lib.call("pscf_pair_targets", P, device.ptr(d_info), device.ptr(valid), device.ptr(b.d_desc), device.ptr(b.D),
+              device.ptr(Y), device.ptr(mask), st)
+    Dconv = [b.matrix(b.D, m) for m in range(b.M)]
+    Yh, Mh = Y.cpu().numpy(), mask.cpu().numpy()
+    for k in range(0, P, 7):
+        m, a0, na, c0, nc = info[k]
+        blk = Dconv[m][a0:a0 + na, c0:c0 + nc]
+        np.testing.assert_allclose(Yh[k, :25].reshape(5, 5)[:na, :nc], blk.astype(np.float32), rtol=0, atol=0)
+        assert Mh[k].sum() == na * nc
+    # assemble the targets back: every pair block within 5 A is restored
+    b.D.zero_()
+    _lib.call("pscf_pair_assemble", P, device.ptr(d_info), device.ptr(valid), device.ptr(b.d_desc), device.ptr(Y),
+              device.ptr(b.D), st)
+    for m in range(b.M):
+        Dm = b.matrix(b.D, m)
+        covered = Dm != 0
+        np.testing.assert_allclose(Dm[covered], Dconv[m][covered].astype(np.float32), rtol=1e-7)
+    # purification: a converged (idempotent) density is a fixed point
+    b.D.copy_(torch.cat([torch.from_numpy(x.reshape(-1)) for x in Dconv]).to(b.D.device))
+    _lib.call("pscf_batch_purify", b.code, device.ptr(b.d_desc), device.ptr(b.d_order), b.M, b.max_nao,
+              device.ptr(b.ctl), device.ptr(b.S), device.ptr(b.Linv), device.ptr(b.D), device.ptr(b.Dt), st)
+    for m in range(b.M):
+        assert np.abs(b.matrix(b.D, m) - Dconv[m]).max() < 1e-8
+    # a perturbed, asymmetric density becomes symmetric, idempotent, 2 nocc electrons
+    rng = np.random.default_rng(0)
+    pert = torch.from_numpy(rng.normal(size=b.D.shape[0]) * 0.05).to(b.D.device)
+    b.D.add_(pert)
+    _lib.call("pscf_batch_purify", b.code, device.ptr(b.d_desc), device.ptr(b.d_order), b.M, b.max_nao,
+              device.ptr(b.ctl), device.ptr(b.S), device.ptr(b.Linv), device.ptr(b.D), device.ptr(b.Dt), st)
+    for m in range(b.M):
+        D, S = b.matrix(b.D, m), b.matrix(b.S, m)
+        assert np.abs(D - D.T).max() < 1e-10
+        assert abs(np.trace(D @ S) - 2 * b.nocc[m]) < 1e-9
+        assert np.abs(D @ S @ D - 2 * D).max() < 1e-9
+        np.testing.assert_allclose(b.matrix(b.Dt, m), D.astype(np.float32), rtol=1e-6, atol=1e-7)
+
+
+def test_training_reduces_loss(hip):
+    """A few hundred Adam steps on converged densities of 16 molecules lower
+    the masked MSE well below its starting value."""
+    from pyscf_ipu_amd import train_guess
+    data = train_guess.generate(molgen.batched_set(16, (1, 6), seed=31), dtype="fp32")
+    model = lg.PairMLP(seed=0)
+    hist = train_guess.train(model, data, steps=300, batch=512, lr=2e-3, log_every=0)
+    assert hist[-1] < 0.25 * hist[0], hist[::50]
+
+
+@pytest.mark.skipif(not os.path.exists(lg.DEFAULT_WEIGHTS), reason="trained weights not present")
+def test_trained_guess_gains_over_minao(hip):
+    """Held-out molecules: SCF from the trained network's guess needs fewer
+    iterations than from MINAO and lands on the same energies (1 mHa)."""
+    mols = batch.parse(molgen.batched_set(64, (1, 9), seed=424242))
+    base = lg.solve(mols, None, dtype="fp32", max_its=30)
+    model = lg.PairMLP.load()
+    learned = lg.solve(mols, model, dtype="fp32", max_its=30)
+    assert base.result.converged.all() and learned.result.converged.all()
+    assert np.abs(learned.result.energy - base.result.energy).max() < 1e-3
+    print("mean iterations minao", base.iterations_total.mean(), "learned", learned.iterations_total.mean())
+    assert learned.iterations_total.mean() < 0.8 * base.iterations_total.mean()
