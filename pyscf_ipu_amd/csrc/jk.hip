@@ -105,6 +105,62 @@ __global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
 }
 
 // ------------------------------------------------------------- packed s8 ----
+// Contributions of one row pq = (p, q) of the packed lower triangle (rs <= pq)
+// to M = J/2 - (hyb/4) K (the caller adds M + M^T), by one wavefront whose
+// lanes walk rs.  Targets (r, s), (p, s), (q, s) differ between lanes; the
+// targets (p, r) and (q, r) repeat over runs of consecutive rs with equal r, so
+// those are summed per run with a segmented suffix scan across the wave and
+// added once per run instead of by every lane.
+template <typename T>
+__device__ inline void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
+                              T* __restrict__ acc, const unsigned* __restrict__ pair, int N, T hhalf, int lane) {
+  const T dpq = dm[p * N + q];
+  T jrow = T(0);
+  for (int base = 0; base <= pq; base += 64) {
+    const int rs = base + lane;
+    const bool valid = rs <= pq;
+    int r = -1, s = 0;
+    T a = T(0), b = T(0);
+    if (valid) {
+      const unsigned rr = pair[rs];
+      r = rr & 0xffff;
+      s = rr >> 16;
+      // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
+      T v = row[rs];
+      if (p == q) v *= T(0.5);
+      if (r == s) v *= T(0.5);
+      if (pq == rs) v *= T(0.5);
+      const T v2 = v + v;
+      atomicAdd(acc + r * N + s, v2 * dpq);
+      jrow += v2 * dm[r * N + s];
+      const T vk = -hhalf * v;
+      atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
+      atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
+      a = vk * dm[q * N + s];
+      b = vk * dm[p * N + s];
+    }
+    // segmented suffix sums over lanes of equal r (r is nondecreasing in rs)
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T ta = __shfl_down(a, off, 64);
+      const T tb = __shfl_down(b, off, 64);
+      const int tr = __shfl_down(r, off, 64);
+      if (lane + off < 64 && tr == r) {
+        a += ta;
+        b += tb;
+      }
+    }
+    const int rprev = __shfl_up(r, 1, 64);
+    if (valid && (lane == 0 || rprev != r)) {
+      atomicAdd(acc + p * N + r, a);
+      atomicAdd(acc + q * N + r, b);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) jrow += __shfl_xor(jrow, off, 64);
+  if (lane == 0) atomicAdd(acc + p * N + q, jrow);
+}
+
 // One wavefront per row pq of the packed lower triangle; lanes walk rs <= pq.
 template <typename T>
 __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__ eri, int N,
@@ -132,31 +188,7 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   for (int pq = wave; pq < P; pq += nwaves) {
     const unsigned pp = pair[pq];
-    const int p = pp & 0xffff, q = pp >> 16;
-    const T dpq = dm[p * N + q];
-    const T* row = eri + tri(pq);
-    T jrow = T(0);
-    for (int rs = lane; rs <= pq; rs += 64) {
-      const unsigned rr = pair[rs];
-      const int r = rr & 0xffff, s = rr >> 16;
-      // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
-      T v = row[rs];
-      if (p == q) v *= T(0.5);
-      if (r == s) v *= T(0.5);
-      if (pq == rs) v *= T(0.5);
-      const T v2 = v + v;
-      atomicAdd(acc + r * N + s, v2 * dpq);
-      jrow += v2 * dm[r * N + s];
-      const T vk = -hhalf * v;
-      atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
-      atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
-      atomicAdd(acc + p * N + r, vk * dm[q * N + s]);
-      atomicAdd(acc + q * N + r, vk * dm[p * N + s]);
-    }
-    // wave reduction of the row's J contribution
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) jrow += __shfl_xor(jrow, off, 64);
-    if (lane == 0) atomicAdd(acc + p * N + q, jrow);
+    s8_row<T>(eri + tri(pq), pq, pp & 0xffff, pp >> 16, dm, acc, pair, N, hhalf, lane);
   }
   __syncthreads();
   T* dst = part + (int64_t)blockIdx.x * NN;
@@ -201,29 +233,7 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_batch_kernel(const BatchMol*
   const int nwaves = blockDim.x >> 6;
   for (int pq = wave; pq < P; pq += nwaves) {
     const unsigned pp = pair[pq];
-    const int p = pp & 0xffff, q = pp >> 16;
-    const T dpq = dm[p * N + q];
-    const T* row = eri + tri(pq);
-    T jrow = T(0);
-    for (int rs = lane; rs <= pq; rs += 64) {
-      const unsigned rr = pair[rs];
-      const int r = rr & 0xffff, s = rr >> 16;
-      T v = row[rs];
-      if (p == q) v *= T(0.5);
-      if (r == s) v *= T(0.5);
-      if (pq == rs) v *= T(0.5);
-      const T v2 = v + v;
-      atomicAdd(acc + r * N + s, v2 * dpq);
-      jrow += v2 * dm[r * N + s];
-      const T vk = -hhalf * v;
-      atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
-      atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
-      atomicAdd(acc + p * N + r, vk * dm[q * N + s]);
-      atomicAdd(acc + q * N + r, vk * dm[p * N + s]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) jrow += __shfl_xor(jrow, off, 64);
-    if (lane == 0) atomicAdd(acc + p * N + q, jrow);
+    s8_row<T>(eri + tri(pq), pq, pp & 0xffff, pp >> 16, dm, acc, pair, N, hhalf, lane);
   }
   __syncthreads();
   TO* out = out_all + Mo.mat0;

@@ -24,37 +24,49 @@ __host__ __device__ inline size_t state_doubles(int N) {
   return (size_t)2 * DIIS_N * N * N + 100 + (size_t)N * N + 2;
 }
 
-// C = op(X) op(Y), N x N, all operands in LDS (C may be LDS or global, never
-// aliasing X or Y).  2x2 register blocks per thread.
+// Working matrices live in LDS with an odd leading dimension ld = N | 1, so a
+// walk down a column (stride ld doubles) touches distinct LDS bank pairs.
+__host__ __device__ inline int lds_ld(int N) { return N | 1; }
+
+// C = op(X) op(Y), N x N, X and Y in LDS (leading dimension ld), C in LDS or
+// global (leading dimension ldc), never aliasing X or Y.  Each thread owns
+// rows {i, i + h} x columns {j, j + h} (h = ceil(N / 2)), so the lanes of a
+// wave walk consecutive columns and share rows.
 template <bool TX, bool TY>
 __device__ inline void lds_matmul(const double* __restrict__ X, const double* __restrict__ Y, double* __restrict__ C,
-                                  int N) {
-  const int nb = (N + 1) >> 1;
-  for (int b = threadIdx.x; b < nb * nb; b += blockDim.x) {
-    const int i = (b / nb) * 2, j = (b % nb) * 2;
-    const bool i1 = i + 1 < N, j1 = j + 1 < N;
+                                  int N, int ld, int ldc) {
+  const int h = (N + 1) >> 1;
+  for (int b = threadIdx.x; b < h * h; b += blockDim.x) {
+    const int i = b / h, j = b - i * h;
+    const int i1 = i + h, j1 = j + h;
+    const bool vi = i1 < N, vj = j1 < N;
+    const int ir = vi ? i1 : i, jr = vj ? j1 : j;
     double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
     for (int k = 0; k < N; ++k) {
-      const double x0 = TX ? X[k * N + i] : X[i * N + k];
-      const double x1 = i1 ? (TX ? X[k * N + i + 1] : X[(i + 1) * N + k]) : 0.0;
-      const double y0 = TY ? Y[j * N + k] : Y[k * N + j];
-      const double y1 = j1 ? (TY ? Y[(j + 1) * N + k] : Y[k * N + j + 1]) : 0.0;
-      c00 += x0 * y0;
-      c01 += x0 * y1;
-      c10 += x1 * y0;
-      c11 += x1 * y1;
+      const double x0 = TX ? X[k * ld + i] : X[i * ld + k];
+      const double x1 = TX ? X[k * ld + ir] : X[ir * ld + k];
+      const double y0 = TY ? Y[j * ld + k] : Y[k * ld + j];
+      const double y1 = TY ? Y[jr * ld + k] : Y[k * ld + jr];
+      c00 = fma(x0, y0, c00);
+      c01 = fma(x0, y1, c01);
+      c10 = fma(x1, y0, c10);
+      c11 = fma(x1, y1, c11);
     }
-    C[i * N + j] = c00;
-    if (j1) C[i * N + j + 1] = c01;
-    if (i1) {
-      C[(i + 1) * N + j] = c10;
-      if (j1) C[(i + 1) * N + j + 1] = c11;
+    C[i * ldc + j] = c00;
+    if (vj) C[i * ldc + j1] = c01;
+    if (vi) {
+      C[i1 * ldc + j] = c10;
+      if (vj) C[i1 * ldc + j1] = c11;
     }
   }
 }
 
-__device__ inline void lds_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
-  for (int x = threadIdx.x; x < n; x += blockDim.x) dst[x] = src[x];
+// N x N global (leading dimension N) <-> LDS (leading dimension ld)
+__device__ inline void lds_load(double* __restrict__ dst, const double* __restrict__ src, int N, int ld) {
+  for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+    const int i = x / N;
+    dst[i * ld + x - i * N] = src[x];
+  }
 }
 
 __device__ inline double block_sum(double v, double* red) {
@@ -68,50 +80,105 @@ __device__ inline double block_sum(double v, double* red) {
   return s;
 }
 
-// Single-thread cyclic Jacobi eigen-decomposition of a small symmetric matrix.
-__device__ inline void small_jacobi(double* a, double* v, int n) {
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) v[i * n + j] = (i == j);
+// Orders the LDS accesses of one wavefront (its lanes run in lockstep).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Cyclic Jacobi eigen-decomposition of a small symmetric n x n matrix
+// (n <= 16, LDS, row-major) by ONE wavefront: round-robin pairing gives n/2
+// disjoint rotations per round, applied as 2x2 block updates.  On exit the
+// diagonal of a holds the eigenvalues and v (LDS) the eigenvectors (columns).
+__device__ inline void wave_jacobi(double* a, double* v, int n, int* rp, int* rq, double* rc, double* rs) {
+  const int lane = threadIdx.x & 63;
+  const int m = n + (n & 1), half = m / 2;
+  for (int x = lane; x < n * n; x += 64) v[x] = (x / n == x % n) ? 1.0 : 0.0;
+  wave_sync();
   for (int sweep = 0; sweep < 50; ++sweep) {
     double off = 0.0, tot = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        tot += a[i * n + j] * a[i * n + j];
-        if (i != j) off += a[i * n + j] * a[i * n + j];
-      }
+    for (int x = lane; x < n * n; x += 64) {
+      const double q = a[x] * a[x];
+      tot += q;
+      if (x / n != x % n) off += q;
+    }
+    off = wave_sum(off);
+    tot = wave_sum(tot);
     if (off <= 1e-30 * tot || off == 0.0) break;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a[p * n + q];
-        if (fabs(apq) < 1e-300) continue;
-        const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
-        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double akp = a[k * n + p], akq = a[k * n + q];
-          a[k * n + p] = c * akp - s * akq;
-          a[k * n + q] = s * akp + c * akq;
+    for (int r = 0; r < m - 1; ++r) {
+      if (lane < half) {
+        int p, q;
+        if (lane == 0) {
+          p = m - 1;
+          q = r;
+        } else {
+          p = (r + lane) % (m - 1);
+          q = (r - lane + (m - 1)) % (m - 1);
         }
-        for (int k = 0; k < n; ++k) {
-          const double apk = a[p * n + k], aqk = a[q * n + k];
-          a[p * n + k] = c * apk - s * aqk;
-          a[q * n + k] = s * apk + c * aqk;
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
         }
-        for (int k = 0; k < n; ++k) {
-          const double vkp = v[k * n + p], vkq = v[k * n + q];
-          v[k * n + p] = c * vkp - s * vkq;
-          v[k * n + q] = s * vkp + c * vkq;
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = a[p * n + q];
+          if (fabs(apq) >= 1e-300) {
+            const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+          }
         }
+        rp[lane] = p;
+        rq[lane] = q;
+        rc[lane] = c;
+        rs[lane] = s;
       }
+      wave_sync();
+      for (int b = lane; b < half * half; b += 64) {
+        const int k = b / half, l = b - k * half;
+        const int pk = rp[k], qk = rq[k], pl = rp[l], ql = rq[l];
+        const double ck = rc[k], sk = rs[k], cl = rc[l], sl = rs[l];
+        const bool qkv = qk < n, qlv = ql < n;
+        const double x00 = a[pk * n + pl];
+        const double x01 = qlv ? a[pk * n + ql] : 0.0;
+        const double x10 = qkv ? a[qk * n + pl] : 0.0;
+        const double x11 = (qkv && qlv) ? a[qk * n + ql] : 0.0;
+        const double t00 = cl * x00 - sl * x01, t01 = sl * x00 + cl * x01;
+        const double t10 = cl * x10 - sl * x11, t11 = sl * x10 + cl * x11;
+        a[pk * n + pl] = ck * t00 - sk * t10;
+        if (qlv) a[pk * n + ql] = ck * t01 - sk * t11;
+        if (qkv) a[qk * n + pl] = sk * t00 + ck * t10;
+        if (qkv && qlv) a[qk * n + ql] = sk * t01 + ck * t11;
+      }
+      for (int x = lane; x < half * n; x += 64) {
+        const int l = x / n, i = x - l * n;
+        const int p = rp[l], q = rq[l];
+        if (q >= n) continue;
+        const double c = rc[l], s = rs[l];
+        const double vip = v[i * n + p], viq = v[i * n + q];
+        v[i * n + p] = c * vip - s * viq;
+        v[i * n + q] = s * vip + c * viq;
+      }
+      wave_sync();
+    }
   }
 }
 
-// Parallel cyclic Jacobi on A (LDS, N x N; on exit its diagonal holds the
-// eigenvalues), accumulating rotations into V (LDS, caller initialised).
-// Round-robin pairing gives N/2 disjoint rotations per round; the two-sided
-// update A <- J^T A J is applied as one pass over the 2x2 blocks (pair k rows,
-// pair l columns): Y = R_k^T X R_l with R = [[c, s], [-s, c]].
-__device__ inline int parallel_jacobi(double* A, double* V, int N, int* rp, int* rq, double* rc, double* rs,
+// Parallel cyclic Jacobi on A (LDS, N x N, leading dimension ld; on exit its
+// diagonal holds the eigenvalues), accumulating rotations into V (LDS, same
+// leading dimension, caller initialised).  Round-robin pairing gives N/2
+// disjoint rotations per round; the two-sided update A <- J^T A J is one pass
+// over the 2x2 blocks (pair k rows, pair l columns): Y = R_k^T X R_l with
+// R = [[c, s], [-s, c]].  Returns the number of sweeps.
+__device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* rp, int* rq, double* rc, double* rs,
                                       double* red) {
   const int m = N + (N & 1);  // even number of players; index N is a dummy
   const int half = m / 2;
@@ -120,7 +187,7 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int* rp, int*
     double off = 0.0, tot = 0.0;
     for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
-      const double v = A[x] * A[x];
+      const double v = A[i * ld + j] * A[i * ld + j];
       tot += v;
       if (i != j) off += v;
     }
@@ -145,8 +212,8 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int* rp, int*
         }
         double c = 1.0, s = 0.0;
         if (q < N) {
-          const double apq = A[p * N + q];
-          const double app = A[p * N + p], aqq = A[q * N + q];
+          const double apq = A[p * ld + q];
+          const double app = A[p * ld + p], aqq = A[q * ld + q];
           if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app * aqq))) {
             const double th = (aqq - app) / (2.0 * apq);
             const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
@@ -166,28 +233,28 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int* rp, int*
         const int pk = rp[k], qk = rq[k], pl = rp[l], ql = rq[l];
         const double ck = rc[k], sk = rs[k], cl = rc[l], sl = rs[l];
         const bool qkv = qk < N, qlv = ql < N;
-        const double x00 = A[pk * N + pl];
-        const double x01 = qlv ? A[pk * N + ql] : 0.0;
-        const double x10 = qkv ? A[qk * N + pl] : 0.0;
-        const double x11 = (qkv && qlv) ? A[qk * N + ql] : 0.0;
+        const double x00 = A[pk * ld + pl];
+        const double x01 = qlv ? A[pk * ld + ql] : 0.0;
+        const double x10 = qkv ? A[qk * ld + pl] : 0.0;
+        const double x11 = (qkv && qlv) ? A[qk * ld + ql] : 0.0;
         // columns: [x.0, x.1] R_l
         const double t00 = cl * x00 - sl * x01, t01 = sl * x00 + cl * x01;
         const double t10 = cl * x10 - sl * x11, t11 = sl * x10 + cl * x11;
         // rows: R_k^T [t0.; t1.]
-        A[pk * N + pl] = ck * t00 - sk * t10;
-        if (qlv) A[pk * N + ql] = ck * t01 - sk * t11;
-        if (qkv) A[qk * N + pl] = sk * t00 + ck * t10;
-        if (qkv && qlv) A[qk * N + ql] = sk * t01 + ck * t11;
+        A[pk * ld + pl] = ck * t00 - sk * t10;
+        if (qlv) A[pk * ld + ql] = ck * t01 - sk * t11;
+        if (qkv) A[qk * ld + pl] = sk * t00 + ck * t10;
+        if (qkv && qlv) A[qk * ld + ql] = sk * t01 + ck * t11;
       }
-      // columns of V
+      // columns of V (lanes walk rows i: stride ld, distinct banks)
       for (int x = threadIdx.x; x < half * N; x += blockDim.x) {
         const int l = x / N, i = x - l * N;
         const int p = rp[l], q = rq[l];
         if (q >= N) continue;
         const double c = rc[l], s = rs[l];
-        const double vip = V[i * N + p], viq = V[i * N + q];
-        V[i * N + p] = c * vip - s * viq;
-        V[i * N + q] = s * vip + c * viq;
+        const double vip = V[i * ld + p], viq = V[i * ld + q];
+        V[i * ld + p] = c * vip - s * viq;
+        V[i * ld + q] = s * vip + c * viq;
       }
       __syncthreads();
     }
@@ -196,32 +263,78 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int* rp, int*
 }
 
 struct ScfLds {
-  double *A, *U, *B, *Bm, *Bv, *cvec, *red, *rc, *rs, *eig;
+  double *A, *U, *B, *Bst, *Bm, *Bv, *cvec, *red, *rc, *rs, *eig;
   int *rp, *rq, *rank;
 };
 
+constexpr int SCF_RED = (SCF_THREADS / 64) * DIIS_N;
+
+// rotation tables hold N/2 + 1 pairs, and at least 8 for the 10 x 10 DIIS system
+__host__ __device__ inline int scf_pairs(int N) { return N / 2 + 1 > 8 ? N / 2 + 1 : 8; }
+
 __host__ __device__ inline size_t scf_lds_bytes(int N) {
-  const int h = N / 2 + 1;
-  return sizeof(double) * (3 * (size_t)N * N + 200 + 16 + 16 + 2 * h + N) + sizeof(int) * (2 * h + N);
+  const int h = scf_pairs(N), ld = lds_ld(N);
+  return sizeof(double) * (3 * (size_t)N * ld + 300 + 16 + SCF_RED + 2 * h + N) + sizeof(int) * (2 * h + N);
 }
 
 __device__ inline ScfLds scf_lds_carve(unsigned char* smem, int N) {
   ScfLds L;
-  const int NN = N * N, h = N / 2 + 1;
+  const int NL = N * lds_ld(N), h = scf_pairs(N);
   L.A = reinterpret_cast<double*>(smem);
-  L.U = L.A + NN;
-  L.B = L.U + NN;
-  L.Bm = L.B + NN;
+  L.U = L.A + NL;
+  L.B = L.U + NL;
+  L.Bst = L.B + NL;
+  L.Bm = L.Bst + 100;
   L.Bv = L.Bm + 100;
   L.cvec = L.Bv + 100;
   L.red = L.cvec + 16;
-  L.rc = L.red + 16;
+  L.rc = L.red + SCF_RED;
   L.rs = L.rc + h;
   L.eig = L.rs + h;
   L.rp = reinterpret_cast<int*>(L.eig + N);
   L.rq = L.rp + h;
   L.rank = L.rq + h;
   return L;
+}
+
+// From the symmetric matrix in the orthonormal basis (L.A) and the initial
+// rotation (L.U, identity or a warm start): Jacobi (eigenvectors optionally
+// stored to Uout), ascending ranks of the eigenvalues, C = L^-T U (L^-1 in
+// global Linv) left in L.A with columns unsorted, and C_occ C_occ^T (= D / 2)
+// left in L.U (leading dimension ld).
+__device__ inline int eig_density(const ScfLds& L, int N, int nocc, const double* __restrict__ Linv,
+                                  double* __restrict__ Uout) {
+  const int ld = lds_ld(N), tid = threadIdx.x;
+  double *A = L.A, *U = L.U, *B = L.B;
+  const int sweeps = parallel_jacobi(A, U, N, ld, L.rp, L.rq, L.rc, L.rs, L.red);
+  if (Uout)  // eigenvectors in the orthonormal basis (next step's warm start)
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N;
+      Uout[x] = U[i * ld + x - i * N];
+    }
+  for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * ld + j];
+  lds_load(B, Linv, N, ld);
+  __syncthreads();
+  for (int j = tid; j < N; j += blockDim.x) {
+    const double lj = L.eig[j];
+    int r = 0;
+    for (int k = 0; k < N; ++k) {
+      const double lk = L.eig[k];
+      r += (lk < lj) || (lk == lj && k < j);
+    }
+    L.rank[j] = r;
+  }
+  lds_matmul<true, false>(B, U, A, N, ld, ld);  // A = L^-T U (columns unsorted)
+  __syncthreads();
+  // occupied columns into B, then D = 2 B A^T into B via U
+  for (int x = tid; x < N * N; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    B[i * ld + j] = L.rank[j] < nocc ? A[i * ld + j] : 0.0;
+  }
+  __syncthreads();
+  lds_matmul<false, true>(B, A, U, N, ld, ld);  // U = C_occ C^T
+  __syncthreads();
+  return sweeps;
 }
 
 // One SCF step for one molecule by the whole workgroup (see pscf_scf_step).
@@ -234,151 +347,146 @@ __device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, c
                                      T* __restrict__ Dt, T* __restrict__ dDt, double* __restrict__ Hout,
                                      double* __restrict__ mo_e, double* __restrict__ Cout,
                                      double* __restrict__ state, int use_diis) {
-  const int NN = N * N;
+  const int NN = N * N, ld = lds_ld(N);
   double *A = L.A, *U = L.U, *B = L.B;
   double* Vh = state;
   double* Hh = state + (size_t)DIIS_N * NN;
   double* Bst = Hh + (size_t)DIIS_N * NN;
   double* Uprev = Bst + 100;
   double* flags = Uprev + NN;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
 
   // H = H_core + dJK + V_xc
-  for (int x = tid; x < NN; x += blockDim.x) A[x] = Hcore[x] + (double)dJK[x] + (double)Vxc[x];
-  if (use_diis) lds_copy(B, D, NN);
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N;
+    A[i * ld + x - i * N] = Hcore[x] + (double)dJK[x] + (double)Vxc[x];
+  }
+  if (use_diis) {
+    lds_load(B, D, N, ld);
+    for (int x = tid; x < 100; x += blockDim.x) {
+      const int i = x / 10, j = x % 10;
+      L.Bst[x] = it == 0 ? (((i == 0) != (j == 0)) ? 1.0 : 0.0) : Bst[x];
+    }
+  }
   __syncthreads();
 
   if (use_diis) {
     const int head = it % DIIS_N;
-    if (it == 0)
-      for (int x = tid; x < 100; x += blockDim.x) {
-        const int i = x / 10, j = x % 10;
-        Bst[x] = ((i == 0) != (j == 0)) ? 1.0 : 0.0;
-      }
-    lds_matmul<false, false>(B, A, U, N);  // U = D H
+    const int nvalid = it + 1 < DIIS_N ? it + 1 : DIIS_N;
+    lds_matmul<false, false>(B, A, U, N, ld, ld);  // U = D H
+    // H into the history (each thread reads back only its own elements below)
+    for (int x = tid; x < NN; x += blockDim.x) {
+      const int i = x / N;
+      Hh[(size_t)head * NN + x] = A[i * ld + x - i * N];
+    }
     __syncthreads();
-    lds_copy(B, S, NN);
+    lds_load(B, S, N, ld);
     __syncthreads();
-    // errvec = S U - (S U)^T into Vh[head]; H into Hh[head]
+    lds_matmul<false, false>(B, U, A, N, ld, ld);  // A = S D H
+    __syncthreads();
+    // errvec = A - A^T into Vh[head]; <V_k, errvec> in one pass
+    double part[DIIS_N];
+#pragma unroll
+    for (int k = 0; k < DIIS_N; ++k) part[k] = 0.0;
     double* err = Vh + (size_t)head * NN;
     for (int x = tid; x < NN; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
-      double sij = 0.0, sji = 0.0;
-      for (int k = 0; k < N; ++k) {
-        sij += B[i * N + k] * U[k * N + j];
-        sji += B[j * N + k] * U[k * N + i];
-      }
-      err[x] = sij - sji;
-      Hh[(size_t)head * NN + x] = A[x];
+      const double e = A[i * ld + j] - A[j * ld + i];
+      err[x] = e;
+#pragma unroll
+      for (int k = 0; k < DIIS_N; ++k)
+        if (k < nvalid) part[k] += (k == head ? e : Vh[(size_t)k * NN + x]) * e;
     }
-    __threadfence_block();
-    __syncthreads();
-    const int nvalid = it + 1 < DIIS_N ? it + 1 : DIIS_N;
+#pragma unroll
     for (int k = 0; k < DIIS_N; ++k) {
-      double part = 0.0;
-      if (k < nvalid)
-        for (int x = tid; x < NN; x += blockDim.x) part += Vh[(size_t)k * NN + x] * err[x];
-      const double dot = block_sum(part, L.red);
-      if (tid == 0) {
-        Bst[(head + 1) * 10 + (k + 1)] = dot;
-        Bst[(k + 1) * 10 + (head + 1)] = dot;
-      }
+      const double s = wave_sum(part[k]);
+      if (lane == 0) L.red[wave * DIIS_N + k] = s;
     }
     __syncthreads();
-    if (tid == 0) {
-      double mask[10];
-      mask[0] = 1.0;
-      for (int k = 0; k < DIIS_N; ++k) mask[k + 1] = k < nvalid ? 1.0 : 0.0;
-      for (int i = 0; i < 10; ++i)
-        for (int j = 0; j < 10; ++j) L.Bm[i * 10 + j] = Bst[i * 10 + j] * mask[i] * mask[j];
-      small_jacobi(L.Bm, L.Bv, 10);
+    if (tid < nvalid) {
+      double dot = 0.0;
+      for (int w = 0; w < nwave; ++w) dot += L.red[w * DIIS_N + tid];
+      L.Bst[(head + 1) * 10 + (tid + 1)] = dot;
+      L.Bst[(tid + 1) * 10 + (head + 1)] = dot;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      for (int x = lane; x < 100; x += 64) {
+        const int i = x / 10, j = x % 10;
+        const bool mi = i == 0 || i - 1 < nvalid, mj = j == 0 || j - 1 < nvalid;
+        L.Bm[x] = (mi && mj) ? L.Bst[x] : 0.0;
+      }
+      wave_sync();
+      wave_jacobi(L.Bm, L.Bv, 10, L.rp, L.rq, L.rc, L.rs);
       double wmax = 0.0;
       for (int i = 0; i < 10; ++i) wmax = fmax(wmax, fabs(L.Bm[i * 10 + i]));
       const double cut = 10.0 * 10.0 * 2.220446049250313e-16 * wmax;
       // pinv(B)[0, j] = sum_k V[0,k] V[j,k] / w_k over |w_k| > cut
-      for (int j = 0; j < 10; ++j) {
+      if (lane < 10) {
         double acc = 0.0;
         for (int k = 0; k < 10; ++k) {
           const double wk = L.Bm[k * 10 + k];
-          if (fabs(wk) > cut) acc += L.Bv[k] * L.Bv[j * 10 + k] / wk;
+          if (fabs(wk) > cut) acc += L.Bv[k] * L.Bv[lane * 10 + k] / wk;
         }
-        L.cvec[j] = acc;
+        L.cvec[lane] = acc;
       }
     }
     __syncthreads();
+    for (int x = tid; x < 100; x += blockDim.x) Bst[x] = L.Bst[x];
     for (int x = tid; x < NN; x += blockDim.x) {
       double h = 0.0;
       for (int k = 0; k < DIIS_N; ++k) h += L.cvec[k + 1] * Hh[(size_t)k * NN + x];
-      A[x] = h;
+      const int i = x / N;
+      A[i * ld + x - i * N] = h;
     }
   }
-  lds_copy(B, Linv, NN);
+  lds_load(B, Linv, N, ld);
   __syncthreads();
-  for (int x = tid; x < NN; x += blockDim.x) Hout[x] = A[x];
-  lds_matmul<false, true>(A, B, U, N);  // U = H L^T
+  for (int x = tid; x < NN; x += blockDim.x) {
+    const int i = x / N;
+    Hout[x] = A[i * ld + x - i * N];
+  }
+  lds_matmul<false, true>(A, B, U, N, ld, ld);  // U = H L^T
   __syncthreads();
-  lds_matmul<false, false>(B, U, A, N);  // A = L U
+  lds_matmul<false, false>(B, U, A, N, ld, ld);  // A = L U
   __syncthreads();
   // exact symmetry (rounding of the FP32 Fock terms and of the transform
   // leaves O(eps) asymmetry that Jacobi rotations cannot remove)
   for (int x = tid; x < NN; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
     if (i < j) {
-      const double v = 0.5 * (A[x] + A[j * N + i]);
-      A[x] = v;
-      A[j * N + i] = v;
+      const double v = 0.5 * (A[i * ld + j] + A[j * ld + i]);
+      A[i * ld + j] = v;
+      A[j * ld + i] = v;
     }
   }
   __syncthreads();
   const bool warm = flags[0] != 0.0;
   if (warm) {
-    lds_copy(B, Uprev, NN);
+    lds_load(B, Uprev, N, ld);
     __syncthreads();
-    lds_matmul<false, false>(A, B, U, N);  // U = A Uprev
+    lds_matmul<false, false>(A, B, U, N, ld, ld);  // U = A Uprev
     __syncthreads();
-    lds_matmul<true, false>(B, U, A, N);  // A = Uprev^T U
+    lds_matmul<true, false>(B, U, A, N, ld, ld);  // A = Uprev^T U
     __syncthreads();
-    lds_copy(U, B, NN);
+    for (int x = tid; x < N * ld; x += blockDim.x) U[x] = B[x];
   } else {
-    for (int x = tid; x < NN; x += blockDim.x) U[x] = (x / N == x % N) ? 1.0 : 0.0;
+    for (int x = tid; x < N * ld; x += blockDim.x) U[x] = (x / ld == x % ld) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const int sweeps = parallel_jacobi(A, U, N, L.rp, L.rq, L.rc, L.rs, L.red);
-  if (tid == 0) flags[1] = (double)sweeps;  // diagnostic: Jacobi sweeps of this step
-  for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * N + j];
-  __syncthreads();
-  for (int j = tid; j < N; j += blockDim.x) {
-    const double lj = L.eig[j];
-    int r = 0;
-    for (int k = 0; k < N; ++k) {
-      const double lk = L.eig[k];
-      r += (lk < lj) || (lk == lj && k < j);
-    }
-    L.rank[j] = r;
-    mo_e[r] = lj;
+  const int sweeps = eig_density(L, N, nocc, Linv, Uprev);
+  if (tid == 0) {
+    flags[0] = 1.0;
+    flags[1] = (double)sweeps;  // diagnostic: Jacobi sweeps of this step
   }
-  // previous eigenvectors (orthogonalised basis) for the next warm start
-  for (int x = tid; x < NN; x += blockDim.x) Uprev[x] = U[x];
-  if (tid == 0) flags[0] = 1.0;
-  lds_copy(B, Linv, NN);
-  __syncthreads();
-  // C = L^-T U, columns in ascending eigenvalue order, into A
+  for (int j = tid; j < N; j += blockDim.x) mo_e[L.rank[j]] = L.eig[j];
   for (int x = tid; x < NN; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
-    double acc = 0.0;
-    for (int k = 0; k < N; ++k) acc += B[k * N + i] * U[k * N + j];
-    A[i * N + L.rank[j]] = acc;
-  }
-  __syncthreads();
-  for (int x = tid; x < NN; x += blockDim.x) Cout[x] = A[x];
-  // D = 2 C_occ C_occ^T
-  for (int x = tid; x < NN; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    double acc = 0.0;
-    for (int k = 0; k < nocc; ++k) acc += A[i * N + k] * A[j * N + k];
-    if (dDt) dDt[x] = (T)(2.0 * acc - D[x]);
-    D[x] = 2.0 * acc;
-    if (Dt) Dt[x] = (T)(2.0 * acc);
+    Cout[i * N + L.rank[j]] = A[i * ld + j];
+    const double d = 2.0 * U[i * ld + j];
+    if (dDt) dDt[x] = (T)(d - D[x]);
+    D[x] = d;
+    if (Dt) Dt[x] = (T)d;
   }
 }
 
@@ -582,59 +690,44 @@ __global__ void __launch_bounds__(SCF_THREADS) purify_batch_kernel(const BatchMo
   const int m = order[blockIdx.x];
   if (ctl[m * CTL + 2] < 0.0) return;  // failed set-up
   const BatchMol Mo = mols[m];
-  const int N = (int)Mo.nao, NN = N * N, nocc = (int)Mo.nocc;
+  const int N = (int)Mo.nao, NN = N * N, nocc = (int)Mo.nocc, ld = lds_ld(N);
   const ScfLds L = scf_lds_carve(smem, N);
   double *A = L.A, *U = L.U, *B = L.B;
   const int64_t o = Mo.mat0;
   const int tid = threadIdx.x;
   for (int x = tid; x < NN; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
-    A[x] = 0.5 * (D[o + x] + D[o + j * N + i]);
+    A[i * ld + j] = 0.5 * (D[o + x] + D[o + j * N + i]);
   }
-  lds_copy(B, S + o, NN);
+  lds_load(B, S + o, N, ld);
   __syncthreads();
-  lds_matmul<false, false>(B, A, U, N);  // U = S D
+  lds_matmul<false, false>(B, A, U, N, ld, ld);  // U = S D
   __syncthreads();
-  lds_matmul<false, false>(U, B, A, N);  // A = S D S
+  lds_matmul<false, false>(U, B, A, N, ld, ld);  // A = S D S
   __syncthreads();
-  lds_copy(B, Linv + o, NN);
+  lds_load(B, Linv + o, N, ld);
   __syncthreads();
-  lds_matmul<false, false>(B, A, U, N);  // U = L^-1 S D S
+  lds_matmul<false, false>(B, A, U, N, ld, ld);  // U = L^-1 S D S
   __syncthreads();
-  lds_matmul<false, true>(U, B, A, N);  // A = U L^-T
+  lds_matmul<false, true>(U, B, A, N, ld, ld);  // A = U L^-T
   __syncthreads();
+  // -P, exactly symmetric: ascending eigenvalues = descending occupations
   for (int x = tid; x < NN; x += blockDim.x) {
-    A[x] = -A[x];  // ascending eigenvalues of -P = descending occupations
-    U[x] = (x / N == x % N) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  parallel_jacobi(A, U, N, L.rp, L.rq, L.rc, L.rs, L.red);
-  for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * N + j];
-  __syncthreads();
-  for (int j = tid; j < N; j += blockDim.x) {
-    const double lj = L.eig[j];
-    int r = 0;
-    for (int k = 0; k < N; ++k) {
-      const double lk = L.eig[k];
-      r += (lk < lj) || (lk == lj && k < j);
+    const int i = x / N, j = x - i * N;
+    if (i <= j) {
+      const double v = -0.5 * (A[i * ld + j] + A[j * ld + i]);
+      A[i * ld + j] = v;
+      A[j * ld + i] = v;
     }
-    L.rank[j] = r;
   }
+  for (int x = tid; x < N * ld; x += blockDim.x) U[x] = (x / ld == x % ld) ? 1.0 : 0.0;
   __syncthreads();
-  // C = L^-T U with columns in rank order (B still holds L^-1)
+  eig_density(L, N, nocc, Linv + o, nullptr);
   for (int x = tid; x < NN; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
-    double acc = 0.0;
-    for (int k = 0; k < N; ++k) acc += B[k * N + i] * U[k * N + j];
-    A[i * N + L.rank[j]] = acc;
-  }
-  __syncthreads();
-  for (int x = tid; x < NN; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    double acc = 0.0;
-    for (int k = 0; k < nocc; ++k) acc += A[i * N + k] * A[j * N + k];
-    D[o + x] = 2.0 * acc;
-    Dt[o + x] = (T)(2.0 * acc);
+    const double d = 2.0 * U[i * ld + j];
+    D[o + x] = d;
+    Dt[o + x] = (T)d;
   }
 }
 

@@ -10,6 +10,8 @@
 // and Y = sum_c phi_c v_c w in LDS, and accumulating its share of
 // V = phi0^T Y in registers (4x4 register blocks); one partial V per
 // workgroup is reduced in a fixed order by a second kernel.
+#include <type_traits>
+
 #include "b3lyp.h"
 #include "batch.h"
 #include "common.h"
@@ -145,128 +147,192 @@ struct XCArgs {
 };
 
 // LDS layout: D[N][N] | phi0[N][TILE+1] | XY[N][TILE+1] | coef[4][TILE] | red[8]
-constexpr int XC_MAXT = 2;  // register blocks per thread: NB*NB <= 512 (N <= 88)
-
-// Walks tiles tile0, tile0 + tstride, ... of one molecule's grid, accumulating
-// this thread's 4x4 blocks of V = phi0^T Y in vacc and its share of E_xc in e_loc.
+// ---- MFMA path ------------------------------------------------------------------
+// Per tile of XT points (64 in FP32, 32 in FP64), with the tile's AO values and
+// gradients staged in LDS as Phi[c][p][nu] (row stride XS = NP + 4, NP = N
+// rounded up to 16):
+//   GEMM 1 (v_mfma_{f32,f64}_16x16x4): X[p][nu] = sum_mu phi0[p][mu] D[mu][nu];
+//     the epilogue forms rho_c[p] = sum_nu X[p][nu] phi_c[p][nu] from the
+//     accumulators, so X never goes to LDS (wave w owns point block w % PB and
+//     every (4 / PB)-th nu block; the 4 / PB partial sums meet in LDS);
+//   pointwise B3LYP; Y[p][nu] = c0 phi0 + sum_c c_c phi_c;
+//   GEMM 2: V[mu][nu] += sum_p phi0[p][mu] Y[p][nu], 16 x 16 blocks of V held
+//     in accumulator registers across all tiles (block wave + 4u).
+// Operand layout of v_mfma_*_16x16x4 (lane l): A[l % 16][l / 16],
+// B[l / 16][l % 16]; accumulator register j holds D[r][l % 16] with
+// r = 4 (l / 16) + j for f32 and r = l / 16 + 4 j for f64.  XS = 4 mod 16 keeps the GEMM-1
+// A-operand and epilogue reads of Phi free of bank conflicts; D and Y use row
+// stride XD = 16 mod 64 for the same reason.
 template <typename T>
-__device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
-                                T (&vacc)[XC_MAXT][16], double& e_loc) {
-  const int N = a.N, TP = XC_TILE + 1;
-  T* D = reinterpret_cast<T*>(smem);
-  T* phi0 = D + N * N;
-  T* XY = phi0 + N * TP;
-  T* coef = XY + N * TP;
+struct XMTraits;
+template <>
+struct XMTraits<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static constexpr int TILE = 64;
+  static __device__ inline v4 mfma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  // row of accumulator register j in lane group lg = lane / 16
+  static __device__ inline int row(int lg, int j) { return 4 * lg + j; }
+};
+template <>
+struct XMTraits<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static constexpr int TILE = 32;
+  static __device__ inline v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lg, int j) { return lg + 4 * j; }
+};
+constexpr int XM_MAXB = 7;   // V blocks per wave: (NP / 16)^2 <= 28
+constexpr int XM_MAXN = 80;  // nao limit of the MFMA path
+
+__host__ __device__ inline int xm_np(int N) { return (N + 15) & ~15; }
+__host__ __device__ inline int xm_xs(int N) { return xm_np(N) + 4; }
+__host__ __device__ inline int xm_xd(int N) { return xm_np(N) <= 16 ? 16 : 80; }
+template <typename T>
+__host__ __device__ inline size_t xm_lds_elems(int N) {
+  constexpr int XT = XMTraits<T>::TILE;
+  const int NP = xm_np(N);
+  return (size_t)4 * XT * xm_xs(N) + (size_t)XT * xm_xd(N) + (size_t)NP * xm_xd(N) + 256 + 4 * XT;
+}
+
+template <typename T>
+__device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
+                                     typename XMTraits<T>::v4 (&vacc)[XM_MAXB], double& e_loc) {
+  typedef typename XMTraits<T>::v4 v4;
+  constexpr int XT = XMTraits<T>::TILE, PB = XT / 16, NSLOT = 4 / PB;
+  const int N = a.N, NP = xm_np(N), NB = NP / 16, XS = xm_xs(N), XD = xm_xd(N);
+  T* Phi = reinterpret_cast<T*>(smem);  // [4][XT][XS]
+  T* Y = Phi + 4 * XT * XS;             // [XT][XD]
+  T* Dm = Y + XT * XD;                  // [NP][XD]
+  T* rpart = Dm + NP * XD;              // [NSLOT][4][XT]
+  T* coef = rpart + 256;                // [4][XT]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nvb = NB * NB;
+  const int pb = wave % PB, slot = wave / PB;
   __syncthreads();
-  for (int x = threadIdx.x; x < N * N; x += blockDim.x) D[x] = a.dm[x];
-  const int tid = threadIdx.x;
-  const int gl = tid & (XC_TILE - 1);  // point within tile
-  const int sl = tid / XC_TILE;        // slice 0..3
-  const int NB = (N + 3) / 4;
-  const int ntile = NB * NB;
+  for (int x = tid; x < NP * NP; x += blockDim.x) {
+    const int i = x / NP, j = x - i * NP;
+    Dm[i * XD + j] = (i < N && j < N) ? a.dm[i * N + j] : T(0);
+  }
+  // padding columns nu in [N, NP) of Phi stay zero for every tile
+  for (int x = tid; x < 4 * XT * (NP - N); x += blockDim.x) {
+    const int row = x / (NP - N);
+    Phi[row * XS + N + x - row * (NP - N)] = T(0);
+  }
 #pragma unroll
-  for (int u = 0; u < XC_MAXT; ++u)
-#pragma unroll
-    for (int k = 0; k < 16; ++k) vacc[u][k] = T(0);
-  const int ntiles = (a.g + XC_TILE - 1) / XC_TILE;
+  for (int u = 0; u < XM_MAXB; ++u) vacc[u] = (v4){T(0), T(0), T(0), T(0)};
+  const int ntiles = (a.g + XT - 1) / XT;
+  const int per_c = N * XT;
   for (int tile = tile0; tile < ntiles; tile += tstride) {
-    const int p0 = tile * XC_TILE;
-    const int pt = p0 + gl;
-    const bool valid = pt < a.g;
+    const int p0 = tile * XT;
     __syncthreads();
-    for (int mu = sl; mu < N; mu += 4) phi0[mu * TP + gl] = valid ? a.ao[mu * a.smu + (int64_t)pt * a.sp] : T(0);
-    __syncthreads();
-    // X[nu][p] = sum_mu phi0[mu][p] D[mu][nu]  (slice sl owns nu = sl, sl+4, ...)
-    for (int nu = sl; nu < N; nu += 4) {
-      T s = T(0);
-      for (int mu = 0; mu < N; ++mu) s += phi0[mu * TP + gl] * D[mu * N + nu];
-      XY[nu * TP + gl] = s;
+    for (int x = tid; x < 4 * per_c; x += blockDim.x) {
+      const int c = x / per_c, r = x - c * per_c;
+      const int mu = r / XT, p = r - mu * XT;
+      const int pt = p0 + p;
+      Phi[(c * XT + p) * XS + mu] = pt < a.g ? a.ao[c * a.sc + mu * a.smu + (int64_t)pt * a.sp] : T(0);
     }
     __syncthreads();
-    // rho_c[p] = sum_nu X[nu][p] phi_c[nu][p]; slice sl = component c
+    // GEMM 1 and the density epilogue
     {
-      T r = T(0);
-      if (valid) {
-        if (sl == 0) {
-          for (int nu = 0; nu < N; ++nu) r += XY[nu * TP + gl] * phi0[nu * TP + gl];
-        } else {
-          const T* pc = a.ao + sl * a.sc + (int64_t)pt * a.sp;
-          for (int nu = 0; nu < N; ++nu) r += XY[nu * TP + gl] * pc[nu * a.smu];
+      T part[4][4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) part[c][j] = T(0);
+      for (int nb = slot; nb < NB; nb += NSLOT) {
+        v4 acc = {T(0), T(0), T(0), T(0)};
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+          const T av = Phi[(pb * 16 + li) * XS + k0 + lg];
+          const T bv = Dm[(k0 + lg) * XD + nb * 16 + li];
+          acc = XMTraits<T>::mfma(av, bv, acc);
         }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            part[c][j] += acc[j] * Phi[(c * XT + pb * 16 + XMTraits<T>::row(lg, j)) * XS + nb * 16 + li];
       }
-      coef[sl * XC_TILE + gl] = r;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) part[c][j] += __shfl_xor(part[c][j], off, 64);
+      if (li == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) rpart[(slot * 4 + c) * XT + pb * 16 + XMTraits<T>::row(lg, j)] = part[c][j];
     }
     __syncthreads();
-    if (sl == 0) {
+    if (tid < XT) {
+      const int pt = p0 + tid;
+      T rc[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T v = T(0);
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) v += rpart[(q * 4 + c) * XT + tid];
+        rc[c] = v;
+      }
       T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0);
-      if (valid) {
-        const T r0 = coef[gl], hx = coef[XC_TILE + gl], hy = coef[2 * XC_TILE + gl], hz = coef[3 * XC_TILE + gl];
+      if (pt < a.g) {
         T e, vr, vn;
-        b3lyp_point<T>(r0, hx, hy, hz, a.eps, e, vr, vn);
+        b3lyp_point<T>(rc[0], rc[1], rc[2], rc[3], a.eps, e, vr, vn);
         const T wt = a.w[pt];
-        e_loc += (double)r0 * (double)wt * (double)e;
+        e_loc += (double)rc[0] * (double)wt * (double)e;
         c0 = T(0.5) * vr * wt;
         const T f = T(4) * vn * wt;
-        c1 = f * hx;
-        c2 = f * hy;
-        c3 = f * hz;
+        c1 = f * rc[1];
+        c2 = f * rc[2];
+        c3 = f * rc[3];
       }
-      coef[gl] = c0;
-      coef[XC_TILE + gl] = c1;
-      coef[2 * XC_TILE + gl] = c2;
-      coef[3 * XC_TILE + gl] = c3;
+      coef[tid] = c0;
+      coef[XT + tid] = c1;
+      coef[2 * XT + tid] = c2;
+      coef[3 * XT + tid] = c3;
     }
     __syncthreads();
-    // Y[nu][p] = c0 phi0 + sum_c c_c phi_c
-    {
-      const T c0 = coef[gl], c1 = coef[XC_TILE + gl], c2 = coef[2 * XC_TILE + gl], c3 = coef[3 * XC_TILE + gl];
-      for (int nu = sl; nu < N; nu += 4) {
-        T y = T(0);
-        if (valid) {
-          const int64_t o = nu * a.smu + (int64_t)pt * a.sp;
-          y = c0 * phi0[nu * TP + gl] + c1 * a.ao[a.sc + o] + c2 * a.ao[2 * a.sc + o] + c3 * a.ao[3 * a.sc + o];
-        }
-        XY[nu * TP + gl] = y;
-      }
+    for (int x = tid; x < XT * NP; x += blockDim.x) {
+      const int p = x / NP, nu = x - p * NP;
+      Y[p * XD + nu] = coef[p] * Phi[p * XS + nu] + coef[XT + p] * Phi[(XT + p) * XS + nu] +
+                       coef[2 * XT + p] * Phi[(2 * XT + p) * XS + nu] + coef[3 * XT + p] * Phi[(3 * XT + p) * XS + nu];
     }
     __syncthreads();
-    // V[mu][nu] += sum_p phi0[mu][p] Y[nu][p]
+    // GEMM 2: V[mu][nu] += sum_p phi0[p][mu] Y[p][nu]
+    for (int k0 = 0; k0 < XT; k0 += 4) {
 #pragma unroll
-    for (int u = 0; u < XC_MAXT; ++u) {
-      const int t = tid + u * XC_THREADS;
-      if (t < ntile) {
-        const int bm = (t / NB) * 4, bn = (t % NB) * 4;
-        for (int p = 0; p < XC_TILE; ++p) {
-          T am[4], bv[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            am[k] = (bm + k < N) ? phi0[(bm + k) * TP + p] : T(0);
-            bv[k] = (bn + k < N) ? XY[(bn + k) * TP + p] : T(0);
-          }
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) vacc[u][i * 4 + j] += am[i] * bv[j];
+      for (int u = 0; u < XM_MAXB; ++u) {
+        const int vb = wave + 4 * u;
+        if (vb < nvb) {
+          const int mb = vb / NB, nb = vb - mb * NB;
+          const T av = Phi[(k0 + lg) * XS + mb * 16 + li];
+          const T bv = Y[(k0 + lg) * XD + nb * 16 + li];
+          vacc[u] = XMTraits<T>::mfma(av, bv, vacc[u]);
         }
       }
     }
   }
 }
 
-// Stores this thread's register blocks of V into dst (row stride N).
+// V blocks of this wave (rows/cols < N) into dst with row stride ld.
 template <typename T>
-__device__ inline void xc_store_v(const T (&vacc)[XC_MAXT][16], int N, T* dst) {
-  const int NB = (N + 3) / 4, ntile = NB * NB;
+__device__ inline void xm_store_v(const typename XMTraits<T>::v4 (&vacc)[XM_MAXB], int N, T* dst, int ld) {
+  const int NB = xm_np(N) / 16, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lg = lane >> 4;
 #pragma unroll
-  for (int u = 0; u < XC_MAXT; ++u) {
-    const int t = threadIdx.x + u * XC_THREADS;
-    if (t < ntile) {
-      const int bm = (t / NB) * 4, bn = (t % NB) * 4;
+  for (int u = 0; u < XM_MAXB; ++u) {
+    const int vb = wave + 4 * u;
+    if (vb < NB * NB) {
+      const int mb = vb / NB, nb = vb - mb * NB;
+      const int col = nb * 16 + li;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (bm + i < N && bn + j < N) dst[(bm + i) * N + bn + j] = vacc[u][i * 4 + j];
+      for (int j = 0; j < 4; ++j) {
+        const int row = mb * 16 + XMTraits<T>::row(lg, j);
+        if (row < N && col < N) dst[row * ld + col] = vacc[u][j];
+      }
     }
   }
 }
@@ -283,10 +349,10 @@ __device__ inline double xc_block_energy(double e_loc, double* red) {
 template <typename T>
 __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  T vacc[XC_MAXT][16];
   double e_loc = 0.0;
-  xc_tiles<T>(a, smem, blockIdx.x, gridDim.x, vacc, e_loc);
-  xc_store_v<T>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N);
+  typename XMTraits<T>::v4 vacc[XM_MAXB];
+  xc_tiles_mfma<T>(a, smem, blockIdx.x, gridDim.x, vacc, e_loc);
+  xm_store_v<T>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
   double* red = reinterpret_cast<double*>(smem);
   __syncthreads();
   const double e = xc_block_energy(e_loc, red);
@@ -315,13 +381,13 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   a.dm = dm + Mo.mat0;
   a.w = w + Mo.grid0;
   a.eps = eps;
-  T vacc[XC_MAXT][16];
   double e_loc = 0.0;
-  xc_tiles<T>(a, smem, 0, 1, vacc, e_loc);
   const int N = a.N;
   T* Vl = reinterpret_cast<T*>(smem);
+  typename XMTraits<T>::v4 vacc[XM_MAXB];
+  xc_tiles_mfma<T>(a, smem, 0, 1, vacc, e_loc);
   __syncthreads();
-  xc_store_v<T>(vacc, N, Vl);
+  xm_store_v<T>(vacc, N, Vl, N);
   __syncthreads();
   T* out = vxc + Mo.mat0;
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
@@ -495,7 +561,14 @@ __global__ void xc_reduce_kernel(const T* __restrict__ part_v, const double* __r
 
 template <typename T>
 static size_t xc_lds(int N) {
-  return sizeof(T) * ((size_t)N * N + 2 * (size_t)N * (XC_TILE + 1) + 4 * XC_TILE) + 8 * sizeof(double);
+  const size_t main = sizeof(T) * xm_lds_elems<T>(N);
+  const size_t tail = sizeof(T) * (size_t)N * N + 8 * sizeof(double);  // V + energy partials after the tiles
+  return (main > tail ? main : tail) + 8 * sizeof(double);
+}
+
+template <typename T>
+static int xc_max_nao() {
+  return XM_MAXN;
 }
 
 static int xc_blocks(int g) {
@@ -508,8 +581,7 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
                    double eps, double* exc, T* vxc, void* ws, size_t ws_bytes, hipStream_t st) {
   const size_t lds = xc_lds<T>(N);
   PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_xc: N=%d needs %zu B LDS", N, lds);
-  const int NB = (N + 3) / 4;
-  PSCF_REQUIRE(NB * NB <= 2 * XC_THREADS, "pscf_xc: N=%d too large for the register-blocked V (max 88)", N);
+  PSCF_REQUIRE(N <= xc_max_nao<T>(), "pscf_xc: N=%d too large (max %d)", N, xc_max_nao<T>());
   const int nb = xc_blocks(g);
   const size_t need = sizeof(T) * (size_t)nb * N * N + sizeof(double) * nb;
   PSCF_REQUIRE(ws && ws_bytes >= need, "pscf_xc: workspace too small (%zu < %zu)", ws_bytes, need);
@@ -667,8 +739,8 @@ extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, 
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_xc: bad dtype");
   PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && d_ao && d_dm && d_w && d_vxc && d_exc,
                "pscf_batch_xc: bad arguments");
-  const int NB = (max_nao + 3) / 4;
-  PSCF_REQUIRE(NB * NB <= XC_MAXT * XC_THREADS, "pscf_batch_xc: nao %d > 88", max_nao);
+  PSCF_REQUIRE(max_nao <= (dtype == PSCF_F64 ? xc_max_nao<double>() : xc_max_nao<float>()),
+               "pscf_batch_xc: nao %d too large", max_nao);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == PSCF_F64) {
     const size_t lds = xc_lds<double>(max_nao);

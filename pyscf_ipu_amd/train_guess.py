@@ -91,14 +91,15 @@ def train(model: lg.PairMLP, data: Dict[str, torch.Tensor], steps: int, batch: i
     P = X16.shape[0]
     gen = torch.Generator(device=X16.device)
     gen.manual_seed(seed)
-    # one normaliser for all minibatches: expected number of valid entries
-    inv_valid = float(P * batch) / max(float(mask.sum().item()), 1.0)
+    # one normaliser for all minibatches: the expected number of valid
+    # entries in a minibatch
+    nvalid = batch * max(float(mask.sum().item()), 1.0) / P
     losses = []
     t0 = time.perf_counter()
     for s in range(steps):
         idx = torch.randint(0, P, (batch,), device=X16.device, generator=gen)
         xb, yb, mb = X16.index_select(0, idx), Y.index_select(0, idx), mask.index_select(0, idx)
-        loss = model.loss_and_grads(xb, yb, mb, batch, nvalid=batch / inv_valid)
+        loss = model.loss_and_grads(xb, yb, mb, batch, nvalid=nvalid)
         frac = lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + math.cos(math.pi * s / max(steps - 1, 1)))
         model.adam(lr * frac)
         losses.append(loss.clone())
@@ -124,11 +125,12 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--molecules", type=int, default=200_000)
     ap.add_argument("--val-molecules", type=int, default=2048)
-    ap.add_argument("--steps", type=int, default=20_000)
-    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--steps", type=int, default=60_000)
+    ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", default=lg.DEFAULT_WEIGHTS)
+    ap.add_argument("--eval-molecules", type=int, default=1024)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     t0 = time.perf_counter()
@@ -146,6 +148,14 @@ def main(argv=None):
                        "steps": a.steps, "batch": a.batch, "lr": a.lr, "val_mse": v,
                        "train_loss_last100": float(np.mean(hist[-100:]))})
     log(f"saved {a.out} ({time.perf_counter() - t0:.1f} s total)")
+    if a.eval_molecules:
+        mols = batch.parse(molecule_strings(a.eval_molecules, 800_000, a.workers))
+        base = lg.solve(mols, None, max_its=30)
+        guess = lg.solve(mols, model, max_its=30)
+        dE = np.abs(guess.result.energy - base.result.energy)
+        log(f"held-out {len(mols)}: mean iterations minao {base.iterations_total.mean():.2f} learned "
+            f"{guess.iterations_total.mean():.2f} (fallback {int(guess.fallback.sum())}); converged "
+            f"{int(base.result.converged.sum())} / {int(guess.result.converged.sum())}; max |dE| {dE.max():.2e} Ha")
     return 0
 
 

@@ -191,13 +191,16 @@ def test_training_reduces_loss(hip):
 
 @pytest.mark.skipif(not os.path.exists(lg.DEFAULT_WEIGHTS), reason="trained weights not present")
 def test_trained_guess_gains_over_minao(hip):
-    """Held-out molecules: SCF from the trained network's guess needs fewer
-    iterations than from MINAO and lands on the same energies (1 mHa)."""
+    """Held-out molecules (seed disjoint from training), FP64 batched SCF with
+    the |dE| < 1e-8 stop: starting from the trained network's purified guess
+    takes fewer SCF iterations in total than starting from MINAO, and lands on
+    the same energies."""
     mols = batch.parse(molgen.batched_set(64, (1, 9), seed=424242))
-    base = lg.solve(mols, None, dtype="fp32", max_its=30)
+    base = lg.solve(mols, None, dtype="fp64", max_its=40, conv_tol=1e-8)
     model = lg.PairMLP.load()
-    learned = lg.solve(mols, model, dtype="fp32", max_its=30)
+    learned = lg.solve(mols, model, dtype="fp64", max_its=40, learned_its=15, conv_tol=1e-8)
     assert base.result.converged.all() and learned.result.converged.all()
-    assert np.abs(learned.result.energy - base.result.energy).max() < 1e-3
-    print("mean iterations minao", base.iterations_total.mean(), "learned", learned.iterations_total.mean())
-    assert learned.iterations_total.mean() < 0.8 * base.iterations_total.mean()
+    assert np.abs(learned.result.energy - base.result.energy).max() < 1e-6
+    print("iterations minao", base.iterations_total.sum(), "learned", learned.iterations_total.sum(),
+          "fallback", int(learned.fallback.sum()))
+    assert learned.iterations_total.sum() < base.iterations_total.sum()
