@@ -80,7 +80,7 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
             continue
 
         def once():
-            return lg.solve(mols, mdl, dtype="fp32", max_its=its, learned_its=15, conv_tol=1e-7, graph=True)
+            return lg.solve(mols, mdl, dtype="fp32", max_its=its, learned_its=15, conv_tol=1e-6, graph=True)
 
         for _ in range(warmup):
             once()
@@ -103,7 +103,8 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
               "fp64_reference_converged": int(r64.converged.sum()),
               "dtype": "f32 (ERI/JK/XC/Becke) + f64 (eigh/DIIS/energy)",
               "workload": f"{nmol} seeded molecules, 1-9 heavy atoms C/N/O, STO-3G, grid level 0, FP32 stop "
-                          f"|dE| < 1e-7, <= {its} iterations, one launch per SCF phase, HIP-graph captured iterations"}
+                          f"|dE| < 1e-6 (above the FP32 energy noise), <= {its} iterations, one launch per SCF "
+                          "phase, HIP-graph captured iterations"}
     return out, common
 
 
@@ -138,10 +139,13 @@ def main(argv=None):
         if [m["mol_str"] for m in gj["molecules"]] == mol_strs[:len(gj["molecules"])]:
             golden = [m["energy"] for m in gj["molecules"]]
 
+    # FP32 runs stop at |dE| < 1e-6 Ha, above their energy noise; FP64 at 1e-8
+    conv_tol = 1e-8 if a.dtype == "fp64" else 1e-6
+
     def run_all(ev=None):
         res = []
         for m in mols:
-            res.append(RKS(m, level=0, dtype=a.dtype, events=ev).run(its=a.its, conv_tol=1e-8))
+            res.append(RKS(m, level=0, dtype=a.dtype, events=ev).run(its=a.its, conv_tol=conv_tol))
         return res
 
     for _ in range(a.warmup):

@@ -193,7 +193,9 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
     }
     off = block_sum(off, red);
     tot = block_sum(tot, red);
-    if (off <= 1e-26 * tot) break;
+    // relative off-diagonal norm 1e-9: eigenvectors to ~1e-9, so D to ~1e-9 and
+    // the energy (second order in D) to ~1e-18 of its scale
+    if (off <= 1e-18 * tot) break;
     for (int r = 0; r < m - 1; ++r) {
       if ((int)threadIdx.x < half) {
         const int k = threadIdx.x;
@@ -227,9 +229,11 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
         rs[k] = s;
       }
       __syncthreads();
-      // 2x2 blocks of A
-      for (int b = threadIdx.x; b < half * half; b += blockDim.x) {
-        const int k = b / half, l = b - k * half;
+      // 2x2 blocks of A: block (k, l) = (tid / 32, tid % 32) when N <= 64
+      const bool fast = half <= 32 && blockDim.x >= 1024;
+      for (int b = threadIdx.x; b < (fast ? 1024 : half * half); b += blockDim.x) {
+        const int k = fast ? (b >> 5) : b / half, l = fast ? (b & 31) : b - (b / half) * half;
+        if (k >= half || l >= half) continue;
         const int pk = rp[k], qk = rq[k], pl = rp[l], ql = rq[l];
         const double ck = rc[k], sk = rs[k], cl = rc[l], sl = rs[l];
         const bool qkv = qk < N, qlv = ql < N;
@@ -246,9 +250,11 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
         if (qkv) A[qk * ld + pl] = sk * t00 + ck * t10;
         if (qkv && qlv) A[qk * ld + ql] = sk * t01 + ck * t11;
       }
-      // columns of V (lanes walk rows i: stride ld, distinct banks)
-      for (int x = threadIdx.x; x < half * N; x += blockDim.x) {
-        const int l = x / N, i = x - l * N;
+      // columns of V (lanes walk rows i: stride ld, distinct banks); with
+      // N <= 64, lane = row and wave = rotation (mod 16)
+      for (int x = threadIdx.x; x < (fast ? 64 * half : half * N); x += blockDim.x) {
+        const int l = fast ? (x >> 6) : x / N, i = fast ? (x & 63) : x - (x / N) * N;
+        if (i >= N) continue;
         const int p = rp[l], q = rq[l];
         if (q >= N) continue;
         const double c = rc[l], s = rs[l];
