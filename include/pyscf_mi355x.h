@@ -217,6 +217,43 @@ int pscf_batch_energy(int dtype, const void* d_mols, int nmol, const double* d_e
                       const double* d_Hcore, const double* d_dJK, const double* d_exc, const double* d_mo_energy,
                       double* d_ctl, double* d_E, int init, double tol, int max_its, void* stream);
 
+/* ---- large molecules and the precision study ---------------------------------
+ * For 6-31G molecules (nao 130-200) whose matrices exceed the LDS-resident
+ * kernels.  Same quantities as the calls above, other data paths. */
+
+/* The full s1 ERI tensor T[p][q][r][s] = (pq|rs) (N^4 elements of dtype_out)
+ * from the packed s8 array (PySCF aosym="s8", nanoDFT.py:285). */
+int pscf_eri_s8_to_s1(int dtype_in, const void* d_s8, int dtype_out, void* d_s1, int N, void* stream);
+/* dJK = J - hyb/2 K (FP64, N x N) from the s1 tensor and an FP64 density
+ * (the einsum definitions at nanoDFT.py:130-132), one pass over the tensor. */
+int pscf_jk_s1(int dtype, const void* d_s1, int N, const double* d_dm, double hyb, double* d_out, void* stream);
+/* General GEMM C (FP64) = op(A) op(B) on MFMA: inputs stored as in_type
+ * (0 BF16, 1 FP16, 2 FP32, 3 FP64), accumulation FP64 (acc64) or FP32;
+ * row-major with leading dimensions; atomic = add into C (K split over
+ * ksplit workgroups). */
+int pscf_gemm(int in_type, int acc64, int trans_a, int trans_b, int atomic, int M, int N, int K, const void* d_A,
+              int64_t lda, const void* d_B, int64_t ldb, double* d_C, int64_t ldc, int ksplit, void* stream);
+/* FP64 -> out_type (0 BF16, 1 FP16, 2 FP32, 3 FP64), round to nearest. */
+int pscf_cast_to(int out_type, const double* d_x, void* d_y, int64_t n, void* stream);
+/* Pointwise stages of exchange_correlation (nanoDFT.py:105-123) around the
+ * AO-grid GEMMs X = D phi0 and V = phi0 Y^T, AO values phi [4][N][g] stored as
+ * ao_type:  stage 0: rho_c = sum_nu X phi_c, B3LYP (b3lyp.py:13-39), E_xc
+ * block partials (pscf_xc_large_parts(g) of them), coefficients coef [4][g];
+ * stage 1: Y = sum_c coef_c phi_c (ao_type).  finish: V_xc = Vh + Vh^T and
+ * E_xc from the partials. */
+int pscf_xc_large_parts(int g);
+int pscf_xc_large_stage(int stage, int ao_type, int N, int g, const double* d_X, const void* d_phi, const double* d_w,
+                        double eps, double* d_coef, double* d_e_part, void* d_Y, void* stream);
+int pscf_xc_large_finish(int N, const double* d_Vh, const double* d_e_part, int nparts, double* d_vxc, double* d_exc,
+                         void* stream);
+/* pscf_scf_step with the working matrices in a caller workspace of
+ * pscf_scf_step_ws_bytes(N) bytes (N <= 512). */
+size_t pscf_scf_step_ws_bytes(int N);
+int pscf_scf_step_ws(int dtype, int iteration, int N, int nocc, const double* d_Hcore, const double* d_S,
+                     const double* d_Linv, const void* d_dJK, const void* d_Vxc, double* d_D, double* d_H,
+                     double* d_mo_energy, double* d_C, void* d_state, int use_diis, void* d_ws, size_t ws_bytes,
+                     void* stream);
+
 /* ---- learned initial guess --------------------------------------------------
  * No counterpart in the reference, whose SCF always starts from PySCF's MINAO
  * guess (nanoDFT.py:233); these replace that call when a trained network is

@@ -65,6 +65,20 @@ _SIGS = {
                                     c_void_p, c_int, c_void_p]),
     "pscf_batch_energy": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
+    # large molecules and the precision study
+    "pscf_eri_s8_to_s1": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "pscf_jk_s1": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p]),
+    "pscf_gemm": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                          c_void_p, c_int64, c_int, c_void_p]),
+    "pscf_cast_to": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "pscf_xc_large_parts": (c_int, [c_int]),
+    "pscf_xc_large_stage": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "pscf_xc_large_finish": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "pscf_scf_step_ws_bytes": (c_size_t, [c_int]),
+    "pscf_scf_step_ws": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_size_t,
+                                 c_void_p]),
     # learned initial guess
     "pscf_batch_purify": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),

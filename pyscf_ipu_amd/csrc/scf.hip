@@ -510,6 +510,45 @@ __global__ void __launch_bounds__(SCF_THREADS) scf_step_kernel(int it, int N, in
                       use_diis);
 }
 
+// The same step for molecules whose three N x N working matrices exceed the
+// LDS: they live in a caller workspace (L2-resident at N <= 512), the small
+// DIIS and rotation tables stay in LDS.
+constexpr int SCF_WS_MAXN = 512;
+
+__host__ __device__ inline size_t scf_ws_doubles(int N) { return 3 * (size_t)N * lds_ld(N); }
+
+template <typename T>
+__global__ void __launch_bounds__(SCF_THREADS) scf_step_ws_kernel(int it, int N, int nocc, const double* __restrict__ Hcore,
+                                                                  const double* __restrict__ S,
+                                                                  const double* __restrict__ Linv,
+                                                                  const T* __restrict__ dJK, const T* __restrict__ Vxc,
+                                                                  double* __restrict__ D, double* __restrict__ Hout,
+                                                                  double* __restrict__ mo_e, double* __restrict__ Cout,
+                                                                  double* __restrict__ state, int use_diis,
+                                                                  double* __restrict__ ws) {
+  constexpr int H = SCF_WS_MAXN / 2 + 1;
+  __shared__ double sm_d[300 + 16 + SCF_RED + 2 * H + SCF_WS_MAXN];
+  __shared__ int sm_i[2 * H + SCF_WS_MAXN];
+  ScfLds L;
+  const int NL = N * lds_ld(N);
+  L.A = ws;
+  L.U = ws + NL;
+  L.B = ws + 2 * (size_t)NL;
+  L.Bst = sm_d;
+  L.Bm = L.Bst + 100;
+  L.Bv = L.Bm + 100;
+  L.cvec = L.Bv + 100;
+  L.red = L.cvec + 16;
+  L.rc = L.red + SCF_RED;
+  L.rs = L.rc + H;
+  L.eig = L.rs + H;
+  L.rp = sm_i;
+  L.rq = L.rp + H;
+  L.rank = L.rq + H;
+  scf_step_body<T, T>(L, it, N, nocc, Hcore, S, Linv, dJK, Vxc, D, (T*)nullptr, (T*)nullptr, Hout, mo_e, Cout, state,
+                      use_diis);
+}
+
 template <typename T>
 __global__ void energy_kernel(int N, const double* __restrict__ D, const double* __restrict__ Hcore,
                               const T* __restrict__ dJK, const double* __restrict__ exc, double e_nuc,
@@ -830,6 +869,31 @@ extern "C" int pscf_scf_step(int dtype, int iteration, int N, int nocc, const do
                                                         (const float*)d_Vxc, d_D, d_H, d_mo_energy, d_C,
                                                         (double*)d_state, use_diis);
   }
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+extern "C" size_t pscf_scf_step_ws_bytes(int N) { return sizeof(double) * scf_ws_doubles(N); }
+
+extern "C" int pscf_scf_step_ws(int dtype, int iteration, int N, int nocc, const double* d_Hcore, const double* d_S,
+                                const double* d_Linv, const void* d_dJK, const void* d_Vxc, double* d_D, double* d_H,
+                                double* d_mo_energy, double* d_C, void* d_state, int use_diis, void* d_ws,
+                                size_t ws_bytes, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_scf_step_ws: bad dtype");
+  PSCF_REQUIRE(N > 1 && N <= SCF_WS_MAXN && nocc >= 0 && nocc <= N && iteration >= 0,
+               "pscf_scf_step_ws: bad sizes (N <= %d)", SCF_WS_MAXN);
+  PSCF_REQUIRE(d_Hcore && d_S && d_Linv && d_dJK && d_Vxc && d_D && d_H && d_mo_energy && d_C && d_state && d_ws,
+               "pscf_scf_step_ws: null pointer");
+  PSCF_REQUIRE(ws_bytes >= sizeof(double) * scf_ws_doubles(N), "pscf_scf_step_ws: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    scf_step_ws_kernel<double><<<1, SCF_THREADS, 0, st>>>(iteration, N, nocc, d_Hcore, d_S, d_Linv,
+                                                          (const double*)d_dJK, (const double*)d_Vxc, d_D, d_H,
+                                                          d_mo_energy, d_C, (double*)d_state, use_diis, (double*)d_ws);
+  else
+    scf_step_ws_kernel<float><<<1, SCF_THREADS, 0, st>>>(iteration, N, nocc, d_Hcore, d_S, d_Linv, (const float*)d_dJK,
+                                                         (const float*)d_Vxc, d_D, d_H, d_mo_energy, d_C,
+                                                         (double*)d_state, use_diis, (double*)d_ws);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
