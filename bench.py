@@ -80,7 +80,7 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
             continue
 
         def once():
-            return lg.solve(mols, mdl, dtype="fp32", max_its=its, learned_its=15, conv_tol=1e-6, graph=True)
+            return lg.solve(mols, mdl, dtype="fp32", max_its=its, learned_its=15, conv_tol=1e-6, graph=True, tune=True)
 
         for _ in range(warmup):
             once()
@@ -98,7 +98,11 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
                      "ms_per_step": wall * 1e3, "passed_1mha": int(ok.sum()), "max_abs_dE_vs_fp64_ha": float(dE.max()),
                      "mean_abs_dE_vs_fp64_ha": float(dE.mean()),
                      "mean_scf_iters": float(g.iterations_total.mean()), "fallback_to_minao": int(g.fallback.sum()),
-                     "converged": int(res.converged.sum())}
+                     "converged": int(res.converged.sum()),
+                     "autotune_ms_per_launch": {k: {a: round(b, 3) for a, b in v.items()}
+                                                for k, v in (g.tuning or {}).get("ms", {}).items()},
+                     "autotune_chosen": {"xc_tile": (g.tuning or {}).get("chosen_tile"),
+                                         "split": (g.tuning or {}).get("chosen_split")}}
     common = {"mean_scf_iters_fp64_minao": float(r64.iterations.mean()),
               "fp64_reference_converged": int(r64.converged.sum()),
               "dtype": "f32 (ERI/JK/XC/Becke) + f64 (eigh/DIIS/energy)",

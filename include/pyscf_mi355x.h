@@ -193,15 +193,24 @@ int pscf_batch_becke(int dtype, const void* d_mols, const int* d_ptmol, const in
 /* AO values and gradients of all points (eval_gto, nanoDFT.py:219-220). */
 int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, const double* d_coords, int64_t G,
                   const int* d_atm, const int* d_bas, const double* d_env, void* d_ao, void* stream);
-/* E_xc (d_exc[nmol]) and V_xc of all active molecules (nanoDFT.py:105-123). */
+/* E_xc (d_exc[m]) and V_xc of the active molecules d_order[0..nmol)
+ * (nanoDFT.py:105-123), grid tiles of `tile` points (32, or 64 in FP32);
+ * max_nao bounds the nao of these molecules (it sizes the LDS).  split
+ * workgroups share each molecule's tiles; split > 1 needs part buffers of
+ * split * mat_total elements of dtype and split * m_total doubles
+ * (mat_total = length of the concatenated N x N matrices, m_total = number
+ * of molecules in the descriptors). */
 int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                   const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
+                  int split, int tile, void* d_part_v, double* d_part_e, int64_t mat_total, int m_total,
                   void* stream);
 /* dJK of all active molecules from their packed s8 ERIs (sparse_symmetric_ERI.py:86-137).
  * accumulate = 0: d_out (dtype) = dJK(d_dm); accumulate = 1: d_out (FP64) +=
- * dJK(d_dm), the incremental build from density changes. */
+ * dJK(d_dm), the incremental build from density changes.  split workgroups
+ * share each molecule's ERI rows (atomic accumulation). */
 int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
-                     const void* d_eri, const void* d_dm, double hyb, void* d_out, int accumulate, void* stream);
+                     const void* d_eri, const void* d_dm, double hyb, void* d_out, int accumulate, int split,
+                     void* stream);
 /* One SCF step of all active molecules (nanoDFT.py:64-72, 338-393); iteration
  * index per molecule from d_ctl; dJK in FP64 (written or accumulated by
  * pscf_batch_jk_s8), V_xc in dtype; writes D (FP64), D in dtype (d_Dt) and,

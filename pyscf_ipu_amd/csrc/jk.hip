@@ -196,10 +196,12 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
 }
 
 // ------------------------------------------------------------- batched ----
-// One workgroup per molecule (molecule order[blockIdx.x]); converged molecules
-// (ctl active == 0) are skipped.  dJK = M + M^T is written directly, or, with
-// ACC, added to an FP64 accumulator (incremental build from density changes:
-// the FP32 rounding then scales with |dD| instead of |D|).
+// Workgroup (blockIdx.x, blockIdx.y) takes molecule order[blockIdx.x] and the
+// rows pq of part blockIdx.y; converged molecules (ctl active == 0) are
+// skipped.  dJK = M + M^T is written directly (one part), or added with
+// atomics (several parts: the caller zeroes the output first unless ACC), or,
+// with ACC, added to an FP64 accumulator (incremental build from density
+// changes: the FP32 rounding then scales with |dD| instead of |D|).
 template <typename T, typename TO, bool ACC>
 __global__ void __launch_bounds__(JK_THREADS) jk_s8_batch_kernel(const BatchMol* __restrict__ mols,
                                                                  const int* __restrict__ order,
@@ -229,19 +231,34 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_batch_kernel(const BatchMol*
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
-  for (int pq = wave; pq < P; pq += nwaves) {
+  const int wave = (threadIdx.x >> 6) + blockIdx.y * nwaves;
+  for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
     const unsigned pp = pair[pq];
     s8_row<T>(eri + tri(pq), pq, pp & 0xffff, pp >> 16, dm, acc, pair, N, hhalf, lane);
   }
   __syncthreads();
   TO* out = out_all + Mo.mat0;
+  const bool atomic = gridDim.y > 1;
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
     const TO v = (TO)(acc[x] + acc[j * N + i]);
-    out[x] = ACC ? out[x] + v : v;
+    if (atomic)
+      atomicAdd(out + x, v);
+    else
+      out[x] = ACC ? out[x] + v : v;
   }
+}
+
+// Zeroes the dJK blocks of the active molecules (before a split build).
+template <typename TO>
+__global__ void __launch_bounds__(256) zero_active_kernel(const BatchMol* __restrict__ mols,
+                                                          const double* __restrict__ ctl, TO* __restrict__ out) {
+  const int m = blockIdx.x;
+  if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
+  const BatchMol Mo = mols[m];
+  const int NN = (int)(Mo.nao * Mo.nao);
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) out[Mo.mat0 + x] = TO(0);
 }
 
 template <typename T>
@@ -339,22 +356,31 @@ extern "C" int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_
 
 extern "C" int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
                                 const double* d_ctl, const void* d_eri, const void* d_dm, double hyb, void* d_out,
-                                int accumulate, void* stream) {
+                                int accumulate, int split, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_jk_s8: bad dtype");
   PSCF_REQUIRE(nmol > 0 && max_nao > 0 && max_nao < 65536 && d_mols && d_order && d_eri && d_dm && d_out,
                "pscf_batch_jk_s8: bad arguments");
+  PSCF_REQUIRE(split >= 1 && split <= 64, "pscf_batch_jk_s8: bad split %d", split);
   hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(nmol, split);
+  if (split > 1 && !accumulate) {
+    if (dtype == PSCF_F64)
+      zero_active_kernel<double><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_ctl, (double*)d_out);
+    else
+      zero_active_kernel<float><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_ctl, (float*)d_out);
+    PSCF_HIP(hipGetLastError());
+  }
   if (dtype == PSCF_F64) {
     const size_t lds = jk_s8_lds<double>(max_nao);
     PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
     if (accumulate) {
       PSCF_SET_LDS((jk_s8_batch_kernel<double, double, true>), lds);
-      jk_s8_batch_kernel<double, double, true><<<nmol, JK_THREADS, lds, st>>>(
+      jk_s8_batch_kernel<double, double, true><<<grid, JK_THREADS, lds, st>>>(
           (const BatchMol*)d_mols, d_order, d_ctl, (const double*)d_eri, (const double*)d_dm, hyb / 2.0,
           (double*)d_out);
     } else {
       PSCF_SET_LDS((jk_s8_batch_kernel<double, double, false>), lds);
-      jk_s8_batch_kernel<double, double, false><<<nmol, JK_THREADS, lds, st>>>(
+      jk_s8_batch_kernel<double, double, false><<<grid, JK_THREADS, lds, st>>>(
           (const BatchMol*)d_mols, d_order, d_ctl, (const double*)d_eri, (const double*)d_dm, hyb / 2.0,
           (double*)d_out);
     }
@@ -363,12 +389,12 @@ extern "C" int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_orde
     PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
     if (accumulate) {
       PSCF_SET_LDS((jk_s8_batch_kernel<float, double, true>), lds);
-      jk_s8_batch_kernel<float, double, true><<<nmol, JK_THREADS, lds, st>>>(
+      jk_s8_batch_kernel<float, double, true><<<grid, JK_THREADS, lds, st>>>(
           (const BatchMol*)d_mols, d_order, d_ctl, (const float*)d_eri, (const float*)d_dm, (float)(hyb / 2.0),
           (double*)d_out);
     } else {
       PSCF_SET_LDS((jk_s8_batch_kernel<float, float, false>), lds);
-      jk_s8_batch_kernel<float, float, false><<<nmol, JK_THREADS, lds, st>>>(
+      jk_s8_batch_kernel<float, float, false><<<grid, JK_THREADS, lds, st>>>(
           (const BatchMol*)d_mols, d_order, d_ctl, (const float*)d_eri, (const float*)d_dm, (float)(hyb / 2.0),
           (float*)d_out);
     }

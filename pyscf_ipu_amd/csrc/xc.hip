@@ -188,18 +188,17 @@ constexpr int XM_MAXN = 80;  // nao limit of the MFMA path
 __host__ __device__ inline int xm_np(int N) { return (N + 15) & ~15; }
 __host__ __device__ inline int xm_xs(int N) { return xm_np(N) + 4; }
 __host__ __device__ inline int xm_xd(int N) { return xm_np(N) <= 16 ? 16 : 80; }
-template <typename T>
+template <typename T, int XT = XMTraits<T>::TILE>
 __host__ __device__ inline size_t xm_lds_elems(int N) {
-  constexpr int XT = XMTraits<T>::TILE;
   const int NP = xm_np(N);
   return (size_t)4 * XT * xm_xs(N) + (size_t)XT * xm_xd(N) + (size_t)NP * xm_xd(N) + 256 + 4 * XT;
 }
 
-template <typename T>
+template <typename T, int XT = XMTraits<T>::TILE>
 __device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
                                      typename XMTraits<T>::v4 (&vacc)[XM_MAXB], double& e_loc) {
   typedef typename XMTraits<T>::v4 v4;
-  constexpr int XT = XMTraits<T>::TILE, PB = XT / 16, NSLOT = 4 / PB;
+  constexpr int PB = XT / 16, NSLOT = 4 / PB;
   const int N = a.N, NP = xm_np(N), NB = NP / 16, XS = xm_xs(N), XD = xm_xd(N);
   T* Phi = reinterpret_cast<T*>(smem);  // [4][XT][XS]
   T* Y = Phi + 4 * XT * XS;             // [XT][XD]
@@ -227,11 +226,29 @@ __device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, in
   for (int tile = tile0; tile < ntiles; tile += tstride) {
     const int p0 = tile * XT;
     __syncthreads();
-    for (int x = tid; x < 4 * per_c; x += blockDim.x) {
-      const int c = x / per_c, r = x - c * per_c;
-      const int mu = r / XT, p = r - mu * XT;
-      const int pt = p0 + p;
-      Phi[(c * XT + p) * XS + mu] = pt < a.g ? a.ao[c * a.sc + mu * a.smu + (int64_t)pt * a.sp] : T(0);
+    // eight independent loads in flight per thread before their LDS stores
+    for (int x0 = tid; x0 < 4 * per_c; x0 += 8 * blockDim.x) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int x = x0 + u * blockDim.x;
+        v[u] = T(0);
+        if (x < 4 * per_c) {
+          const int row = x / XT, p = x - row * XT;  // row = c * N + mu
+          const int c = row / N, mu = row - c * N;
+          const int pt = p0 + p;
+          if (pt < a.g) v[u] = a.ao[c * a.sc + mu * a.smu + (int64_t)pt * a.sp];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int x = x0 + u * blockDim.x;
+        if (x < 4 * per_c) {
+          const int row = x / XT, p = x - row * XT;
+          const int c = row / N, mu = row - c * N;
+          Phi[(c * XT + p) * XS + mu] = v[u];
+        }
+      }
     }
     __syncthreads();
     // GEMM 1 and the density epilogue
@@ -359,14 +376,19 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
   if (threadIdx.x == 0) a.part_e[blockIdx.x] = e;
 }
 
-// Batched: one workgroup per molecule (order[blockIdx.x]); writes V + V^T and
-// E_xc of that molecule; skips converged molecules.
-template <typename T>
+// Batched: workgroup (blockIdx.x, blockIdx.y) takes molecule order[blockIdx.x]
+// and its tiles blockIdx.y, blockIdx.y + gridDim.y, ...; skips converged
+// molecules.  With one part per molecule it writes V + V^T and E_xc directly;
+// otherwise V and E_xc partials go to part_v (part-major over the concatenated
+// matrices) and part_e ([part][molecule]) for xc_batch_reduce_kernel.
+template <typename T, int XT>
 __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __restrict__ mols,
                                                               const int* __restrict__ order,
                                                               const double* __restrict__ ctl, const T* __restrict__ ao,
                                                               const T* __restrict__ dm, const T* __restrict__ w,
-                                                              T eps, T* __restrict__ vxc, double* __restrict__ exc) {
+                                                              T eps, T* __restrict__ vxc, double* __restrict__ exc,
+                                                              T* __restrict__ part_v, double* __restrict__ part_e,
+                                                              int64_t mat_total, int nmol) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
@@ -383,10 +405,18 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   a.eps = eps;
   double e_loc = 0.0;
   const int N = a.N;
-  T* Vl = reinterpret_cast<T*>(smem);
+  const int part = blockIdx.y, nparts = gridDim.y;
   typename XMTraits<T>::v4 vacc[XM_MAXB];
-  xc_tiles_mfma<T>(a, smem, 0, 1, vacc, e_loc);
+  xc_tiles_mfma<T, XT>(a, smem, part, nparts, vacc, e_loc);
   __syncthreads();
+  double* red = reinterpret_cast<double*>(smem + sizeof(T) * N * N);
+  if (nparts > 1) {
+    xm_store_v<T>(vacc, N, part_v + part * mat_total + Mo.mat0, N);
+    const double e = xc_block_energy(e_loc, red);
+    if (threadIdx.x == 0) part_e[(int64_t)part * nmol + m] = e;
+    return;
+  }
+  T* Vl = reinterpret_cast<T*>(smem);
   xm_store_v<T>(vacc, N, Vl, N);
   __syncthreads();
   T* out = vxc + Mo.mat0;
@@ -394,10 +424,39 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
     const int i = x / N, j = x - i * N;
     out[x] = Vl[x] + Vl[j * N + i];
   }
-  double* red = reinterpret_cast<double*>(smem + sizeof(T) * N * N);
   __syncthreads();
   const double e = xc_block_energy(e_loc, red);
   if (threadIdx.x == 0) exc[m] = e;
+}
+
+// Sums the parts of xc_batch_kernel: V_xc = sum_k (V_k + V_k^T), E_xc = sum_k e_k
+// (fixed order), one workgroup per active molecule.
+template <typename T>
+__global__ void __launch_bounds__(256) xc_batch_reduce_kernel(const BatchMol* __restrict__ mols,
+                                                              const int* __restrict__ order,
+                                                              const double* __restrict__ ctl,
+                                                              const T* __restrict__ part_v,
+                                                              const double* __restrict__ part_e, int64_t mat_total,
+                                                              int nmol, int nparts, T* __restrict__ vxc,
+                                                              double* __restrict__ exc) {
+  const int m = order[blockIdx.x];
+  if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
+  const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao;
+  for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    T v = T(0);
+    for (int k = 0; k < nparts; ++k) {
+      const T* P = part_v + k * mat_total + Mo.mat0;
+      v += P[x] + P[j * N + i];
+    }
+    vxc[Mo.mat0 + x] = v;
+  }
+  if (threadIdx.x == 0) {
+    double e = 0.0;
+    for (int k = 0; k < nparts; ++k) e += part_e[(int64_t)k * nmol + m];
+    exc[m] = e;
+  }
 }
 
 // Grid points of one atom's template: chunk (atom, first point) per workgroup.
@@ -733,30 +792,46 @@ extern "C" int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, 
   return PSCF_OK;
 }
 
+template <typename T, int XT>
+static int batch_xc_launch(const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
+                           const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc,
+                           double* d_exc, int split, void* d_part_v, double* d_part_e, int64_t mat_total,
+                           int m_total, hipStream_t st) {
+  const size_t main = sizeof(T) * xm_lds_elems<T, XT>(max_nao);
+  const size_t tail = sizeof(T) * (size_t)max_nao * max_nao + 8 * sizeof(double);
+  const size_t lds = (main > tail ? main : tail) + 8 * sizeof(double);
+  PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
+  PSCF_SET_LDS((xc_batch_kernel<T, XT>), lds);
+  xc_batch_kernel<T, XT><<<dim3(nmol, split), XC_THREADS, lds, st>>>(
+      (const BatchMol*)d_mols, d_order, d_ctl, (const T*)d_ao, (const T*)d_dm, (const T*)d_w, (T)eps, (T*)d_vxc,
+      d_exc, (T*)d_part_v, d_part_e, mat_total, m_total);
+  PSCF_HIP(hipGetLastError());
+  if (split > 1) {
+    xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, d_ctl, (const T*)d_part_v,
+                                                   d_part_e, mat_total, m_total, split, (T*)d_vxc, d_exc);
+    PSCF_HIP(hipGetLastError());
+  }
+  return PSCF_OK;
+}
+
 extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
                              const double* d_ctl, const void* d_ao, const void* d_dm, const void* d_w, double eps,
-                             void* d_vxc, double* d_exc, void* stream) {
+                             void* d_vxc, double* d_exc, int split, int tile, void* d_part_v, double* d_part_e,
+                             int64_t mat_total, int m_total, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_xc: bad dtype");
   PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && d_ao && d_dm && d_w && d_vxc && d_exc,
                "pscf_batch_xc: bad arguments");
-  PSCF_REQUIRE(max_nao <= (dtype == PSCF_F64 ? xc_max_nao<double>() : xc_max_nao<float>()),
-               "pscf_batch_xc: nao %d too large", max_nao);
+  PSCF_REQUIRE(split >= 1 && split <= 64 && (split == 1 || (d_part_v && d_part_e && mat_total > 0 && m_total > 0)),
+               "pscf_batch_xc: split %d needs part buffers", split);
+  PSCF_REQUIRE(tile == 32 || (tile == 64 && dtype == PSCF_F32), "pscf_batch_xc: tile %d (32, or 64 in FP32)", tile);
+  PSCF_REQUIRE(max_nao <= XM_MAXN, "pscf_batch_xc: nao %d too large", max_nao);
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == PSCF_F64) {
-    const size_t lds = xc_lds<double>(max_nao);
-    PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
-    PSCF_SET_LDS(xc_batch_kernel<double>, lds);
-    xc_batch_kernel<double><<<nmol, XC_THREADS, lds, st>>>((const BatchMol*)d_mols, d_order, d_ctl,
-                                                           (const double*)d_ao, (const double*)d_dm,
-                                                           (const double*)d_w, eps, (double*)d_vxc, d_exc);
-  } else {
-    const size_t lds = xc_lds<float>(max_nao);
-    PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
-    PSCF_SET_LDS(xc_batch_kernel<float>, lds);
-    xc_batch_kernel<float><<<nmol, XC_THREADS, lds, st>>>((const BatchMol*)d_mols, d_order, d_ctl,
-                                                          (const float*)d_ao, (const float*)d_dm, (const float*)d_w,
-                                                          (float)eps, (float*)d_vxc, d_exc);
-  }
-  PSCF_HIP(hipGetLastError());
-  return PSCF_OK;
+  if (dtype == PSCF_F64)
+    return batch_xc_launch<double, 32>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
+                                       split, d_part_v, d_part_e, mat_total, m_total, st);
+  if (tile == 64)
+    return batch_xc_launch<float, 64>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
+                                      split, d_part_v, d_part_e, mat_total, m_total, st);
+  return batch_xc_launch<float, 32>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
+                                    split, d_part_v, d_part_e, mat_total, m_total, st);
 }

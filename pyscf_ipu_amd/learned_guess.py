@@ -257,20 +257,22 @@ class GuessedResult:
     result: BatchResult          # merged per-molecule result
     fallback: np.ndarray         # (M,) bool: re-run from MINAO
     iterations_total: np.ndarray  # learned-guess iterations + fallback iterations
+    tuning: dict = None          # autotuner record of the main batch (BatchRKS.tuning)
 
 
 def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_its: int = 20,
-          learned_its: int = 15, conv_tol: float = 1e-7, graph: bool = True) -> GuessedResult:
+          learned_its: int = 15, conv_tol: float = 1e-7, graph: bool = True, tune: bool = False) -> GuessedResult:
     """Batched SCF from the learned guess; molecules not converged after
     `learned_its` iterations are re-run from MINAO with `max_its` iterations."""
     b = BatchRKS(molecules, dtype=dtype)
     if model is not None:
         b.setup(guess=learned_guess(model))
-        r = b.run(max_its=learned_its, conv_tol=conv_tol, graph=graph)
+        r = b.run(max_its=learned_its, conv_tol=conv_tol, graph=graph, tune=tune)
     else:
         b.setup()
-        r = b.run(max_its=max_its, conv_tol=conv_tol, graph=graph)
-        return GuessedResult(r, np.zeros(len(molecules), bool), r.iterations.copy())
+        r = b.run(max_its=max_its, conv_tol=conv_tol, graph=graph, tune=tune)
+        return GuessedResult(r, np.zeros(len(molecules), bool), r.iterations.copy(), b.tuning)
+    tuning = b.tuning
     redo = np.where(~r.converged & ~r.failed)[0]
     total = r.iterations.copy()
     fb = np.zeros(len(molecules), bool)
@@ -288,4 +290,4 @@ def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_i
             total[i] += r2.iterations[k]
         fb[redo] = True
         r.n_iter_launches += r2.n_iter_launches
-    return GuessedResult(r, fb, total)
+    return GuessedResult(r, fb, total, tuning)

@@ -171,10 +171,16 @@ class RKS:
                   device.ptr(S), device.ptr(T), device.ptr(V), device.stream())
         self.S = S
         self.Hcore = (T + V).contiguous()
-        S_h = S.cpu().numpy()
-        L = np.linalg.cholesky(S_h)
-        self.L_inv_h = np.linalg.inv(L)
-        self.L_inv = device.to_dev(self.L_inv_h)
+        # overlap Cholesky -> L^-1 and the atomic-density guess on the device
+        # (the batched kernels over a one-molecule descriptor)
+        desc = np.zeros((1, 16), np.int64)
+        desc[0, :4] = (mol.natm, mol.nbas, N, self.nocc)
+        self.d_desc = device.to_dev(desc)
+        ctl = device.zeros((1, 8), f64)
+        self.L_inv = device.empty((N, N), f64)
+        _lib.call("pscf_batch_chol", device.ptr(self.d_desc), 1, N, device.ptr(S), device.ptr(self.L_inv),
+                  device.ptr(ctl), device.stream())
+        self._ctl1 = ctl
         # grid: atom-centred templates on the host, Becke weights on the device
         g = self.grid if self.grid is not None else grids.atomic_grids(mol, self.level)
         self.g = g.size
@@ -197,7 +203,14 @@ class RKS:
         with _timed(self.events, "eri_s8"):
             self.eri = direct.eri_s8(mol, self.dtype, dev_mol=self.dev_mol)
         # work buffers
-        self.D = device.to_dev(guess.minao(mol, S_h) if dm0 is None else dm0, f64)
+        if dm0 is None:
+            self.D = device.empty((N, N), f64)
+            shell_ao = device.to_dev(mol.ao_loc_nr()[:-1].astype(np.int32))
+            _lib.call("pscf_batch_guess", _lib.F64, device.ptr(self.d_desc), 1, device.ptr(atm), device.ptr(bas),
+                      device.ptr(shell_ao), device.ptr(S), device.ptr(self.D), None, device.stream())
+            self._shell_ao = shell_ao
+        else:
+            self.D = device.to_dev(dm0, f64)
         self.Dt = self.D.to(self.tdt) if self.tdt != f64 else self.D
         self.dJK = device.empty((N, N), self.tdt)
         self.Vxc = device.empty((N, N), self.tdt)

@@ -64,3 +64,32 @@ def test_batch_matches_plain_path(hip):
     p = RKS(build_mol(s), level=0, dtype="fp64").run(its=30, conv_tol=1e-9)
     assert abs(b.energy[0] - p.energy) < 1e-8
     assert b.iterations[0] == p.iterations
+
+
+def test_batch_split_and_tuner_fp64(hip):
+    """Several workgroups per molecule (XC tiles / J/K rows split, partials
+    reduced or atomically added) give the one-workgroup result, and the
+    autotuner picks one of its candidates for each launch."""
+    mols = batch.parse(STRS)
+    ref = batch.BatchRKS(mols, dtype="fp64").run(max_its=30, conv_tol=1e-9, graph=False)
+    b = batch.BatchRKS(mols, dtype="fp64")
+    b.setup()
+    tun = b.tune()
+    assert set(tun["chosen_split"]) == {"xc", "jk"} and all(v in batch.SPLITS for v in tun["chosen_split"].values())
+    assert b.tune().get("cached")
+    b.split = {"xc": 4, "jk": 8}
+    got = b.run(max_its=30, conv_tol=1e-9, graph=True)
+    assert np.abs(got.energy - ref.energy).max() < 1e-9
+    assert np.array_equal(got.iterations, ref.iterations)
+
+
+def test_batch_fp32_tiles_agree(hip):
+    """FP32 XC with 64-point tiles and split workgroups equals 32-point tiles."""
+    mols = batch.parse(STRS)
+    runs = []
+    for tile, split in ((32, 1), (64, 2)):
+        b = batch.BatchRKS(mols, dtype="fp32")
+        b.setup()
+        b.xc_tile, b.split = tile, {"xc": split, "jk": split}
+        runs.append(b.run(max_its=30, conv_tol=1e-6, graph=False))
+    assert np.abs(runs[0].energy - runs[1].energy).max() < 1e-4  # FP32 summation-order noise at a 1e-6 stop
