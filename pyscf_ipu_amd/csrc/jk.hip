@@ -111,49 +111,64 @@ __global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
 // targets (p, r) and (q, r) repeat over runs of consecutive rs with equal r, so
 // those are summed per run with a segmented suffix scan across the wave and
 // added once per run instead of by every lane.
+constexpr int S8_GROUP = 4;
+
 template <typename T>
 __device__ inline void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
                               T* __restrict__ acc, const unsigned* __restrict__ pair, int N, T hhalf, int lane) {
   const T dpq = dm[p * N + q];
   T jrow = T(0);
-  for (int base = 0; base <= pq; base += 64) {
-    const int rs = base + lane;
-    const bool valid = rs <= pq;
-    int r = -1, s = 0;
-    T a = T(0), b = T(0);
-    if (valid) {
-      const unsigned rr = pair[rs];
-      r = rr & 0xffff;
-      s = rr >> 16;
-      // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
-      T v = row[rs];
-      if (p == q) v *= T(0.5);
-      if (r == s) v *= T(0.5);
-      if (pq == rs) v *= T(0.5);
-      const T v2 = v + v;
-      atomicAdd(acc + r * N + s, v2 * dpq);
-      jrow += v2 * dm[r * N + s];
-      const T vk = -hhalf * v;
-      atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
-      atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
-      a = vk * dm[q * N + s];
-      b = vk * dm[p * N + s];
-    }
-    // segmented suffix sums over lanes of equal r (r is nondecreasing in rs)
+  // the row is read in groups of S8_GROUP chunks of 64 elements whose global
+  // loads are all issued before any is used, so their latency overlaps
+  for (int gbase = 0; gbase <= pq; gbase += 64 * S8_GROUP) {
+    T vv[S8_GROUP];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const T ta = __shfl_down(a, off, 64);
-      const T tb = __shfl_down(b, off, 64);
-      const int tr = __shfl_down(r, off, 64);
-      if (lane + off < 64 && tr == r) {
-        a += ta;
-        b += tb;
-      }
+    for (int u = 0; u < S8_GROUP; ++u) {
+      const int rs = gbase + u * 64 + lane;
+      vv[u] = rs <= pq ? row[rs] : T(0);
     }
-    const int rprev = __shfl_up(r, 1, 64);
-    if (valid && (lane == 0 || rprev != r)) {
-      atomicAdd(acc + p * N + r, a);
-      atomicAdd(acc + q * N + r, b);
+#pragma unroll
+    for (int u = 0; u < S8_GROUP; ++u) {
+      const int base = gbase + u * 64;
+      if (base > pq) continue;  // uniform across the wavefront
+      const int rs = base + lane;
+      const bool valid = rs <= pq;
+      int r = -1, s = 0;
+      T a = T(0), b = T(0);
+      if (valid) {
+        const unsigned rr = pair[rs];
+        r = rr & 0xffff;
+        s = rr >> 16;
+        // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
+        T v = vv[u];
+        if (p == q) v *= T(0.5);
+        if (r == s) v *= T(0.5);
+        if (pq == rs) v *= T(0.5);
+        const T v2 = v + v;
+        atomicAdd(acc + r * N + s, v2 * dpq);
+        jrow += v2 * dm[r * N + s];
+        const T vk = -hhalf * v;
+        atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
+        atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
+        a = vk * dm[q * N + s];
+        b = vk * dm[p * N + s];
+      }
+      // segmented suffix sums over lanes of equal r (r is nondecreasing in rs)
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const T ta = __shfl_down(a, off, 64);
+        const T tb = __shfl_down(b, off, 64);
+        const int tr = __shfl_down(r, off, 64);
+        if (lane + off < 64 && tr == r) {
+          a += ta;
+          b += tb;
+        }
+      }
+      const int rprev = __shfl_up(r, 1, 64);
+      if (valid && (lane == 0 || rprev != r)) {
+        atomicAdd(acc + p * N + r, a);
+        atomicAdd(acc + q * N + r, b);
+      }
     }
   }
 #pragma unroll
