@@ -102,6 +102,7 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
                      "autotune_ms_per_launch": {k: {a: round(b, 3) for a, b in v.items()}
                                                 for k, v in (g.tuning or {}).get("ms", {}).items()},
                      "autotune_chosen": {"xc_tile": (g.tuning or {}).get("chosen_tile"),
+                                         "jk_threads": (g.tuning or {}).get("chosen_jk_threads"),
                                          "split": (g.tuning or {}).get("chosen_split")}}
     common = {"mean_scf_iters_fp64_minao": float(r64.iterations.mean()),
               "fp64_reference_converged": int(r64.converged.sum()),

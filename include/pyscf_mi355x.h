@@ -204,13 +204,15 @@ int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, i
                   const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
                   int split, int tile, void* d_part_v, double* d_part_e, int64_t mat_total, int m_total,
                   void* stream);
-/* dJK of all active molecules from their packed s8 ERIs (sparse_symmetric_ERI.py:86-137).
- * accumulate = 0: d_out (dtype) = dJK(d_dm); accumulate = 1: d_out (FP64) +=
- * dJK(d_dm), the incremental build from density changes.  split workgroups
+/* dJK of the active molecules d_order[0..nmol) from their packed s8 ERIs
+ * (sparse_symmetric_ERI.py:86-137); max_nao bounds their nao (it sizes the
+ * LDS; a larger molecule gets NaN).  accumulate = 0: d_out (dtype) =
+ * dJK(d_dm); accumulate = 1: d_out (FP64) += dJK(d_dm), the incremental build
+ * from density changes.  split workgroups of `threads` (256 or 512) threads
  * share each molecule's ERI rows (atomic accumulation). */
 int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                      const void* d_eri, const void* d_dm, double hyb, void* d_out, int accumulate, int split,
-                     void* stream);
+                     int threads, void* stream);
 /* One SCF step of all active molecules (nanoDFT.py:64-72, 338-393); iteration
  * index per molecule from d_ctl; dJK in FP64 (written or accumulated by
  * pscf_batch_jk_s8), V_xc in dtype; writes D (FP64), D in dtype (d_Dt) and,

@@ -31,6 +31,7 @@ DIIS_N = 9
 CTL = 8
 SPLITS = (1, 2, 4, 8)   # workgroups per molecule tried by the autotuner
 MAX_SPLIT = max(SPLITS)
+JK_THREADS = (256, 512)   # J/K workgroup sizes tried by the autotuner
 XC_TILES = {"fp32": (32, 64), "fp64": (32,)}   # grid points per XC tile
 _TUNE_CACHE = {}        # tuning results per batch shape
 CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
@@ -84,6 +85,7 @@ class BatchRKS:
         self.tdt, self.code, _ = device.DTYPES[dtype]
         self.split = {"xc": 1, "jk": 1}
         self.xc_tile = 32
+        self.jk_threads = 256
         self.tuning = None
         self._assemble()
 
@@ -192,13 +194,13 @@ class BatchRKS:
             self.eri_work.append(_chunks(nq, 256))
         self.order = np.argsort(-nao, kind="stable").astype(np.int32)
         self.max_nao = int(nao.max())
-        # XC launches per class of nao rounded up to 16 (LDS sized per class):
-        # (first position in `order`, count, largest nao of the class)
+        # XC and J/K launches per class of nao rounded up to 16 (LDS sized per
+        # class): (first position in `order`, count, largest nao of the class)
         npad = (nao[self.order] + 15) // 16
-        self.xc_classes = []
+        self.nao_classes = []
         for v in np.unique(npad)[::-1]:
             pos = np.where(npad == v)[0]
-            self.xc_classes.append((int(pos[0]), len(pos), int(nao[self.order[pos[0]]])))
+            self.nao_classes.append((int(pos[0]), len(pos), int(nao[self.order[pos[0]]])))
         # nuclear repulsion, vectorised over molecules of equal atom count
         e_nuc = np.zeros(M)
         for na in np.unique(natm):
@@ -291,43 +293,48 @@ class BatchRKS:
     # ---------------------------------------------------------------- iterations ---
     def _xc(self, ctl_ptr, split, tile, vxc=None, exc=None):
         P, st = device.ptr, device.stream()
-        for start, count, cmax in self.xc_classes:
+        for start, count, cmax in self.nao_classes:
             order = ctypes.c_void_p(self.d_order.data_ptr() + 4 * start)
             _lib.call("pscf_batch_xc", self.code, P(self.d_desc), order, count, cmax, ctl_ptr, P(self.ao), P(self.Dt),
                       P(self.w), 1e-20, P(self.Vxc if vxc is None else vxc), P(self.exc if exc is None else exc),
                       split, tile, P(self.xc_parts), P(self.xc_parts_e), self.sizes["mat"], self.M, st)
 
-    def _jk(self, ctl_ptr, split, dm, out=None):
+    def _jk(self, ctl_ptr, split, threads, dm, out=None):
         P, st = device.ptr, device.stream()
-        _lib.call("pscf_batch_jk_s8", self.code, P(self.d_desc), P(self.d_order), self.M, self.max_nao, ctl_ptr,
-                  P(self.eri), P(dm), 0.2, P(self.dJK if out is None else out), int(self.incremental), split, st)
+        for start, count, cmax in self.nao_classes:
+            order = ctypes.c_void_p(self.d_order.data_ptr() + 4 * start)
+            _lib.call("pscf_batch_jk_s8", self.code, P(self.d_desc), order, count, cmax, ctl_ptr, P(self.eri), P(dm),
+                      0.2, P(self.dJK if out is None else out), int(self.incremental), split, threads, st)
 
     def _fock(self, ctl_ptr, first=False):
         self._xc(ctl_ptr, self.split["xc"], self.xc_tile)
         dm = self.Dt if (first or not self.incremental) else self.dDt
-        self._jk(ctl_ptr, self.split["jk"], dm)
+        self._jk(ctl_ptr, self.split["jk"], self.jk_threads, dm)
 
     def tune(self, repeats: int = 2):
         """Autotuner: times the XC launch for every (tile, split) candidate and
-        the J/K launch for every split (workgroups per molecule) on this batch
-        (all molecules active, current density, scratch outputs), keeps the
-        fastest, and caches the choice for batches of the same shape.
+        the J/K launch for every (workgroup size, split) candidate on this
+        batch (all molecules active, current density, scratch outputs), keeps
+        the fastest, and caches the choice for batches of the same shape.
         `self.tuning` records the timings (ms per launch) next to the fixed
-        tiling (32-point tiles, one workgroup per molecule)."""
+        tiling (32-point tiles, 256-thread J/K workgroups, one workgroup per
+        molecule)."""
         key = (self.dtype, self.M, self.max_nao, self.sizes["eri"], self.sizes["grid"])
         if key in _TUNE_CACHE:
             self.tuning = dict(_TUNE_CACHE[key], cached=True)
             self.split = dict(self.tuning["chosen_split"])
             self.xc_tile = self.tuning["chosen_tile"]
+            self.jk_threads = self.tuning["chosen_jk_threads"]
             return self.tuning
         P = device.ptr
         ctl = P(self.ctl)
         vxc, exc = torch.empty_like(self.Vxc), torch.empty_like(self.exc)
         out = torch.zeros_like(self.dJK)
         cands = {"xc": [(t, k) for t in XC_TILES[self.dtype] for k in SPLITS[:3]],
-                 "jk": [(0, k) for k in SPLITS]}
+                 "jk": [(t, k) for t in JK_THREADS for k in SPLITS]}
         fns = {"xc": lambda c: self._xc(ctl, c[1], c[0], vxc, exc),
-               "jk": lambda c: self._jk(ctl, c[1], self.Dt, out)}
+               "jk": lambda c: self._jk(ctl, c[1], c[0], self.Dt, out)}
+        names = {"xc": lambda c: f"tile{c[0]}/split{c[1]}", "jk": lambda c: f"threads{c[0]}/split{c[1]}"}
         res = {"xc": {}, "jk": {}}
         for name in ("xc", "jk"):
             for c in cands[name]:
@@ -338,13 +345,14 @@ class BatchRKS:
                     fns[name](c)
                 b.record()
                 b.synchronize()
-                res[name][f"tile{c[0]}/split{c[1]}" if name == "xc" else f"split{c[1]}"] = a.elapsed_time(b) / repeats
-        best_xc = min(cands["xc"], key=lambda c: res["xc"][f"tile{c[0]}/split{c[1]}"])
-        best_jk = min(cands["jk"], key=lambda c: res["jk"][f"split{c[1]}"])
-        self.xc_tile = best_xc[0]
+                res[name][names[name](c)] = a.elapsed_time(b) / repeats
+        best_xc = min(cands["xc"], key=lambda c: res["xc"][names["xc"](c)])
+        best_jk = min(cands["jk"], key=lambda c: res["jk"][names["jk"](c)])
+        self.xc_tile, self.jk_threads = best_xc[0], best_jk[0]
         self.split = {"xc": best_xc[1], "jk": best_jk[1]}
         self.tuning = {"ms": res, "chosen_split": dict(self.split), "chosen_tile": self.xc_tile,
-                       "fixed": {"xc": "tile32/split1", "jk": "split1"}}
+                       "chosen_jk_threads": self.jk_threads,
+                       "fixed": {"xc": "tile32/split1", "jk": "threads256/split1"}}
         _TUNE_CACHE[key] = dict(self.tuning)
         return self.tuning
 

@@ -105,69 +105,124 @@ __global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
 }
 
 // ------------------------------------------------------------- packed s8 ----
+// Lane shifts of the segmented scans below are DPP moves (VALU), not LDS
+// permutes: row_shr:k within rows of 16 lanes, then row_bcast:15 / :31 across
+// rows.  Lanes outside ROW_MASK, or whose source lane is outside the row,
+// receive `old`.
+template <int CTRL, int ROW_MASK>
+__device__ inline int dpp_i(int v, int old) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_t(float v) {
+  return __int_as_float(dpp_i<CTRL, ROW_MASK>(__float_as_int(v), 0));
+}
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_t(double v) {
+  const int lo = dpp_i<CTRL, ROW_MASK>(__double2loint(v), 0);
+  const int hi = dpp_i<CTRL, ROW_MASK>(__double2hiint(v), 0);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int ROW_MASK, typename T>
+__device__ inline void seg_step(T& a, T& b, int key) {
+  const T sa = dpp_t<CTRL, ROW_MASK>(a);
+  const T sb = dpp_t<CTRL, ROW_MASK>(b);
+  const bool same = dpp_i<CTRL, ROW_MASK>(key, -2) == key;
+  a += same ? sa : T(0);
+  b += same ? sb : T(0);
+}
+
+// Inclusive prefix sums of a and b over runs of lanes with equal key (key
+// nondecreasing over the valid lanes; invalid lanes carry key -1 and zeros):
+// afterwards the last lane of each run holds the run's totals.
+template <typename T>
+__device__ inline void seg_scan2(T& a, T& b, int key) {
+  seg_step<0x111, 0xf>(a, b, key);  // row_shr:1
+  seg_step<0x112, 0xf>(a, b, key);  // row_shr:2
+  seg_step<0x114, 0xf>(a, b, key);  // row_shr:4
+  seg_step<0x118, 0xf>(a, b, key);  // row_shr:8
+  seg_step<0x142, 0xa>(a, b, key);  // row_bcast:15 into rows 1, 3
+  seg_step<0x143, 0xc>(a, b, key);  // row_bcast:31 into rows 2, 3
+}
+
+// (r, s) of packed lower-triangle position c = r (r + 1) / 2 + s, s <= r, for
+// c < 2^24 (one correction step each way covers the 1-ulp error of sqrtf)
+__device__ inline void tri_split(int c, int& r, int& s) {
+  int a = (int)((sqrtf(8.0f * (float)c + 1.0f) - 1.0f) * 0.5f);
+  if ((a + 1) * (a + 2) / 2 <= c) ++a;
+  if (a * (a + 1) / 2 > c) --a;
+  r = a;
+  s = c - a * (a + 1) / 2;
+}
+
 // Contributions of one row pq = (p, q) of the packed lower triangle (rs <= pq)
 // to M = J/2 - (hyb/4) K (the caller adds M + M^T), by one wavefront whose
 // lanes walk rs.  Targets (r, s), (p, s), (q, s) differ between lanes; the
 // targets (p, r) and (q, r) repeat over runs of consecutive rs with equal r, so
-// those are summed per run with a segmented suffix scan across the wave and
-// added once per run instead of by every lane.
+// those are summed per run with a segmented scan across the wave and added
+// once per run (by the run's last lane) instead of by every lane.
+//
+// The row is read in groups of S8_GROUP chunks of 64 elements.  All global
+// loads of a group are issued first, then all LDS reads of dm, then the LDS
+// atomics: dm and M share the LDS, so a read placed after an atomic would wait
+// for it, and the group's reads would become a chain of round trips.
 constexpr int S8_GROUP = 4;
 
 template <typename T>
-__device__ inline void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
-                              T* __restrict__ acc, const unsigned* __restrict__ pair, int N, T hhalf, int lane) {
+__device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
+                                       T* __restrict__ acc, int N, T hhalf, int lane) {
   const T dpq = dm[p * N + q];
+  const T fpq = p == q ? T(0.5) : T(1);
+  const T* dmp = dm + p * N;
+  const T* dmq = dm + q * N;
   T jrow = T(0);
-  // the row is read in groups of S8_GROUP chunks of 64 elements whose global
-  // loads are all issued before any is used, so their latency overlaps
   for (int gbase = 0; gbase <= pq; gbase += 64 * S8_GROUP) {
-    T vv[S8_GROUP];
+    T v[S8_GROUP], d_rs[S8_GROUP], d_qr[S8_GROUP], d_pr[S8_GROUP], d_qs[S8_GROUP], d_ps[S8_GROUP];
+    int r[S8_GROUP], s[S8_GROUP];
 #pragma unroll
     for (int u = 0; u < S8_GROUP; ++u) {
       const int rs = gbase + u * 64 + lane;
-      vv[u] = rs <= pq ? row[rs] : T(0);
+      v[u] = rs <= pq ? row[rs] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < S8_GROUP; ++u) {
+      // positions past the row end read the row's last element's operands
+      // (in bounds) and are masked below
+      const int rs = min(gbase + u * 64 + lane, pq);
+      tri_split(rs, r[u], s[u]);
+      d_rs[u] = dm[r[u] * N + s[u]];
+      d_qr[u] = dmq[r[u]];
+      d_pr[u] = dmp[r[u]];
+      d_qs[u] = dmq[s[u]];
+      d_ps[u] = dmp[s[u]];
     }
 #pragma unroll
     for (int u = 0; u < S8_GROUP; ++u) {
       const int base = gbase + u * 64;
-      if (base > pq) continue;  // uniform across the wavefront
+      if (base > pq) break;  // uniform across the wavefront
       const int rs = base + lane;
       const bool valid = rs <= pq;
-      int r = -1, s = 0;
-      T a = T(0), b = T(0);
+      // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
+      T w = v[u] * fpq;
+      if (r[u] == s[u]) w *= T(0.5);
+      if (pq == rs) w *= T(0.5);
+      const T w2 = w + w;
+      const T wk = -hhalf * w;
+      jrow += w2 * d_rs[u];
       if (valid) {
-        const unsigned rr = pair[rs];
-        r = rr & 0xffff;
-        s = rr >> 16;
-        // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
-        T v = vv[u];
-        if (p == q) v *= T(0.5);
-        if (r == s) v *= T(0.5);
-        if (pq == rs) v *= T(0.5);
-        const T v2 = v + v;
-        atomicAdd(acc + r * N + s, v2 * dpq);
-        jrow += v2 * dm[r * N + s];
-        const T vk = -hhalf * v;
-        atomicAdd(acc + p * N + s, vk * dm[q * N + r]);
-        atomicAdd(acc + q * N + s, vk * dm[p * N + r]);
-        a = vk * dm[q * N + s];
-        b = vk * dm[p * N + s];
+        atomicAdd(acc + r[u] * N + s[u], w2 * dpq);
+        atomicAdd(acc + p * N + s[u], wk * d_qr[u]);
+        atomicAdd(acc + q * N + s[u], wk * d_pr[u]);
       }
-      // segmented suffix sums over lanes of equal r (r is nondecreasing in rs)
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const T ta = __shfl_down(a, off, 64);
-        const T tb = __shfl_down(b, off, 64);
-        const int tr = __shfl_down(r, off, 64);
-        if (lane + off < 64 && tr == r) {
-          a += ta;
-          b += tb;
-        }
-      }
-      const int rprev = __shfl_up(r, 1, 64);
-      if (valid && (lane == 0 || rprev != r)) {
-        atomicAdd(acc + p * N + r, a);
-        atomicAdd(acc + q * N + r, b);
+      T a = wk * d_qs[u], b = wk * d_ps[u];
+      const int key = valid ? r[u] : -1;
+      seg_scan2(a, b, key);
+      // the run of this lane's r ends here: the next position starts a new r,
+      // or the chunk or the row ends
+      if (valid && (s[u] == r[u] || lane == 63 || rs == pq)) {
+        atomicAdd(acc + p * N + r[u], a);
+        atomicAdd(acc + q * N + r[u], b);
       }
     }
   }
@@ -186,24 +241,18 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
   const int P = N * (N + 1) / 2;
   T* dm = reinterpret_cast<T*>(smem);
   T* acc = dm + NN;
-  unsigned* pair = reinterpret_cast<unsigned*>(acc + NN);
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     dm[x] = dm_g[x];
     acc[x] = T(0);
-  }
-  for (int c = threadIdx.x; c < P; c += blockDim.x) {
-    int a = (int)((sqrtf(8.0f * (float)c + 1.0f) - 1.0f) * 0.5f);
-    while (tri(a + 1) <= c) ++a;
-    while (tri(a) > c) --a;
-    pair[c] = (unsigned)a | ((unsigned)(c - tri(a)) << 16);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   for (int pq = wave; pq < P; pq += nwaves) {
-    const unsigned pp = pair[pq];
-    s8_row<T>(eri + tri(pq), pq, pp & 0xffff, pp >> 16, dm, acc, pair, N, hhalf, lane);
+    int p, q;
+    tri_split(pq, p, q);
+    s8_row<T>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
   T* dst = part + (int64_t)blockIdx.x * NN;
@@ -216,44 +265,47 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
 // skipped.  dJK = M + M^T is written directly (one part), or added with
 // atomics (several parts: the caller zeroes the output first unless ACC), or,
 // with ACC, added to an FP64 accumulator (incremental build from density
-// changes: the FP32 rounding then scales with |dD| instead of |D|).
+// changes: the FP32 rounding then scales with |dD| instead of |D|).  LDS holds
+// dm and M of one molecule, 2 nao^2 elements, sized by the launch for the
+// largest molecule it covers; 256 or 512 threads share them.
+constexpr int JKB_MAX_THREADS = 512;
+
 template <typename T, typename TO, bool ACC>
-__global__ void __launch_bounds__(JK_THREADS) jk_s8_batch_kernel(const BatchMol* __restrict__ mols,
-                                                                 const int* __restrict__ order,
-                                                                 const double* __restrict__ ctl,
-                                                                 const T* __restrict__ eri_all,
-                                                                 const T* __restrict__ dm_all, T hhalf,
-                                                                 TO* __restrict__ out_all) {
+__global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const BatchMol* __restrict__ mols,
+                                                                      const int* __restrict__ order, int max_nao,
+                                                                      const double* __restrict__ ctl,
+                                                                      const T* __restrict__ eri_all,
+                                                                      const T* __restrict__ dm_all, T hhalf,
+                                                                      TO* __restrict__ out_all) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao, NN = N * N, P = N * (N + 1) / 2;
+  TO* out = out_all + Mo.mat0;
+  if (N > max_nao) {  // LDS was sized for max_nao: flag the block instead of overrunning it
+    if (blockIdx.y == 0)
+      for (int x = threadIdx.x; x < NN; x += blockDim.x) out[x] = TO(NAN);
+    return;
+  }
   const T* eri = eri_all + Mo.eri0;
   const T* dm_g = dm_all + Mo.mat0;
   T* dm = reinterpret_cast<T*>(smem);
   T* acc = dm + NN;
-  unsigned* pair = reinterpret_cast<unsigned*>(acc + NN);
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     dm[x] = dm_g[x];
     acc[x] = T(0);
-  }
-  for (int c = threadIdx.x; c < P; c += blockDim.x) {
-    int a = (int)((sqrtf(8.0f * (float)c + 1.0f) - 1.0f) * 0.5f);
-    while (tri(a + 1) <= c) ++a;
-    while (tri(a) > c) --a;
-    pair[c] = (unsigned)a | ((unsigned)(c - tri(a)) << 16);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int nwaves = blockDim.x >> 6;
   const int wave = (threadIdx.x >> 6) + blockIdx.y * nwaves;
   for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
-    const unsigned pp = pair[pq];
-    s8_row<T>(eri + tri(pq), pq, pp & 0xffff, pp >> 16, dm, acc, pair, N, hhalf, lane);
+    int p, q;
+    tri_split(pq, p, q);
+    s8_row<T>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
-  TO* out = out_all + Mo.mat0;
   const bool atomic = gridDim.y > 1;
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
@@ -265,11 +317,203 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_batch_kernel(const BatchMol*
   }
 }
 
-// Zeroes the dJK blocks of the active molecules (before a split build).
+// ------------------------------------------------------- batched, nao <= 64 ----
+// Column form of the same contraction.  LDS float atomics cost about as much
+// per lane as all the other work of an element together, so this form keeps
+// the per-element targets in registers: lane s walks column s of each row
+// pq = (p, q), r = 0..p in a compile-time unrolled loop, and
+//   J   (r, s) += w2 dm[p][q]      -> jacc[r] of lane s, summed over every row
+//                                     the wavefront handles, added to LDS once;
+//   K   (p, s) += wk dm[q][r]      -> register of lane s, one LDS add per row;
+//   K   (q, s) += wk dm[p][r]      -> register of lane s, one LDS add per row;
+//   K   (p, r) += sum_s wk dm[q][s] \ reductions over the wavefront, 16 at a
+//   K   (q, r) += sum_s wk dm[p][s] / time (8 values of r), one LDS add each;
+//   J   (p, q) += sum_rs w2 dm[r][s]  (one reduction per row),
+// with w the ERI divided by its multiplicity and wk = -(hyb/2) w, w2 = 2 w.
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_add(float x) {
+  return x + dpp_t<CTRL, ROW_MASK>(x);
+}
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_add(double x) {
+  return x + dpp_t<CTRL, ROW_MASK>(x);
+}
+
+// sum over the 64 lanes, valid in lane 63
+template <typename T>
+__device__ inline T wave_sum_last(T x) {
+  x = dpp_add<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]
+  x = dpp_add<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]
+  x = dpp_add<0x141, 0xf>(x);  // row_half_mirror
+  x = dpp_add<0x140, 0xf>(x);  // row_mirror
+  x = dpp_add<0x142, 0xa>(x);  // row_bcast:15 into rows 1, 3
+  x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+
+__device__ inline float lane_read(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+__device__ inline double lane_read(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
+// One halving level of a transpose-reduction: vectors x[i] and x[i + H] of
+// lane l and its DPP partner (which differs from l in lane bit `bit`) become
+// one vector, lanes with the bit clear keeping the sums of x[i], lanes with it
+// set the sums of x[i + H].
+template <int CTRL, int H, typename T>
+__device__ inline void tr_level(T (&x)[16], bool bit) {
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const T keep = bit ? x[i + H] : x[i];
+    const T send = bit ? x[i] : x[i + H];
+    x[i] = keep + dpp_t<CTRL, 0xf>(send);
+  }
+}
+__device__ inline float xor16_sum(float v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ inline float xor32_sum(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ inline double xor16_sum(double v) {
+  const auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+__device__ inline double xor32_sum(double v) {
+  const auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+
+// Sums each of 16 vectors over the 64 lanes at once; lane l returns the total
+// of vector x[l & 15].  Pairs i <-> 15-i (row_mirror), i <-> 7-i
+// (row_half_mirror), i <-> i^2 and i <-> i^1 (quad_perm) halve the vector
+// count while keeping the partners' vector sets equal, then the four rows of
+// 16 lanes are added with the permlane swaps: 49 VALU operations for 16
+// reductions instead of about 160 for 16 separate ones.
+template <typename T>
+__device__ inline T transpose_reduce16(T (&x)[16], int lane) {
+  tr_level<0x140, 8>(x, lane & 8);   // row_mirror
+  tr_level<0x141, 4>(x, lane & 4);   // row_half_mirror
+  tr_level<0x4e, 2>(x, lane & 2);    // quad_perm [2,3,0,1]
+  tr_level<0xb1, 1>(x, lane & 1);    // quad_perm [1,0,3,2]
+  return xor32_sum(xor16_sum(x[0]));
+}
+
+constexpr int JKC_MAXN = 64;
+constexpr int JKC_GROUP = 8;  // r values whose ERI loads are issued together
+
+template <typename T, typename TO, bool ACC>
+__global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const BatchMol* __restrict__ mols,
+                                                                           const int* __restrict__ order, int max_nao,
+                                                                           const double* __restrict__ ctl,
+                                                                           const T* __restrict__ eri_all,
+                                                                           const T* __restrict__ dm_all, T hhalf,
+                                                                           TO* __restrict__ out_all) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = order[blockIdx.x];
+  if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
+  const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao, NN = N * N, P = N * (N + 1) / 2;
+  TO* out = out_all + Mo.mat0;
+  if (N > max_nao || N > JKC_MAXN) {  // LDS and lanes sized for max_nao <= 64: flag the block
+    if (blockIdx.y == 0)
+      for (int x = threadIdx.x; x < NN; x += blockDim.x) out[x] = TO(NAN);
+    return;
+  }
+  const T* eri = eri_all + Mo.eri0;
+  const T* dm_g = dm_all + Mo.mat0;
+  T* dm = reinterpret_cast<T*>(smem);
+  T* acc = dm + NN;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+    dm[x] = dm_g[x];
+    acc[x] = T(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int nwaves = blockDim.x >> 6;
+  const int wave = (threadIdx.x >> 6) + blockIdx.y * nwaves;
+  const int sl = lane < N ? lane : N - 1;  // in-bounds column for lanes past N (their values are masked)
+  T jacc[JKC_MAXN];
+#pragma unroll
+  for (int r = 0; r < JKC_MAXN; ++r) jacc[r] = T(0);
+  for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
+    int p, q;
+    tri_split(pq, p, q);
+    const T* row = eri + tri(pq);
+    const T dpq = dm[p * N + q];
+    const T dp_s = lane < N ? dm[p * N + sl] : T(0);
+    const T dq_s = lane < N ? dm[q * N + sl] : T(0);
+    const T hp = p == q ? T(0.5) : T(1);
+    T kp = T(0), kq = T(0), jrow = T(0);
+#pragma unroll
+    for (int g = 0; g < JKC_MAXN; g += JKC_GROUP) {
+      if (g > p) break;  // uniform across the wavefront
+      T v[JKC_GROUP];
+#pragma unroll
+      for (int u = 0; u < JKC_GROUP; ++u) {
+        const int r = g + u;
+        // row pq holds (r, s) for r < p, s <= r, and for r = p, s <= q
+        const bool valid = r <= p && lane <= (r == p ? q : r);
+        v[u] = valid ? row[r * (r + 1) / 2 + lane] : T(0);
+      }
+      // t[u] = wk dm[q][s] (to K(p, g+u)), t[8+u] = wk dm[p][s] (to K(q, g+u))
+      T t[16];
+#pragma unroll
+      for (int u = 0; u < JKC_GROUP; ++u) {
+        const int r = g + u;
+        t[u] = t[JKC_GROUP + u] = T(0);
+        if (r > p) continue;  // uniform
+        // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
+        T w = v[u] * hp;
+        if (lane == r) w *= T(0.5);
+        if (r == p && lane == q) w *= T(0.5);
+        const T w2 = w + w, wk = -hhalf * w;
+        jacc[r] += w2 * dpq;
+        jrow += w2 * dm[r * N + (lane <= r ? lane : r)];
+        kp += wk * lane_read(dq_s, r);
+        kq += wk * lane_read(dp_s, r);
+        t[u] = wk * dq_s;
+        t[JKC_GROUP + u] = wk * dp_s;
+      }
+      const T tot = transpose_reduce16(t, lane);
+      const int u = lane & 7;
+      if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, tot);
+    }
+    jrow = wave_sum_last(jrow);
+    if (lane < N) {
+      atomicAdd(acc + p * N + lane, kp);
+      atomicAdd(acc + q * N + lane, kq);
+    }
+    if (lane == 63) atomicAdd(acc + p * N + q, jrow);
+  }
+  // (predicated rather than cut short, so the loop stays unrolled and jacc in registers)
+#pragma unroll
+  for (int r = 0; r < JKC_MAXN; ++r)
+    if (r < N && lane <= r) atomicAdd(acc + r * N + lane, jacc[r]);
+  __syncthreads();
+  const bool atomic = gridDim.y > 1;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    const TO val = (TO)(acc[x] + acc[j * N + i]);
+    if (atomic)
+      atomicAdd(out + x, val);
+    else
+      out[x] = ACC ? out[x] + val : val;
+  }
+}
+
+// Zeroes the dJK blocks of the active molecules order[0..gridDim.x) (before a
+// split build).
 template <typename TO>
 __global__ void __launch_bounds__(256) zero_active_kernel(const BatchMol* __restrict__ mols,
+                                                          const int* __restrict__ order,
                                                           const double* __restrict__ ctl, TO* __restrict__ out) {
-  const int m = blockIdx.x;
+  const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
   const int NN = (int)(Mo.nao * Mo.nao);
@@ -310,7 +554,7 @@ static int jk_s8_blocks(int N) {
 
 template <typename T>
 static size_t jk_s8_lds(int N) {
-  return sizeof(T) * 2 * (size_t)N * N + sizeof(unsigned) * (size_t)N * (N + 1) / 2;
+  return sizeof(T) * 2 * (size_t)N * N;
 }
 
 template <typename T>
@@ -369,51 +613,48 @@ extern "C" int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_
   return jk_s8_impl<float>((const float*)d_eri_s8, N, (const float*)d_dm, hyb, (float*)d_out, d_ws, ws_bytes, st);
 }
 
+template <typename T, typename TO, bool ACC>
+static int batch_jk_launch(dim3 grid, int threads, size_t lds, hipStream_t st, const void* mols, const int* order,
+                           int max_nao, const double* ctl, const void* eri, const void* dm, double hyb, void* out) {
+  if (max_nao <= JKC_MAXN) {
+    PSCF_SET_LDS((jk_s8_cols_batch_kernel<T, TO, ACC>), lds);
+    jk_s8_cols_batch_kernel<T, TO, ACC><<<grid, threads, lds, st>>>(
+        (const BatchMol*)mols, order, max_nao, ctl, (const T*)eri, (const T*)dm, (T)(hyb / 2.0), (TO*)out);
+  } else {
+    PSCF_SET_LDS((jk_s8_batch_kernel<T, TO, ACC>), lds);
+    jk_s8_batch_kernel<T, TO, ACC><<<grid, threads, lds, st>>>((const BatchMol*)mols, order, max_nao, ctl,
+                                                               (const T*)eri, (const T*)dm, (T)(hyb / 2.0), (TO*)out);
+  }
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
 extern "C" int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
                                 const double* d_ctl, const void* d_eri, const void* d_dm, double hyb, void* d_out,
-                                int accumulate, int split, void* stream) {
+                                int accumulate, int split, int threads, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_jk_s8: bad dtype");
   PSCF_REQUIRE(nmol > 0 && max_nao > 0 && max_nao < 65536 && d_mols && d_order && d_eri && d_dm && d_out,
                "pscf_batch_jk_s8: bad arguments");
   PSCF_REQUIRE(split >= 1 && split <= 64, "pscf_batch_jk_s8: bad split %d", split);
+  PSCF_REQUIRE(threads == 256 || threads == JKB_MAX_THREADS, "pscf_batch_jk_s8: threads %d not 256 or 512", threads);
+  const size_t lds = dtype == PSCF_F64 ? jk_s8_lds<double>(max_nao) : jk_s8_lds<float>(max_nao);
+  PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(nmol, split);
   if (split > 1 && !accumulate) {
     if (dtype == PSCF_F64)
-      zero_active_kernel<double><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_ctl, (double*)d_out);
+      zero_active_kernel<double><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, d_ctl, (double*)d_out);
     else
-      zero_active_kernel<float><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_ctl, (float*)d_out);
+      zero_active_kernel<float><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, d_ctl, (float*)d_out);
     PSCF_HIP(hipGetLastError());
   }
-  if (dtype == PSCF_F64) {
-    const size_t lds = jk_s8_lds<double>(max_nao);
-    PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
-    if (accumulate) {
-      PSCF_SET_LDS((jk_s8_batch_kernel<double, double, true>), lds);
-      jk_s8_batch_kernel<double, double, true><<<grid, JK_THREADS, lds, st>>>(
-          (const BatchMol*)d_mols, d_order, d_ctl, (const double*)d_eri, (const double*)d_dm, hyb / 2.0,
-          (double*)d_out);
-    } else {
-      PSCF_SET_LDS((jk_s8_batch_kernel<double, double, false>), lds);
-      jk_s8_batch_kernel<double, double, false><<<grid, JK_THREADS, lds, st>>>(
-          (const BatchMol*)d_mols, d_order, d_ctl, (const double*)d_eri, (const double*)d_dm, hyb / 2.0,
-          (double*)d_out);
-    }
-  } else {
-    const size_t lds = jk_s8_lds<float>(max_nao);
-    PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
-    if (accumulate) {
-      PSCF_SET_LDS((jk_s8_batch_kernel<float, double, true>), lds);
-      jk_s8_batch_kernel<float, double, true><<<grid, JK_THREADS, lds, st>>>(
-          (const BatchMol*)d_mols, d_order, d_ctl, (const float*)d_eri, (const float*)d_dm, (float)(hyb / 2.0),
-          (double*)d_out);
-    } else {
-      PSCF_SET_LDS((jk_s8_batch_kernel<float, float, false>), lds);
-      jk_s8_batch_kernel<float, float, false><<<grid, JK_THREADS, lds, st>>>(
-          (const BatchMol*)d_mols, d_order, d_ctl, (const float*)d_eri, (const float*)d_dm, (float)(hyb / 2.0),
-          (float*)d_out);
-    }
-  }
-  PSCF_HIP(hipGetLastError());
-  return PSCF_OK;
+  if (dtype == PSCF_F64)
+    return accumulate ? batch_jk_launch<double, double, true>(grid, threads, lds, st, d_mols, d_order, max_nao, d_ctl,
+                                                              d_eri, d_dm, hyb, d_out)
+                      : batch_jk_launch<double, double, false>(grid, threads, lds, st, d_mols, d_order, max_nao,
+                                                               d_ctl, d_eri, d_dm, hyb, d_out);
+  return accumulate ? batch_jk_launch<float, double, true>(grid, threads, lds, st, d_mols, d_order, max_nao, d_ctl,
+                                                           d_eri, d_dm, hyb, d_out)
+                    : batch_jk_launch<float, float, false>(grid, threads, lds, st, d_mols, d_order, max_nao, d_ctl,
+                                                           d_eri, d_dm, hyb, d_out);
 }

@@ -76,8 +76,10 @@ def test_batch_split_and_tuner_fp64(hip):
     b.setup()
     tun = b.tune()
     assert set(tun["chosen_split"]) == {"xc", "jk"} and all(v in batch.SPLITS for v in tun["chosen_split"].values())
+    assert tun["chosen_jk_threads"] in batch.JK_THREADS
     assert b.tune().get("cached")
     b.split = {"xc": 4, "jk": 8}
+    b.jk_threads = 512
     got = b.run(max_its=30, conv_tol=1e-9, graph=True)
     assert np.abs(got.energy - ref.energy).max() < 1e-9
     assert np.array_equal(got.iterations, ref.iterations)
@@ -91,5 +93,6 @@ def test_batch_fp32_tiles_agree(hip):
         b = batch.BatchRKS(mols, dtype="fp32")
         b.setup()
         b.xc_tile, b.split = tile, {"xc": split, "jk": split}
+        b.jk_threads = 256 * split
         runs.append(b.run(max_its=30, conv_tol=1e-6, graph=False))
     assert np.abs(runs[0].energy - runs[1].energy).max() < 1e-4  # FP32 summation-order noise at a 1e-6 stop
