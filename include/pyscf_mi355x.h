@@ -204,7 +204,13 @@ int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, i
                   const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
                   int split, int tile, void* d_part_v, double* d_part_e, int64_t mat_total, int m_total,
                   void* stream);
+/* Divides every molecule's packed s8 ERIs (pscf_batch_eri_s8) in place by
+ * their multiplicity 2^([p==q] + [r==s] + [pq==rs]), as the reference does
+ * before its contraction (sparse_symmetric_ERI.py:68-79, 180-182); max_nao
+ * bounds the molecules' nao. */
+int pscf_batch_eri_scale(int dtype, const void* d_mols, int nmol, int max_nao, void* d_eri, void* stream);
 /* dJK of the active molecules d_order[0..nmol) from their packed s8 ERIs
+ * divided by their multiplicity (pscf_batch_eri_scale)
  * (sparse_symmetric_ERI.py:86-137); max_nao bounds their nao (it sizes the
  * LDS; a larger molecule gets NaN).  accumulate = 0: d_out (dtype) =
  * dJK(d_dm); accumulate = 1: d_out (FP64) += dJK(d_dm), the incremental build

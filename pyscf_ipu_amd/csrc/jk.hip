@@ -169,11 +169,11 @@ __device__ inline void tri_split(int c, int& r, int& s) {
 // for it, and the group's reads would become a chain of round trips.
 constexpr int S8_GROUP = 4;
 
-template <typename T>
+template <typename T, bool PRESCALED>
 __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
                                        T* __restrict__ acc, int N, T hhalf, int lane) {
   const T dpq = dm[p * N + q];
-  const T fpq = p == q ? T(0.5) : T(1);
+  const T fpq = (p == q && !PRESCALED) ? T(0.5) : T(1);
   const T* dmp = dm + p * N;
   const T* dmq = dm + q * N;
   T jrow = T(0);
@@ -203,10 +203,10 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
       if (base > pq) break;  // uniform across the wavefront
       const int rs = base + lane;
       const bool valid = rs <= pq;
-      // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
+      // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs]), unless the ERIs carry it
       T w = v[u] * fpq;
-      if (r[u] == s[u]) w *= T(0.5);
-      if (pq == rs) w *= T(0.5);
+      if (!PRESCALED && r[u] == s[u]) w *= T(0.5);
+      if (!PRESCALED && pq == rs) w *= T(0.5);
       const T w2 = w + w;
       const T wk = -hhalf * w;
       jrow += w2 * d_rs[u];
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
   for (int pq = wave; pq < P; pq += nwaves) {
     int p, q;
     tri_split(pq, p, q);
-    s8_row<T>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
+    s8_row<T, false>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
   T* dst = part + (int64_t)blockIdx.x * NN;
@@ -265,7 +265,8 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
 // skipped.  dJK = M + M^T is written directly (one part), or added with
 // atomics (several parts: the caller zeroes the output first unless ACC), or,
 // with ACC, added to an FP64 accumulator (incremental build from density
-// changes: the FP32 rounding then scales with |dD| instead of |D|).  LDS holds
+// changes: the FP32 rounding then scales with |dD| instead of |D|).  The ERIs
+// come divided by their multiplicity (pscf_batch_eri_scale).  LDS holds
 // dm and M of one molecule, 2 nao^2 elements, sized by the launch for the
 // largest molecule it covers; 256 or 512 threads share them.
 constexpr int JKB_MAX_THREADS = 512;
@@ -303,7 +304,7 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const Batc
   for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
     int p, q;
     tri_split(pq, p, q);
-    s8_row<T>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
+    s8_row<T, true>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
   const bool atomic = gridDim.y > 1;
@@ -329,7 +330,8 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const Batc
 //   K   (p, r) += sum_s wk dm[q][s] \ reductions over the wavefront, 16 at a
 //   K   (q, r) += sum_s wk dm[p][s] / time (8 values of r), one LDS add each;
 //   J   (p, q) += sum_rs w2 dm[r][s]  (one reduction per row),
-// with w the ERI divided by its multiplicity and wk = -(hyb/2) w, w2 = 2 w.
+// with w the ERI divided by its multiplicity (pscf_batch_eri_scale) and
+// wk = -(hyb/2) w, w2 = 2 w; the constant factors are applied per row.
 template <int CTRL, int ROW_MASK>
 __device__ inline float dpp_add(float x) {
   return x + dpp_t<CTRL, ROW_MASK>(x);
@@ -436,7 +438,9 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int nwaves = blockDim.x >> 6;
-  const int wave = (threadIdx.x >> 6) + blockIdx.y * nwaves;
+  // wave-uniform row index: p, q, the row pointer and dm[p][r], dm[q][r]
+  // then live in scalar registers and come from scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) + blockIdx.y * nwaves);
   const int sl = lane < N ? lane : N - 1;  // in-bounds column for lanes past N (their values are masked)
   T jacc[JKC_MAXN];
 #pragma unroll
@@ -444,50 +448,63 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
   for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
     int p, q;
     tri_split(pq, p, q);
+    p = __builtin_amdgcn_readfirstlane(p);
+    q = __builtin_amdgcn_readfirstlane(q);
     const T* row = eri + tri(pq);
-    const T dpq = dm[p * N + q];
+    const T* gp = dm_g + p * N;
+    const T* gq = dm_g + q * N;
+    const T cj = T(2) * gp[q];
     const T dp_s = lane < N ? dm[p * N + sl] : T(0);
     const T dq_s = lane < N ? dm[q * N + sl] : T(0);
-    const T hp = p == q ? T(0.5) : T(1);
     T kp = T(0), kq = T(0), jrow = T(0);
+    // row pq holds (r, s) for r < p, s <= r, and for r = p, s <= q.  In FP32
+    // the loads of group g + 8 are issued before group g is used (software
+    // pipeline); FP64 has no registers to spare for it.
+    constexpr bool PIPE = sizeof(T) == 4;
+    auto load_group = [&](int g0, T (&dst)[JKC_GROUP]) {
+#pragma unroll
+      for (int u = 0; u < JKC_GROUP; ++u) {
+        const int r = g0 + u;
+        const bool valid = r <= p && lane <= (r == p ? q : r);
+        dst[u] = valid ? row[r * (r + 1) / 2 + lane] : T(0);
+      }
+    };
+    T vn[JKC_GROUP];
+    if constexpr (PIPE) load_group(0, vn);
 #pragma unroll
     for (int g = 0; g < JKC_MAXN; g += JKC_GROUP) {
       if (g > p) break;  // uniform across the wavefront
       T v[JKC_GROUP];
+      if constexpr (PIPE) {
 #pragma unroll
-      for (int u = 0; u < JKC_GROUP; ++u) {
-        const int r = g + u;
-        // row pq holds (r, s) for r < p, s <= r, and for r = p, s <= q
-        const bool valid = r <= p && lane <= (r == p ? q : r);
-        v[u] = valid ? row[r * (r + 1) / 2 + lane] : T(0);
+        for (int u = 0; u < JKC_GROUP; ++u) v[u] = vn[u];
+        if (g + JKC_GROUP <= p && g + JKC_GROUP < JKC_MAXN) load_group(g + JKC_GROUP, vn);
+      } else {
+        load_group(g, v);
       }
-      // t[u] = wk dm[q][s] (to K(p, g+u)), t[8+u] = wk dm[p][s] (to K(q, g+u))
+      // t[u] = w dm[q][s] (to K(p, g+u)), t[8+u] = w dm[p][s] (to K(q, g+u))
       T t[16];
 #pragma unroll
       for (int u = 0; u < JKC_GROUP; ++u) {
         const int r = g + u;
         t[u] = t[JKC_GROUP + u] = T(0);
         if (r > p) continue;  // uniform
-        // 1 / multiplicity: 2^-([p==q] + [r==s] + [pq==rs])
-        T w = v[u] * hp;
-        if (lane == r) w *= T(0.5);
-        if (r == p && lane == q) w *= T(0.5);
-        const T w2 = w + w, wk = -hhalf * w;
-        jacc[r] += w2 * dpq;
-        jrow += w2 * dm[r * N + (lane <= r ? lane : r)];
-        kp += wk * lane_read(dq_s, r);
-        kq += wk * lane_read(dp_s, r);
-        t[u] = wk * dq_s;
-        t[JKC_GROUP + u] = wk * dp_s;
+        const T w = v[u];
+        jacc[r] += w * cj;
+        jrow += w * dm[r * N + (lane <= r ? lane : r)];
+        kp += w * gq[r];
+        kq += w * gp[r];
+        t[u] = w * dq_s;
+        t[JKC_GROUP + u] = w * dp_s;
       }
-      const T tot = transpose_reduce16(t, lane);
+      const T tot = transpose_reduce16(t, lane) * -hhalf;
       const int u = lane & 7;
       if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, tot);
     }
-    jrow = wave_sum_last(jrow);
+    jrow = wave_sum_last(jrow) * T(2);
     if (lane < N) {
-      atomicAdd(acc + p * N + lane, kp);
-      atomicAdd(acc + q * N + lane, kq);
+      atomicAdd(acc + p * N + lane, kp * -hhalf);
+      atomicAdd(acc + q * N + lane, kq * -hhalf);
     }
     if (lane == 63) atomicAdd(acc + p * N + q, jrow);
   }
@@ -504,6 +521,29 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
       atomicAdd(out + x, val);
     else
       out[x] = ACC ? out[x] + val : val;
+  }
+}
+
+// Divides each molecule's packed s8 ERIs by their multiplicity
+// 2^([p==q] + [r==s] + [pq==rs]) in place, the form the reference hands to its
+// sparse contraction (sparse_symmetric_ERI.py:68-79, 180-182).  Workgroup
+// (blockIdx.x, blockIdx.y) strides over molecule blockIdx.x's elements.
+template <typename T>
+__global__ void __launch_bounds__(256) eri_scale_kernel(const BatchMol* __restrict__ mols, T* __restrict__ eri_all) {
+  const BatchMol Mo = mols[blockIdx.x];
+  const int N = (int)Mo.nao;
+  const int64_t P = (int64_t)N * (N + 1) / 2, n = P * (P + 1) / 2;
+  T* eri = eri_all + Mo.eri0;
+  for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.y * blockDim.x) {
+    int64_t a = (int64_t)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+    while (tri(a + 1) <= e) ++a;
+    while (tri(a) > e) --a;
+    const int pq = (int)a, rs = (int)(e - tri(a));
+    int p, q, r, s;
+    tri_split(pq, p, q);
+    tri_split(rs, r, s);
+    const int k = (p == q) + (r == s) + (pq == rs);
+    eri[e] *= T(1.0 / (double)(1 << k));
   }
 }
 
@@ -657,4 +697,19 @@ extern "C" int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_orde
                                                            d_eri, d_dm, hyb, d_out)
                     : batch_jk_launch<float, float, false>(grid, threads, lds, st, d_mols, d_order, max_nao, d_ctl,
                                                            d_eri, d_dm, hyb, d_out);
+}
+
+extern "C" int pscf_batch_eri_scale(int dtype, const void* d_mols, int nmol, int max_nao, void* d_eri, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_eri_scale: bad dtype");
+  PSCF_REQUIRE(nmol > 0 && max_nao > 0 && max_nao <= 4096 && d_mols && d_eri, "pscf_batch_eri_scale: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t P = (int64_t)max_nao * (max_nao + 1) / 2;
+  const int64_t per = (P * (P + 1) / 2 + 255) / 256;
+  const dim3 grid(nmol, (unsigned)(per < 64 ? (per < 1 ? 1 : per) : 64));
+  if (dtype == PSCF_F64)
+    eri_scale_kernel<double><<<grid, 256, 0, st>>>((const BatchMol*)d_mols, (double*)d_eri);
+  else
+    eri_scale_kernel<float><<<grid, 256, 0, st>>>((const BatchMol*)d_mols, (float*)d_eri);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
 }

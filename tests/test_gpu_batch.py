@@ -96,3 +96,57 @@ def test_batch_fp32_tiles_agree(hip):
         b.jk_threads = 256 * split
         runs.append(b.run(max_its=30, conv_tol=1e-6, graph=False))
     assert np.abs(runs[0].energy - runs[1].energy).max() < 1e-4  # FP32 summation-order noise at a 1e-6 stop
+
+
+def _multiplicity(N):
+    """2^([p==q] + [r==s] + [pq==rs]) for every element of the s8 packing,
+    from numpy index arithmetic (sparse_symmetric_ERI.py:68-79)."""
+    p, q = np.tril_indices(N)
+    pq_diag = (p == q).astype(int)
+    P = len(p)
+    a, b = np.tril_indices(P)              # element (pq=a, rs=b), s8 order
+    return 2.0 ** (pq_diag[a] + pq_diag[b] + (a == b))
+
+
+def test_batch_eri_scale_matches_oracle(hip):
+    """The batch ERI buffer holds the oracle's s8 ERIs over their multiplicity."""
+    from oracle import cint
+    b = batch.BatchRKS(batch.parse(STRS[:4]), dtype="fp64")
+    b.setup()
+    eri = b.eri.cpu().numpy()
+    for m, s in enumerate(STRS[:4]):
+        mol = build_mol(s)
+        N = mol.nao_nr()
+        ref = cint.eri_s8(mol) / _multiplicity(N)
+        o = int(b.desc[m, 11])
+        got = eri[o:o + ref.size]
+        np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-13)
+
+
+@pytest.mark.parametrize("dtype", ["fp64", "fp32"])
+def test_batch_jk_row_and_column_forms(hip, dtype):
+    """The row form (used above nao 64) and the column form give the dense
+    J - (hyb/2) K of each molecule's density."""
+    import torch
+    from oracle import cint
+    from pyscf_ipu_amd import _lib, device
+    b = batch.BatchRKS(batch.parse(STRS[:4]), dtype=dtype)
+    b.setup()
+    P, st = device.ptr, device.stream()
+    outs = []
+    for cmax in (b.max_nao, 65):  # 65 sizes the launch past the column form's 64
+        out = torch.zeros_like(b.dJK if dtype == "fp64" else b.Vxc)
+        _lib.call("pscf_batch_jk_s8", b.code, P(b.d_desc), P(b.d_order), b.M, cmax, None, P(b.eri), P(b.Dt), 0.2,
+                  P(out), 0, 1, 256, st)
+        outs.append(out.double())
+    torch.cuda.synchronize()
+    tol = 1e-10 if dtype == "fp64" else 2e-4
+    for m, s in enumerate(STRS[:4]):
+        mol = build_mol(s)
+        N = mol.nao_nr()
+        D = b.matrix(b.D, m)
+        ref = rks.dense_jk(cint.s8_to_s1(cint.eri_s8(mol), N), D)
+        o = int(b.desc[m, 12])
+        for out in outs:
+            got = out[o:o + N * N].cpu().numpy().reshape(N, N)
+            assert np.abs(got - ref).max() < tol * max(1.0, np.abs(ref).max()), (m, np.abs(got - ref).max())
