@@ -195,14 +195,13 @@ int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, const doubl
                   const int* d_atm, const int* d_bas, const double* d_env, void* d_ao, void* stream);
 /* E_xc (d_exc[m]) and V_xc of the active molecules d_order[0..nmol)
  * (nanoDFT.py:105-123), grid tiles of `tile` points (32, or 64 in FP32);
- * max_nao bounds the nao of these molecules (it sizes the LDS).  split
- * workgroups share each molecule's tiles; split > 1 needs part buffers of
- * split * mat_total elements of dtype and split * m_total doubles
- * (mat_total = length of the concatenated N x N matrices, m_total = number
- * of molecules in the descriptors). */
+ * max_nao bounds the nao of these molecules (it sizes the LDS and the tile
+ * shapes; a larger molecule gets NaN).  split workgroups share each molecule's
+ * tiles; split > 1 needs part buffers of part_v_elems >= split * nmol *
+ * max_nao^2 elements of dtype and part_e_elems >= split * nmol doubles. */
 int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                   const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
-                  int split, int tile, void* d_part_v, double* d_part_e, int64_t mat_total, int m_total,
+                  int split, int tile, void* d_part_v, double* d_part_e, int64_t part_v_elems, int64_t part_e_elems,
                   void* stream);
 /* Divides every molecule's packed s8 ERIs (pscf_batch_eri_s8) in place by
  * their multiplicity 2^([p==q] + [r==s] + [pq==rs]), as the reference does

@@ -31,6 +31,12 @@ DIIS_N = 9
 CTL = 8
 SPLITS = (1, 2, 4, 8)   # workgroups per molecule tried by the autotuner
 MAX_SPLIT = max(SPLITS)
+MIN_XC_WORKGROUPS = 512  # a small class of molecules is split further, to fill the 256 CUs
+
+
+def class_split(split: int, count: int) -> int:
+    """Workgroups per molecule of an XC launch over `count` molecules."""
+    return min(64, max(split, -(-MIN_XC_WORKGROUPS // count)))
 JK_THREADS = (256, 512)   # J/K workgroup sizes tried by the autotuner
 XC_TILES = {"fp32": (32, 64), "fp64": (32,)}   # grid points per XC tile
 _TUNE_CACHE = {}        # tuning results per batch shape
@@ -281,9 +287,12 @@ class BatchRKS:
         self.incremental = T != f64
         self.dJK, self.Vxc = device.zeros(nm, f64), device.empty(nm, T)
         self.dDt = device.empty(nm, T) if self.incremental else None
-        # per-part XC buffers for the largest split the tuner may choose
-        self.xc_parts = device.empty(MAX_SPLIT * nm, T)
-        self.xc_parts_e = device.empty(MAX_SPLIT * self.M, f64)
+        # per-part XC buffers ([part][molecule of the class][max nao^2]) for the
+        # largest split any class may use
+        self.xc_parts_n = max(class_split(MAX_SPLIT, c) * c * n * n for _, c, n in self.nao_classes)
+        self.xc_parts_e_n = max(class_split(MAX_SPLIT, c) * c for _, c, _ in self.nao_classes)
+        self.xc_parts = device.empty(self.xc_parts_n, T)
+        self.xc_parts_e = device.empty(self.xc_parts_e_n, f64)
         self.exc = device.empty(self.M, f64)
         self.H, self.C = device.empty(nm, f64), device.empty(nm, f64)
         self.mo_e = device.empty(S["vec"], f64)
@@ -299,7 +308,8 @@ class BatchRKS:
             order = ctypes.c_void_p(self.d_order.data_ptr() + 4 * start)
             _lib.call("pscf_batch_xc", self.code, P(self.d_desc), order, count, cmax, ctl_ptr, P(self.ao), P(self.Dt),
                       P(self.w), 1e-20, P(self.Vxc if vxc is None else vxc), P(self.exc if exc is None else exc),
-                      split, tile, P(self.xc_parts), P(self.xc_parts_e), self.sizes["mat"], self.M, st)
+                      class_split(split, count), tile, P(self.xc_parts), P(self.xc_parts_e), self.xc_parts_n,
+                      self.xc_parts_e_n, st)
 
     def _jk(self, ctl_ptr, split, threads, dm, out=None):
         P, st = device.ptr, device.stream()

@@ -15,6 +15,7 @@
 #include "b3lyp.h"
 #include "batch.h"
 #include "common.h"
+#include "wave.h"
 
 namespace pscf {
 
@@ -170,8 +171,9 @@ struct XMTraits<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   static constexpr int TILE = 64;
   static __device__ inline v4 mfma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-  // row of accumulator register j in lane group lg = lane / 16
+  // row of accumulator register j in lane group lg = lane / 16: row(lg, 0) + j * RSTEP
   static __device__ inline int row(int lg, int j) { return 4 * lg + j; }
+  static constexpr int RSTEP = 1;
 };
 template <>
 struct XMTraits<double> {
@@ -181,33 +183,44 @@ struct XMTraits<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ inline int row(int lg, int j) { return lg + 4 * j; }
+  static constexpr int RSTEP = 4;
 };
-constexpr int XM_MAXB = 7;   // V blocks per wave: (NP / 16)^2 <= 28
 constexpr int XM_MAXN = 80;  // nao limit of the MFMA path
 
 __host__ __device__ inline int xm_np(int N) { return (N + 15) & ~15; }
-__host__ __device__ inline int xm_xs(int N) { return xm_np(N) + 4; }
-__host__ __device__ inline int xm_xd(int N) { return xm_np(N) <= 16 ? 16 : 80; }
-template <typename T, int XT = XMTraits<T>::TILE>
-__host__ __device__ inline size_t xm_lds_elems(int N) {
-  const int NP = xm_np(N);
-  return (size_t)4 * XT * xm_xs(N) + (size_t)XT * xm_xd(N) + (size_t)NP * xm_xd(N) + 256 + 4 * XT;
-}
 
-template <typename T, int XT = XMTraits<T>::TILE>
-__device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
-                                     typename XMTraits<T>::v4 (&vacc)[XM_MAXB], double& e_loc) {
+// Shapes of the tile pipeline for XT points and nao padded to NP (a multiple
+// of 16 up to 80), all compile-time so that every LDS offset in the GEMM and
+// epilogue loops is an immediate.
+template <int XT, int NP>
+struct XTile {
+  static constexpr int PB = XT / 16, NSLOT = 4 / PB;      // GEMM-1: point blocks, column-block slots
+  static constexpr int NB = NP / 16, NJ = (NB + NSLOT - 1) / NSLOT;
+  static constexpr int XS = NP + 4;                        // Phi row stride, 4 mod 16
+  static constexpr int XD = NP <= 16 ? 16 : 80;            // D and Y row stride, 16 mod 64
+  static constexpr int NVB = NB * NB, NBW = (NVB + 3) / 4; // V blocks, per wave
+  static constexpr size_t lds_elems = (size_t)4 * XT * XS + (size_t)XT * XD + (size_t)NP * XD + 256 + 4 * XT;
+};
+
+template <typename T, int XT, int NP>
+using XVacc = typename XMTraits<T>::v4[XTile<XT, NP>::NBW];
+
+template <typename T, int XT, int NP>
+__device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
+                                XVacc<T, XT, NP>& vacc, double& e_loc) {
   typedef typename XMTraits<T>::v4 v4;
-  constexpr int PB = XT / 16, NSLOT = 4 / PB;
-  const int N = a.N, NP = xm_np(N), NB = NP / 16, XS = xm_xs(N), XD = xm_xd(N);
+  typedef XTile<XT, NP> L;
+  constexpr int PB = L::PB, NSLOT = L::NSLOT, NB = L::NB, XS = L::XS, XD = L::XD, NVB = L::NVB, NBW = L::NBW;
+  constexpr int RS = XMTraits<T>::RSTEP;
+  const int N = a.N;
   T* Phi = reinterpret_cast<T*>(smem);  // [4][XT][XS]
   T* Y = Phi + 4 * XT * XS;             // [XT][XD]
   T* Dm = Y + XT * XD;                  // [NP][XD]
   T* rpart = Dm + NP * XD;              // [NSLOT][4][XT]
   T* coef = rpart + 256;                // [4][XT]
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
-  const int nvb = NB * NB;
   const int pb = wave % PB, slot = wave / PB;
   __syncthreads();
   for (int x = tid; x < NP * NP; x += blockDim.x) {
@@ -220,68 +233,67 @@ __device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, in
     Phi[row * XS + N + x - row * (NP - N)] = T(0);
   }
 #pragma unroll
-  for (int u = 0; u < XM_MAXB; ++u) vacc[u] = (v4){T(0), T(0), T(0), T(0)};
+  for (int u = 0; u < NBW; ++u) vacc[u] = (v4){T(0), T(0), T(0), T(0)};
+  // per-lane LDS offsets, fixed for all tiles
+  const int a1 = (pb * 16 + li) * XS + lg;                           // GEMM 1 A: + k0
+  const int b1 = lg * XD + slot * 16 + li;                           // GEMM 1 B: + k0 XD + jb NSLOT 16
+  const int e1 = (pb * 16 + XMTraits<T>::row(lg, 0)) * XS + slot * 16 + li;  // epilogue
+  const int a2 = lg * XS + li, b2 = lg * XD + li;                    // GEMM 2: + block and k0 offsets
   const int ntiles = (a.g + XT - 1) / XT;
-  const int per_c = N * XT;
   for (int tile = tile0; tile < ntiles; tile += tstride) {
     const int p0 = tile * XT;
     __syncthreads();
-    // eight independent loads in flight per thread before their LDS stores
-    for (int x0 = tid; x0 < 4 * per_c; x0 += 8 * blockDim.x) {
-      T v[8];
+    // Stage the tile: Phi[c][p][mu] = ao[c][mu][p0 + p].  Each wave moves
+    // one (c, mu) row of XT points per 64 / XT rows and instruction (lanes
+    // walk points, coalesced), eight rows' loads in flight before their LDS
+    // stores; c = row / N comes from three compares, not a division.
+    {
+      constexpr int RPW = 64 / XT;  // rows per wave instruction
+      const int p = lane % XT, sub = lane / XT, nrow = 4 * N, rstride = 4 * RPW;
+      const int pt = p0 + p;
+      const bool in = pt < a.g;
+      for (int base = wave * RPW + sub; base < nrow; base += 8 * rstride) {
+        T v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int x = x0 + u * blockDim.x;
-        v[u] = T(0);
-        if (x < 4 * per_c) {
-          const int row = x / XT, p = x - row * XT;  // row = c * N + mu
-          const int c = row / N, mu = row - c * N;
-          const int pt = p0 + p;
-          if (pt < a.g) v[u] = a.ao[c * a.sc + mu * a.smu + (int64_t)pt * a.sp];
+        for (int u = 0; u < 8; ++u) {
+          const int row = base + u * rstride;
+          const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
+          v[u] = (row < nrow && in) ? a.ao[c * a.sc + mu * a.smu + (int64_t)pt * a.sp] : T(0);
         }
-      }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int x = x0 + u * blockDim.x;
-        if (x < 4 * per_c) {
-          const int row = x / XT, p = x - row * XT;
-          const int c = row / N, mu = row - c * N;
-          Phi[(c * XT + p) * XS + mu] = v[u];
+        for (int u = 0; u < 8; ++u) {
+          const int row = base + u * rstride;
+          const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
+          if (row < nrow) Phi[(c * XT + p) * XS + mu] = v[u];
         }
       }
     }
     __syncthreads();
-    // GEMM 1 and the density epilogue
+    // GEMM 1 and the density epilogue: wave (pb, slot) takes point block pb
+    // and the column blocks slot, slot + NSLOT, ...; its 16 partial sums
+    // rho_c of 4 points are reduced over the 16 lanes of each DPP row at once,
+    // after which every lane holds one of them.
     {
-      T part[4][4];
+      T part[16];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
+      for (int k = 0; k < 16; ++k) part[k] = T(0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) part[c][j] = T(0);
-      for (int nb = slot; nb < NB; nb += NSLOT) {
-        v4 acc = {T(0), T(0), T(0), T(0)};
-        for (int k0 = 0; k0 < NP; k0 += 4) {
-          const T av = Phi[(pb * 16 + li) * XS + k0 + lg];
-          const T bv = Dm[(k0 + lg) * XD + nb * 16 + li];
-          acc = XMTraits<T>::mfma(av, bv, acc);
+      for (int jb = 0; jb < L::NJ; ++jb) {
+        if (slot + jb * NSLOT < NB) {  // uniform
+          v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+          for (int k0 = 0; k0 < NP; k0 += 4)
+            acc = XMTraits<T>::mfma(Phi[a1 + k0], Dm[b1 + k0 * XD + jb * NSLOT * 16], acc);
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              part[c * 4 + j] += acc[j] * Phi[e1 + c * XT * XS + j * RS * XS + jb * NSLOT * 16];
         }
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            part[c][j] += acc[j] * Phi[(c * XT + pb * 16 + XMTraits<T>::row(lg, j)) * XS + nb * 16 + li];
       }
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) part[c][j] += __shfl_xor(part[c][j], off, 64);
-      if (li == 0)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) rpart[(slot * 4 + c) * XT + pb * 16 + XMTraits<T>::row(lg, j)] = part[c][j];
+      const T tot = transpose_reduce16_row(part, lane);
+      const int c = li >> 2, j = li & 3;
+      rpart[(slot * 4 + c) * XT + pb * 16 + XMTraits<T>::row(lg, j)] = tot;
     }
     __syncthreads();
     if (tid < XT) {
@@ -312,22 +324,24 @@ __device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, in
       coef[3 * XT + tid] = c3;
     }
     __syncthreads();
-    for (int x = tid; x < XT * NP; x += blockDim.x) {
+    // Y[p][nu] = c0 phi0 + sum_c c_c phi_c (XT NP is a multiple of the 256 threads)
+#pragma unroll
+    for (int it = 0; it < XT * NP / XC_THREADS; ++it) {
+      const int x = tid + it * XC_THREADS;
       const int p = x / NP, nu = x - p * NP;
       Y[p * XD + nu] = coef[p] * Phi[p * XS + nu] + coef[XT + p] * Phi[(XT + p) * XS + nu] +
                        coef[2 * XT + p] * Phi[(2 * XT + p) * XS + nu] + coef[3 * XT + p] * Phi[(3 * XT + p) * XS + nu];
     }
     __syncthreads();
-    // GEMM 2: V[mu][nu] += sum_p phi0[p][mu] Y[p][nu]
+    // GEMM 2: V[mu][nu] += sum_p phi0[p][mu] Y[p][nu], blocks wave + 4u
+#pragma unroll
     for (int k0 = 0; k0 < XT; k0 += 4) {
 #pragma unroll
-      for (int u = 0; u < XM_MAXB; ++u) {
+      for (int u = 0; u < NBW; ++u) {
         const int vb = wave + 4 * u;
-        if (vb < nvb) {
+        if (vb < NVB) {  // uniform
           const int mb = vb / NB, nb = vb - mb * NB;
-          const T av = Phi[(k0 + lg) * XS + mb * 16 + li];
-          const T bv = Y[(k0 + lg) * XD + nb * 16 + li];
-          vacc[u] = XMTraits<T>::mfma(av, bv, vacc[u]);
+          vacc[u] = XMTraits<T>::mfma(Phi[a2 + k0 * XS + mb * 16], Y[b2 + k0 * XD + nb * 16], vacc[u]);
         }
       }
     }
@@ -335,12 +349,13 @@ __device__ inline void xc_tiles_mfma(const XCArgs<T>& a, unsigned char* smem, in
 }
 
 // V blocks of this wave (rows/cols < N) into dst with row stride ld.
-template <typename T>
-__device__ inline void xm_store_v(const typename XMTraits<T>::v4 (&vacc)[XM_MAXB], int N, T* dst, int ld) {
-  const int NB = xm_np(N) / 16, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+template <typename T, int XT, int NP>
+__device__ inline void xm_store_v(const XVacc<T, XT, NP>& vacc, int N, T* dst, int ld) {
+  constexpr int NB = NP / 16, NBW = XTile<XT, NP>::NBW;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lg = lane >> 4;
 #pragma unroll
-  for (int u = 0; u < XM_MAXB; ++u) {
+  for (int u = 0; u < NBW; ++u) {
     const int vb = wave + 4 * u;
     if (vb < NB * NB) {
       const int mb = vb / NB, nb = vb - mb * NB;
@@ -363,13 +378,14 @@ __device__ inline double xc_block_energy(double e_loc, double* red) {
   return s;
 }
 
-template <typename T>
+template <typename T, int NP>
 __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int XT = XMTraits<T>::TILE;
   double e_loc = 0.0;
-  typename XMTraits<T>::v4 vacc[XM_MAXB];
-  xc_tiles_mfma<T>(a, smem, blockIdx.x, gridDim.x, vacc, e_loc);
-  xm_store_v<T>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
+  XVacc<T, XT, NP> vacc;
+  xc_tiles<T, XT, NP>(a, smem, blockIdx.x, gridDim.x, vacc, e_loc);
+  xm_store_v<T, XT, NP>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
   double* red = reinterpret_cast<double*>(smem);
   __syncthreads();
   const double e = xc_block_energy(e_loc, red);
@@ -377,24 +393,33 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
 }
 
 // Batched: workgroup (blockIdx.x, blockIdx.y) takes molecule order[blockIdx.x]
-// and its tiles blockIdx.y, blockIdx.y + gridDim.y, ...; skips converged
-// molecules.  With one part per molecule it writes V + V^T and E_xc directly;
-// otherwise V and E_xc partials go to part_v (part-major over the concatenated
-// matrices) and part_e ([part][molecule]) for xc_batch_reduce_kernel.
-template <typename T, int XT>
+// (nao <= max_nao <= NP) and its tiles blockIdx.y, blockIdx.y + gridDim.y, ...;
+// skips converged molecules.  With one part per molecule it writes V + V^T and
+// E_xc directly; otherwise V and E_xc partials go to part_v
+// ([part][blockIdx.x][max_nao^2], row stride nao) and part_e
+// ([part][blockIdx.x]) for xc_batch_reduce_kernel.
+template <typename T, int XT, int NP>
 __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __restrict__ mols,
-                                                              const int* __restrict__ order,
+                                                              const int* __restrict__ order, int max_nao,
                                                               const double* __restrict__ ctl, const T* __restrict__ ao,
                                                               const T* __restrict__ dm, const T* __restrict__ w,
                                                               T eps, T* __restrict__ vxc, double* __restrict__ exc,
-                                                              T* __restrict__ part_v, double* __restrict__ part_e,
-                                                              int64_t mat_total, int nmol) {
+                                                              T* __restrict__ part_v, double* __restrict__ part_e) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
+  const int N = (int)Mo.nao;
+  const int part = blockIdx.y, nparts = gridDim.y;
+  if (N > max_nao) {  // LDS and tiles were sized for max_nao: flag the molecule
+    if (part == 0) {
+      for (int x = threadIdx.x; x < N * N; x += blockDim.x) vxc[Mo.mat0 + x] = T(NAN);
+      if (threadIdx.x == 0) exc[m] = NAN;
+    }
+    return;
+  }
   XCArgs<T> a;
-  a.N = (int)Mo.nao;
+  a.N = N;
   a.g = (int)Mo.ngrid;
   a.ao = ao + Mo.ao0;
   a.sc = (int64_t)a.N * a.g;
@@ -404,20 +429,19 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   a.w = w + Mo.grid0;
   a.eps = eps;
   double e_loc = 0.0;
-  const int N = a.N;
-  const int part = blockIdx.y, nparts = gridDim.y;
-  typename XMTraits<T>::v4 vacc[XM_MAXB];
-  xc_tiles_mfma<T, XT>(a, smem, part, nparts, vacc, e_loc);
+  XVacc<T, XT, NP> vacc;
+  xc_tiles<T, XT, NP>(a, smem, part, nparts, vacc, e_loc);
   __syncthreads();
   double* red = reinterpret_cast<double*>(smem + sizeof(T) * N * N);
   if (nparts > 1) {
-    xm_store_v<T>(vacc, N, part_v + part * mat_total + Mo.mat0, N);
+    const int64_t slot = (int64_t)part * gridDim.x + blockIdx.x;
+    xm_store_v<T, XT, NP>(vacc, N, part_v + slot * max_nao * max_nao, N);
     const double e = xc_block_energy(e_loc, red);
-    if (threadIdx.x == 0) part_e[(int64_t)part * nmol + m] = e;
+    if (threadIdx.x == 0) part_e[slot] = e;
     return;
   }
   T* Vl = reinterpret_cast<T*>(smem);
-  xm_store_v<T>(vacc, N, Vl, N);
+  xm_store_v<T, XT, NP>(vacc, N, Vl, N);
   __syncthreads();
   T* out = vxc + Mo.mat0;
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
@@ -433,28 +457,28 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
 // (fixed order), one workgroup per active molecule.
 template <typename T>
 __global__ void __launch_bounds__(256) xc_batch_reduce_kernel(const BatchMol* __restrict__ mols,
-                                                              const int* __restrict__ order,
+                                                              const int* __restrict__ order, int max_nao,
                                                               const double* __restrict__ ctl,
                                                               const T* __restrict__ part_v,
-                                                              const double* __restrict__ part_e, int64_t mat_total,
-                                                              int nmol, int nparts, T* __restrict__ vxc,
-                                                              double* __restrict__ exc) {
+                                                              const double* __restrict__ part_e, int nparts,
+                                                              T* __restrict__ vxc, double* __restrict__ exc) {
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao;
+  if (N > max_nao) return;  // flagged by xc_batch_kernel
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
     T v = T(0);
     for (int k = 0; k < nparts; ++k) {
-      const T* P = part_v + k * mat_total + Mo.mat0;
+      const T* P = part_v + ((int64_t)k * gridDim.x + blockIdx.x) * max_nao * max_nao;
       v += P[x] + P[j * N + i];
     }
     vxc[Mo.mat0 + x] = v;
   }
   if (threadIdx.x == 0) {
     double e = 0.0;
-    for (int k = 0; k < nparts; ++k) e += part_e[(int64_t)k * nmol + m];
+    for (int k = 0; k < nparts; ++k) e += part_e[(int64_t)k * gridDim.x + blockIdx.x];
     exc[m] = e;
   }
 }
@@ -618,16 +642,11 @@ __global__ void xc_reduce_kernel(const T* __restrict__ part_v, const double* __r
   }
 }
 
-template <typename T>
+template <typename T, int XT, int NP>
 static size_t xc_lds(int N) {
-  const size_t main = sizeof(T) * xm_lds_elems<T>(N);
+  const size_t main = sizeof(T) * XTile<XT, NP>::lds_elems;
   const size_t tail = sizeof(T) * (size_t)N * N + 8 * sizeof(double);  // V + energy partials after the tiles
   return (main > tail ? main : tail) + 8 * sizeof(double);
-}
-
-template <typename T>
-static int xc_max_nao() {
-  return XM_MAXN;
 }
 
 static int xc_blocks(int g) {
@@ -635,12 +654,20 @@ static int xc_blocks(int g) {
   return ntiles < XC_MAX_BLOCKS ? (ntiles < 1 ? 1 : ntiles) : XC_MAX_BLOCKS;
 }
 
+template <typename T, int NP>
+static int xc_fused_launch(const XCArgs<T>& a, int nb, hipStream_t st) {
+  const size_t lds = xc_lds<T, XMTraits<T>::TILE, NP>(a.N);
+  PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_xc: N=%d needs %zu B LDS", a.N, lds);
+  PSCF_SET_LDS((xc_fused_kernel<T, NP>), lds);
+  xc_fused_kernel<T, NP><<<nb, XC_THREADS, lds, st>>>(a);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
 template <typename T>
 static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int N, const T* dm, const T* w,
                    double eps, double* exc, T* vxc, void* ws, size_t ws_bytes, hipStream_t st) {
-  const size_t lds = xc_lds<T>(N);
-  PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_xc: N=%d needs %zu B LDS", N, lds);
-  PSCF_REQUIRE(N <= xc_max_nao<T>(), "pscf_xc: N=%d too large (max %d)", N, xc_max_nao<T>());
+  PSCF_REQUIRE(N <= XM_MAXN, "pscf_xc: N=%d too large (max %d)", N, XM_MAXN);
   const int nb = xc_blocks(g);
   const size_t need = sizeof(T) * (size_t)nb * N * N + sizeof(double) * nb;
   PSCF_REQUIRE(ws && ws_bytes >= need, "pscf_xc: workspace too small (%zu < %zu)", ws_bytes, need);
@@ -656,9 +683,15 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
   a.eps = (T)eps;
   a.part_v = (T*)ws;
   a.part_e = (double*)((char*)ws + sizeof(T) * (size_t)nb * N * N);
-  PSCF_SET_LDS(xc_fused_kernel<T>, lds);
-  xc_fused_kernel<T><<<nb, XC_THREADS, lds, st>>>(a);
-  PSCF_HIP(hipGetLastError());
+  int rc;
+  switch (xm_np(N)) {
+    case 16: rc = xc_fused_launch<T, 16>(a, nb, st); break;
+    case 32: rc = xc_fused_launch<T, 32>(a, nb, st); break;
+    case 48: rc = xc_fused_launch<T, 48>(a, nb, st); break;
+    case 64: rc = xc_fused_launch<T, 64>(a, nb, st); break;
+    default: rc = xc_fused_launch<T, 80>(a, nb, st); break;
+  }
+  if (rc) return rc;
   xc_reduce_kernel<T><<<(N * N + 3) / 4, 256, 0, st>>>(a.part_v, a.part_e, nb, N, vxc, exc);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
@@ -792,46 +825,67 @@ extern "C" int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, 
   return PSCF_OK;
 }
 
-template <typename T, int XT>
+template <typename T, int XT, int NP>
 static int batch_xc_launch(const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                            const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc,
-                           double* d_exc, int split, void* d_part_v, double* d_part_e, int64_t mat_total,
-                           int m_total, hipStream_t st) {
-  const size_t main = sizeof(T) * xm_lds_elems<T, XT>(max_nao);
-  const size_t tail = sizeof(T) * (size_t)max_nao * max_nao + 8 * sizeof(double);
-  const size_t lds = (main > tail ? main : tail) + 8 * sizeof(double);
+                           double* d_exc, int split, void* d_part_v, double* d_part_e, hipStream_t st) {
+  const size_t lds = xc_lds<T, XT, NP>(max_nao);
   PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
-  PSCF_SET_LDS((xc_batch_kernel<T, XT>), lds);
-  xc_batch_kernel<T, XT><<<dim3(nmol, split), XC_THREADS, lds, st>>>(
-      (const BatchMol*)d_mols, d_order, d_ctl, (const T*)d_ao, (const T*)d_dm, (const T*)d_w, (T)eps, (T*)d_vxc,
-      d_exc, (T*)d_part_v, d_part_e, mat_total, m_total);
+  PSCF_SET_LDS((xc_batch_kernel<T, XT, NP>), lds);
+  xc_batch_kernel<T, XT, NP><<<dim3(nmol, split), XC_THREADS, lds, st>>>(
+      (const BatchMol*)d_mols, d_order, max_nao, d_ctl, (const T*)d_ao, (const T*)d_dm, (const T*)d_w, (T)eps,
+      (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e);
   PSCF_HIP(hipGetLastError());
   if (split > 1) {
-    xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, d_ctl, (const T*)d_part_v,
-                                                   d_part_e, mat_total, m_total, split, (T*)d_vxc, d_exc);
+    xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, max_nao, d_ctl,
+                                                   (const T*)d_part_v, d_part_e, split, (T*)d_vxc, d_exc);
     PSCF_HIP(hipGetLastError());
   }
   return PSCF_OK;
 }
 
+template <typename T, int XT>
+static int batch_xc_np(int np, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
+                       const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
+                       int split, void* d_part_v, double* d_part_e, hipStream_t st) {
+#define PSCF_XC_NP(V)                                                                                              \
+  case V:                                                                                                          \
+    return batch_xc_launch<T, XT, V>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, \
+                                     split, d_part_v, d_part_e, st);
+  switch (np) {
+    PSCF_XC_NP(16)
+    PSCF_XC_NP(32)
+    PSCF_XC_NP(48)
+    PSCF_XC_NP(64)
+    PSCF_XC_NP(80)
+  }
+#undef PSCF_XC_NP
+  PSCF_REQUIRE(false, "pscf_batch_xc: nao padding %d unsupported", np);
+  return PSCF_EINVAL;
+}
+
 extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
                              const double* d_ctl, const void* d_ao, const void* d_dm, const void* d_w, double eps,
                              void* d_vxc, double* d_exc, int split, int tile, void* d_part_v, double* d_part_e,
-                             int64_t mat_total, int m_total, void* stream) {
+                             int64_t part_v_elems, int64_t part_e_elems, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_xc: bad dtype");
   PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && d_ao && d_dm && d_w && d_vxc && d_exc,
                "pscf_batch_xc: bad arguments");
-  PSCF_REQUIRE(split >= 1 && split <= 64 && (split == 1 || (d_part_v && d_part_e && mat_total > 0 && m_total > 0)),
-               "pscf_batch_xc: split %d needs part buffers", split);
+  PSCF_REQUIRE(split >= 1 && split <= 64, "pscf_batch_xc: bad split %d", split);
+  PSCF_REQUIRE(split == 1 || (d_part_v && d_part_e && part_v_elems >= (int64_t)split * nmol * max_nao * max_nao &&
+                              part_e_elems >= (int64_t)split * nmol),
+               "pscf_batch_xc: split %d needs part buffers of %lld and %lld elements", split,
+               (long long)split * nmol * max_nao * max_nao, (long long)split * nmol);
   PSCF_REQUIRE(tile == 32 || (tile == 64 && dtype == PSCF_F32), "pscf_batch_xc: tile %d (32, or 64 in FP32)", tile);
   PSCF_REQUIRE(max_nao <= XM_MAXN, "pscf_batch_xc: nao %d too large", max_nao);
   hipStream_t st = (hipStream_t)stream;
+  const int np = xm_np(max_nao);
   if (dtype == PSCF_F64)
-    return batch_xc_launch<double, 32>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                       split, d_part_v, d_part_e, mat_total, m_total, st);
+    return batch_xc_np<double, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
+                                   split, d_part_v, d_part_e, st);
   if (tile == 64)
-    return batch_xc_launch<float, 64>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                      split, d_part_v, d_part_e, mat_total, m_total, st);
-  return batch_xc_launch<float, 32>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                    split, d_part_v, d_part_e, mat_total, m_total, st);
+    return batch_xc_np<float, 64>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
+                                  split, d_part_v, d_part_e, st);
+  return batch_xc_np<float, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, split,
+                                d_part_v, d_part_e, st);
 }
