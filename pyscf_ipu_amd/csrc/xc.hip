@@ -239,6 +239,9 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
   const int b1 = lg * XD + slot * 16 + li;                           // GEMM 1 B: + k0 XD + jb NSLOT 16
   const int e1 = (pb * 16 + XMTraits<T>::row(lg, 0)) * XS + slot * 16 + li;  // epilogue
   const int a2 = lg * XS + li, b2 = lg * XD + li;                    // GEMM 2: + block and k0 offsets
+  // AO strides as 32-bit element offsets (the host checks that a molecule's
+  // AO block is below 2^31 elements)
+  const int sc = (int)a.sc, smu = (int)a.smu, sp = (int)a.sp;
   const int ntiles = (a.g + XT - 1) / XT;
   for (int tile = tile0; tile < ntiles; tile += tstride) {
     const int p0 = tile * XT;
@@ -252,13 +255,14 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
       const int p = lane % XT, sub = lane / XT, nrow = 4 * N, rstride = 4 * RPW;
       const int pt = p0 + p;
       const bool in = pt < a.g;
+      const int ptoff = pt * sp;
       for (int base = wave * RPW + sub; base < nrow; base += 8 * rstride) {
         T v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int row = base + u * rstride;
           const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
-          v[u] = (row < nrow && in) ? a.ao[c * a.sc + mu * a.smu + (int64_t)pt * a.sp] : T(0);
+          v[u] = (row < nrow && in) ? a.ao[c * sc + mu * smu + ptoff] : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -279,17 +283,21 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
       for (int k = 0; k < 16; ++k) part[k] = T(0);
 #pragma unroll
       for (int jb = 0; jb < L::NJ; ++jb) {
-        if (slot + jb * NSLOT < NB) {  // uniform
-          v4 acc = {T(0), T(0), T(0), T(0)};
+        // column block slot + jb NSLOT; past NB the wave repeats block NB - 1
+        // with its sums scaled by zero (no branch, so the accumulators stay
+        // in place)
+        const bool on = slot + jb * NSLOT < NB;
+        const int cb = on ? jb * NSLOT * 16 : (NB - 1 - slot) * 16;
+        const T scale = on ? T(1) : T(0);
+        v4 acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-          for (int k0 = 0; k0 < NP; k0 += 4)
-            acc = XMTraits<T>::mfma(Phi[a1 + k0], Dm[b1 + k0 * XD + jb * NSLOT * 16], acc);
+        for (int k0 = 0; k0 < NP; k0 += 4) acc = XMTraits<T>::mfma(Phi[a1 + k0], Dm[b1 + k0 * XD + cb], acc);
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
+        for (int j = 0; j < 4; ++j) acc[j] *= scale;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-              part[c * 4 + j] += acc[j] * Phi[e1 + c * XT * XS + j * RS * XS + jb * NSLOT * 16];
-        }
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) part[c * 4 + j] += acc[j] * Phi[e1 + c * XT * XS + j * RS * XS + cb];
       }
       const T tot = transpose_reduce16_row(part, lane);
       const int c = li >> 2, j = li & 3;
@@ -333,16 +341,15 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
                        coef[2 * XT + p] * Phi[(2 * XT + p) * XS + nu] + coef[3 * XT + p] * Phi[(3 * XT + p) * XS + nu];
     }
     __syncthreads();
-    // GEMM 2: V[mu][nu] += sum_p phi0[p][mu] Y[p][nu], blocks wave + 4u
+    // GEMM 2: V[mu][nu] += sum_p phi0[p][mu] Y[p][nu], blocks wave + 4u (a
+    // block index past NVB repeats the last block, whose copy is not stored)
 #pragma unroll
     for (int k0 = 0; k0 < XT; k0 += 4) {
 #pragma unroll
       for (int u = 0; u < NBW; ++u) {
-        const int vb = wave + 4 * u;
-        if (vb < NVB) {  // uniform
-          const int mb = vb / NB, nb = vb - mb * NB;
-          vacc[u] = XMTraits<T>::mfma(Phi[a2 + k0 * XS + mb * 16], Y[b2 + k0 * XD + nb * 16], vacc[u]);
-        }
+        const int vb = min(wave + 4 * u, NVB - 1);
+        const int mb = vb / NB, nb = vb - mb * NB;
+        vacc[u] = XMTraits<T>::mfma(Phi[a2 + k0 * XS + mb * 16], Y[b2 + k0 * XD + nb * 16], vacc[u]);
       }
     }
   }
@@ -411,7 +418,9 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao;
   const int part = blockIdx.y, nparts = gridDim.y;
-  if (N > max_nao) {  // LDS and tiles were sized for max_nao: flag the molecule
+  // LDS and tiles were sized for max_nao, offsets are 32-bit: flag a molecule
+  // outside those bounds
+  if (N > max_nao || (int64_t)4 * N * Mo.ngrid >= ((int64_t)1 << 31)) {
     if (part == 0) {
       for (int x = threadIdx.x; x < N * N; x += blockDim.x) vxc[Mo.mat0 + x] = T(NAN);
       if (threadIdx.x == 0) exc[m] = NAN;
@@ -466,7 +475,7 @@ __global__ void __launch_bounds__(256) xc_batch_reduce_kernel(const BatchMol* __
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao;
-  if (N > max_nao) return;  // flagged by xc_batch_kernel
+  if (N > max_nao || (int64_t)4 * N * Mo.ngrid >= ((int64_t)1 << 31)) return;  // flagged by xc_batch_kernel
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
     T v = T(0);
@@ -668,6 +677,8 @@ template <typename T>
 static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int N, const T* dm, const T* w,
                    double eps, double* exc, T* vxc, void* ws, size_t ws_bytes, hipStream_t st) {
   PSCF_REQUIRE(N <= XM_MAXN, "pscf_xc: N=%d too large (max %d)", N, XM_MAXN);
+  PSCF_REQUIRE(3 * sc + (int64_t)(N - 1) * smu + (int64_t)(g - 1) * sp < ((int64_t)1 << 31),
+               "pscf_xc: AO block of %lld elements needs 32-bit offsets", (long long)(4 * sc));
   const int nb = xc_blocks(g);
   const size_t need = sizeof(T) * (size_t)nb * N * N + sizeof(double) * nb;
   PSCF_REQUIRE(ws && ws_bytes >= need, "pscf_xc: workspace too small (%zu < %zu)", ws_bytes, need);
