@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 METRIC = "molecules/sec at |ΔE|≤1 mHa vs FP64; mean SCF iters; batched+learned vs plain"
 GOLDEN = os.path.join(ROOT, "tests", "golden", "plain_sto3g_l0.json")
 HBM_PEAK_GBS = 8000.0
+PLAIN_STREAMS = 4
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 
@@ -55,6 +56,11 @@ def _cpu_baseline(mol_strs, budget_s: float = 25.0):
     return {"value": done / dt, "unit": "molecules/s", "cores": cores, "kind": "port",
             "sample": f"{done} of the benchmark's 9-heavy-atom molecules, FP64 oracle (C Rys ERIs with OpenMP over "
                       f"{cores} threads + numpy SCF), {dt:.1f} s"}
+
+
+def _roof(bound, achieved, peak, unit, ms, model):
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+            "mean_launch_ms": ms, "work_model": model}
 
 
 def _batched(nmol: int, steps: int, warmup: int, its: int):
@@ -97,6 +103,7 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
         out[name] = {"value": float(ok.sum() / wall), "unit": "molecules/s", "n_molecules": nmol,
                      "ms_per_step": wall * 1e3, "passed_1mha": int(ok.sum()), "max_abs_dE_vs_fp64_ha": float(dE.max()),
                      "mean_abs_dE_vs_fp64_ha": float(dE.mean()),
+                     "max_abs_dgap_vs_fp64_ha": float(np.abs(res.hlgap - r64.hlgap)[ok].max()) if ok.any() else None,
                      "mean_scf_iters": float(g.iterations_total.mean()), "fallback_to_minao": int(g.fallback.sum()),
                      "converged": int(res.converged.sum()),
                      "autotune_ms_per_launch": {k: {a: round(b, 3) for a, b in v.items()}
@@ -104,6 +111,30 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
                      "autotune_chosen": {"xc_tile": (g.tuning or {}).get("chosen_tile"),
                                          "jk_threads": (g.tuning or {}).get("chosen_jk_threads"),
                                          "split": (g.tuning or {}).get("chosen_split")}}
+    # per-kernel rooflines of the batched launches over all 4096 molecules:
+    # J/K and XC from the autotuner's per-launch times of the chosen tiling
+    # (every molecule active), the ERIs from the set-up events
+    from pyscf_ipu_amd import roofline
+    sizes_b = batch.BatchRKS(mols, dtype="fp32")
+    tun = g.tuning or {}
+    ms = tun.get("ms", {})
+    rl = {}
+    if ms:
+        jk_ms = ms["jk"][f"threads{tun['chosen_jk_threads']}/split{tun['chosen_split']['jk']}"]
+        xc_ms = ms["xc"][f"tile{tun['chosen_tile']}/split{tun['chosen_split']['xc']}"]
+        jk_b = 4.0 * sizes_b.sizes["eri"]
+        xc_b = 4.0 * (sizes_b.sizes["ao"] + sizes_b.sizes["grid"])
+        rl["jk_s8"] = _roof("hbm", jk_b / (jk_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", jk_ms,
+                            "4 B per unique s8 ERI of all molecules")
+        rl["xc"] = _roof("hbm", xc_b / (xc_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", xc_ms,
+                         "4 B per AO value/gradient (4 g N) + weight")
+    sm = g.setup_ms or {}
+    if "eri_s8" in sm:
+        fl = roofline.batch_eri_flops(sizes_b)
+        rl["eri_s8"] = _roof("valu", fl / (sm["eri_s8"] * 1e-3) / 1e12, PEAK_TFLOPS["fp32"], "TFLOP/s", sm["eri_s8"],
+                             "Rys op-count model (pyscf_ipu_amd/roofline.py); peak = FP32 vector rate")
+    out["minao"]["kernel_rooflines"] = rl
+    out["minao"]["setup_ms"] = {k: round(v, 2) for k, v in sm.items()}
     common = {"mean_scf_iters_fp64_minao": float(r64.iterations.mean()),
               "fp64_reference_converged": int(r64.converged.sum()),
               "dtype": "f32 (ERI/JK/XC/Becke) + f64 (eigh/DIIS/energy)",
@@ -132,7 +163,7 @@ def main(argv=None):
     import torch
     from pyscf_ipu_amd import molgen
     from pyscf_ipu_amd.gto import build_mol
-    from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS, KernelEvents
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import KernelEvents, run_molecules
     from pyscf_ipu_amd import roofline
 
     torch.cuda.set_device(0)
@@ -147,20 +178,15 @@ def main(argv=None):
     # FP32 runs stop at |dE| < 1e-6 Ha, above their energy noise; FP64 at 1e-8
     conv_tol = 1e-8 if a.dtype == "fp64" else 1e-6
 
-    def run_all(ev=None):
-        res = []
-        for m in mols:
-            res.append(RKS(m, level=0, dtype=a.dtype, events=ev).run(its=a.its, conv_tol=conv_tol))
-        return res
-
+    # timed: each molecule's SCF with its own launches (one molecule per
+    # launch), 4 molecules in flight on 4 HIP streams
     for _ in range(a.warmup):
-        run_all()
+        run_molecules(mols, streams=PLAIN_STREAMS, dtype=a.dtype, its=a.its, conv_tol=conv_tol)
     torch.cuda.synchronize()
-    ev = KernelEvents()
     t0 = time.perf_counter()
     results = []
     for _ in range(a.steps):
-        results = run_all(ev)
+        results = run_molecules(mols, streams=PLAIN_STREAMS, dtype=a.dtype, its=a.its, conv_tol=conv_tol)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
@@ -173,6 +199,14 @@ def main(argv=None):
     else:
         dE, passed, max_dE = None, len(results), None
     value = passed * a.steps / wall
+    # one molecule at a time (the reference's serial loop), with HIP events
+    # around every launch group for the per-kernel times and the roofline
+    ev = KernelEvents()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    run_molecules(mols, streams=1, dtype=a.dtype, its=a.its, conv_tol=conv_tol, events=ev)
+    torch.cuda.synchronize()
+    serial_wall = time.perf_counter() - t2
 
     # roofline of the dominant kernel group, from the HIP events of the timed region
     ks = ev.summary()
@@ -188,6 +222,18 @@ def main(argv=None):
     roof = {"kernel": dominant, "bound": work["bound"], "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": None, "mean_launch_ms": ks[dominant]["mean_ms"],
             "work_model": work["model"]}
+    kernel_rooflines = {}
+    for k, v in ks.items():
+        try:
+            w = roofline.per_launch_work(k, mols, a.dtype)
+        except KeyError:
+            continue
+        sec = v["mean_ms"] * 1e-3
+        if w["bound"] == "hbm":
+            kernel_rooflines[k] = _roof("hbm", w["bytes"] / sec / 1e9, HBM_PEAK_GBS, "GB/s", v["mean_ms"], w["model"])
+        else:
+            kernel_rooflines[k] = _roof(w["bound"], w["flops"] / sec / 1e12, PEAK_TFLOPS[a.dtype], "TFLOP/s",
+                                        v["mean_ms"], w["model"])
     cpu = None if a.no_cpu_baseline else _cpu_baseline(mol_strs)
     forms = {}
     if a.batched_nmol > 0:
@@ -202,12 +248,14 @@ def main(argv=None):
         "dtype": "f64" if a.dtype == "fp64" else "f32 (ERI/JK/XC) + f64 (eigh/DIIS/energy)",
         "data": "synthetic",
         "config": {"workload": "plain one-molecule-per-launch RKS B3LYP SCF: 32 seeded 9-heavy-atom C/N/O "
-                               "molecules, STO-3G, grid level 0, |dE|<1e-8 stop, <=20 iterations",
+                               "molecules, STO-3G, grid level 0, |dE|<1e-8 stop, <=20 iterations, 4 molecules in "
+                               "flight on 4 HIP streams",
                    "n_molecules": a.nmol, "basis": "sto-3g", "grid_level": 0, "form": "plain",
                    "nao_range": [int(min(m.nao_nr() for m in mols)), int(max(m.nao_nr() for m in mols))]},
         "mean_scf_iters": float(iters.mean()), "passed_1mha": passed, "max_abs_dE_ha": max_dE,
-        "kernel_ms_per_step": {k: v["total_ms"] / a.steps for k, v in ks.items()},
-        "roofline": roof, "cpu_baseline": cpu, "forms": forms,
+        "serial_one_stream": {"value": passed / serial_wall, "ms_per_step": serial_wall * 1e3},
+        "kernel_ms_per_step": {k: v["total_ms"] for k, v in ks.items()},
+        "roofline": roof, "kernel_rooflines": kernel_rooflines, "cpu_baseline": cpu, "forms": forms,
     }
     print(json.dumps(line), flush=True)
     return 0

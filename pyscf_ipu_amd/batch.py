@@ -31,12 +31,31 @@ DIIS_N = 9
 CTL = 8
 SPLITS = (1, 2, 4, 8)   # workgroups per molecule tried by the autotuner
 MAX_SPLIT = max(SPLITS)
-MIN_XC_WORKGROUPS = 512  # a small class of molecules is split further, to fill the 256 CUs
+MIN_WORKGROUPS = 512  # a small class of molecules is split further, to fill the 256 CUs
+
+
+class PhaseTimer:
+    """HIP events around named phases on the current stream; `ms()` reads them
+    after the caller has synchronised (recording adds no host sync)."""
+
+    def __init__(self):
+        self.marks = []
+
+    def __call__(self, name: str):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.marks.append((name, e))
+
+    def ms(self) -> dict:
+        out = {}
+        for (name, a), (_, b) in zip(self.marks, self.marks[1:]):
+            out[name] = out.get(name, 0.0) + a.elapsed_time(b)
+        return out
 
 
 def class_split(split: int, count: int) -> int:
-    """Workgroups per molecule of an XC launch over `count` molecules."""
-    return min(64, max(split, -(-MIN_XC_WORKGROUPS // count)))
+    """Workgroups per molecule of an XC or J/K launch over `count` molecules."""
+    return min(64, max(split, -(-MIN_WORKGROUPS // count)))
 JK_THREADS = (256, 512)   # J/K workgroup sizes tried by the autotuner
 XC_TILES = {"fp32": (32, 64), "fp64": (32,)}   # grid points per XC tile
 _TUNE_CACHE = {}        # tuning results per batch shape
@@ -245,12 +264,16 @@ class BatchRKS:
         toff, tn, tg = device.to_dev(self.tmpl_off), device.to_dev(self.tmpl_n), device.to_dev(self.tmpl_grid)
         P = device.ptr
         st = device.stream()
+        tm = self.setup_timer = PhaseTimer()
+        tm("grid")
         _lib.call("pscf_batch_grid", P(gw), len(self.grid_work), P(ae), P(ap0), P(am), P(toff), P(tn), P(tg),
                   P(self.d_atm), P(self.d_env), P(self.coords), P(self.vol), P(self.owner), P(self.ptmol), st)
         self.w = device.empty(G, T)
+        tm("becke")
         _lib.call("pscf_batch_becke", self.code, P(self.d_desc), P(self.ptmol), P(self.owner), P(self.coords),
                   P(self.vol), G, P(self.d_atm), P(self.d_env), P(self.w), None, st)
         self.ao = device.empty(S["ao"], T)
+        tm("ao")
         _lib.call("pscf_batch_ao", self.code, P(self.d_desc), P(self.ptmol), P(self.coords), G, P(self.d_atm),
                   P(self.d_bas), P(self.d_env), P(self.ao), st)
         # one-electron part, orthogonaliser, guess
@@ -258,8 +281,10 @@ class BatchRKS:
         self.S, self.Hc, self.Linv = device.empty(nm, f64), device.empty(nm, f64), device.empty(nm, f64)
         self.ctl = device.zeros((self.M, CTL), f64)
         self.ctl[:, 1] = 1.0
+        tm("int1e")
         _lib.call("pscf_batch_int1e", P(self.d_desc), self.M, P(self.d_atm), P(self.d_bas), P(self.d_env),
                   P(self.d_shell_ao), P(self.S), P(self.Hc), st)
+        tm("chol_guess")
         _lib.call("pscf_batch_chol", P(self.d_desc), self.M, self.max_nao, P(self.S), P(self.Linv), P(self.ctl), st)
         self.D = device.empty(nm, f64)
         self.Dt = device.empty(nm, T)
@@ -276,10 +301,13 @@ class BatchRKS:
         works = [device.to_dev(w) for w in self.eri_work]
         arr = (ctypes.c_void_p * 6)(*[w.data_ptr() for w in works])
         cnt = (ctypes.c_int * 6)(*[len(w) for w in self.eri_work])
+        tm("eri_s8")
         _lib.call("pscf_batch_eri_s8", self.code, P(self.d_desc), self.M, arr, cnt, P(self.d_atm), P(self.d_bas),
                   P(self.d_env), P(self.d_shell_ao), P(self.d_shl), P(self.eri), st)
         # stored over their multiplicity, the form the J/K contraction takes
+        tm("eri_scale")
         _lib.call("pscf_batch_eri_scale", self.code, P(self.d_desc), self.M, self.max_nao, P(self.eri), st)
+        tm("end")
         # iteration buffers.  dJK is FP64; in FP32 mode it is built
         # incrementally, dJK += JK(D_new - D_old) with the change in FP32, so the
         # FP32 rounding shrinks with the density change and the energy can
@@ -316,7 +344,8 @@ class BatchRKS:
         for start, count, cmax in self.nao_classes:
             order = ctypes.c_void_p(self.d_order.data_ptr() + 4 * start)
             _lib.call("pscf_batch_jk_s8", self.code, P(self.d_desc), order, count, cmax, ctl_ptr, P(self.eri), P(dm),
-                      0.2, P(self.dJK if out is None else out), int(self.incremental), split, threads, st)
+                      0.2, P(self.dJK if out is None else out), int(self.incremental), class_split(split, count),
+                      threads, st)
 
     def _fock(self, ctl_ptr, first=False):
         self._xc(ctl_ptr, self.split["xc"], self.xc_tile)

@@ -258,6 +258,7 @@ class GuessedResult:
     fallback: np.ndarray         # (M,) bool: re-run from MINAO
     iterations_total: np.ndarray  # learned-guess iterations + fallback iterations
     tuning: dict = None          # autotuner record of the main batch (BatchRKS.tuning)
+    setup_ms: dict = None        # set-up phase times of the main batch (BatchRKS.setup_timer)
 
 
 def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_its: int = 20,
@@ -271,8 +272,8 @@ def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_i
     else:
         b.setup()
         r = b.run(max_its=max_its, conv_tol=conv_tol, graph=graph, tune=tune)
-        return GuessedResult(r, np.zeros(len(molecules), bool), r.iterations.copy(), b.tuning)
-    tuning = b.tuning
+        return GuessedResult(r, np.zeros(len(molecules), bool), r.iterations.copy(), b.tuning, b.setup_timer.ms())
+    tuning, setup_ms = b.tuning, b.setup_timer.ms()
     redo = np.where(~r.converged & ~r.failed)[0]
     total = r.iterations.copy()
     fb = np.zeros(len(molecules), bool)
@@ -290,4 +291,4 @@ def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_i
             total[i] += r2.iterations[k]
         fb[redo] = True
         r.n_iter_launches += r2.n_iter_launches
-    return GuessedResult(r, fb, total, tuning)
+    return GuessedResult(r, fb, total, tuning, setup_ms)

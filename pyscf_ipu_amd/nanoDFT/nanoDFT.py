@@ -287,6 +287,49 @@ class RKS:
                          self.D.cpu().numpy(), len(log), converged, {"setup_s": t1 - t0, "scf_s": t2 - t1})
 
 
+def run_molecules(mols, streams: int = 4, level: int = 0, dtype: str = "fp64", its: int = 20,
+                  conv_tol: float = 1e-8, diis: bool = True, events: Optional[KernelEvents] = None):
+    """SCF of several independent molecules, each with its own launches (one
+    molecule per launch, as nanoDFT runs them), up to `streams` molecules in
+    flight on their own HIP streams from as many host threads.  A molecule's
+    SCF is latency-bound (its eigensolver is one workgroup), so concurrent
+    molecules fill otherwise idle CUs; 4 matches the box's hardware queues.
+    Results come back in input order and equal the one-at-a-time results."""
+    import threading
+    out = [None] * len(mols)
+    if streams <= 1:
+        for i, m in enumerate(mols):
+            out[i] = RKS(m, level=level, dtype=dtype, events=events).run(its=its, conv_tol=conv_tol, diis=diis)
+        return out
+    nxt, lock, errors = [0], threading.Lock(), []
+
+    def worker():
+        s = torch.cuda.Stream()
+        try:
+            with torch.cuda.stream(s):
+                while True:
+                    with lock:
+                        i = nxt[0]
+                        nxt[0] += 1
+                    if i >= len(mols):
+                        break
+                    out[i] = RKS(mols[i], level=level, dtype=dtype, events=events).run(its=its, conv_tol=conv_tol,
+                                                                                        diis=diis)
+        except BaseException as e:  # re-raised on the calling thread
+            errors.append(e)
+        finally:
+            s.synchronize()
+
+    threads = [threading.Thread(target=worker) for _ in range(min(streams, len(mols)))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return out
+
+
 def nanoDFT(mol, opts):
     """Reference-shaped entry point: returns (energies (n, 5) in eV,
     (logged_energies eV, hlgaps eV, mo_energy, mo_coeff, grid_coords, grid_weights))."""

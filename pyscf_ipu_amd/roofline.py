@@ -33,8 +33,20 @@ def eri_flops_per_primitive(li, lj, lk, ll) -> int:
 
 def eri_s8_flops(mol) -> float:
     """Algorithmic FLOPs of one pscf_eri_s8 call (all unique shell quartets)."""
-    l = mol._bas[:, 1].astype(int)
-    npr = mol._bas[:, 2].astype(int)
+    return _eri_flops_shells(mol._bas[:, 1].astype(int), mol._bas[:, 2].astype(int))
+
+
+def batch_eri_flops(b) -> float:
+    """Algorithmic FLOPs of pscf_batch_eri_s8 over every molecule of a
+    BatchRKS (shells bas0 .. bas0 + nbas of each molecule)."""
+    tot = 0.0
+    for m in range(b.M):
+        b0, nb = int(b.desc[m, 5]), int(b.desc[m, 1])
+        tot += _eri_flops_shells(b.bas[b0:b0 + nb, 1].astype(int), b.bas[b0:b0 + nb, 2].astype(int))
+    return tot
+
+
+def _eri_flops_shells(l, npr) -> float:
     pairs = [(i, j) for i in range(len(l)) for j in range(i + 1)]
     # aggregate pairs by (l_a, l_b sorted, nprim product)
     from collections import Counter

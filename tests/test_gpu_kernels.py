@@ -259,3 +259,16 @@ def test_gpu_becke_grid_scf(hip):
     a = RKS(mol, level=0, dtype="fp64").run(its=30, conv_tol=1e-9)
     b = RKS(mol, level=0, dtype="fp64", grid=grids.build_grid(mol, 0)).run(its=30, conv_tol=1e-9)
     assert abs(a.energy - b.energy) < 1e-9
+
+
+def test_run_molecules_streams_equal_serial(hip):
+    """Molecules in flight on four HIP streams give the one-at-a-time results
+    (to the FP64 rounding of the LDS-atomic J/K order) in input order."""
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import run_molecules
+    mols = [build_mol(s) for s in [METHANE] + molgen.plain_set(5, 3, seed=11)]
+    serial = run_molecules(mols, streams=1, its=30, conv_tol=1e-9)
+    par = run_molecules(mols, streams=4, its=30, conv_tol=1e-9)
+    for a, b in zip(serial, par):
+        assert abs(a.energy - b.energy) < 1e-9
+        assert a.iterations == b.iterations
+        assert a.converged and b.converged
