@@ -20,6 +20,7 @@
 
 #include "batch.h"
 #include "common.h"
+#include "wave.h"
 
 namespace pscf {
 
@@ -447,7 +448,8 @@ __device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, i
       PB[d] = P[d] - RB[d];
     }
     if (iat == 0) {  // overlap and kinetic once per primitive pair
-      const double s00 = pow(PI * rp, 1.5) * ex;
+      const double prp = PI * rp;
+      const double s00 = prp * sqrt(prp) * ex;
       const double h = 0.5 * rp;
       double s1[3][2][4];
       for (int d = 0; d < 3; ++d) {
@@ -512,15 +514,15 @@ __device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, i
         }
     }
   }
+  // wavefront sums by DPP (totals in lane 63), only of the na nb used entries
 #pragma unroll
   for (int k = 0; k < 9; ++k)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      s_acc[k] += __shfl_xor(s_acc[k], off, 64);
-      t_acc[k] += __shfl_xor(t_acc[k], off, 64);
-      v_acc[k] += __shfl_xor(v_acc[k], off, 64);
+    if (k / 3 < na && k % 3 < nb) {  // uniform
+      s_acc[k] = wave_sum_last(s_acc[k]);
+      t_acc[k] = wave_sum_last(t_acc[k]);
+      v_acc[k] = wave_sum_last(v_acc[k]);
     }
-  if (lane == 0)
+  if (lane == 63)
     for (int fa = 0; fa < na; ++fa)
       for (int fb = 0; fb < nb; ++fb) {
         const int64_t r = ao_a + fa, c = ao_b + fb;

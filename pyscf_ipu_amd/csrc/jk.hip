@@ -317,27 +317,6 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const Batc
 //   J   (p, q) += sum_rs w2 dm[r][s]  (one reduction per row),
 // with w the ERI divided by its multiplicity (pscf_batch_eri_scale) and
 // wk = -(hyb/2) w, w2 = 2 w; the constant factors are applied per row.
-template <int CTRL, int ROW_MASK>
-__device__ inline float dpp_add(float x) {
-  return x + dpp_t<CTRL, ROW_MASK>(x);
-}
-template <int CTRL, int ROW_MASK>
-__device__ inline double dpp_add(double x) {
-  return x + dpp_t<CTRL, ROW_MASK>(x);
-}
-
-// sum over the 64 lanes, valid in lane 63
-template <typename T>
-__device__ inline T wave_sum_last(T x) {
-  x = dpp_add<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]
-  x = dpp_add<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]
-  x = dpp_add<0x141, 0xf>(x);  // row_half_mirror
-  x = dpp_add<0x140, 0xf>(x);  // row_mirror
-  x = dpp_add<0x142, 0xa>(x);  // row_bcast:15 into rows 1, 3
-  x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2, 3
-  return x;
-}
-
 __device__ inline float lane_read(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
 __device__ inline double lane_read(double x, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),

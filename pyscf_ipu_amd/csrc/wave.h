@@ -83,4 +83,25 @@ __device__ inline T transpose_reduce16_row(T (&x)[16], int lane) {
   return x[0];
 }
 
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_add(float x) {
+  return x + dpp_t<CTRL, ROW_MASK>(x);
+}
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_add(double x) {
+  return x + dpp_t<CTRL, ROW_MASK>(x);
+}
+
+// sum over the 64 lanes, valid in lane 63
+template <typename T>
+__device__ inline T wave_sum_last(T x) {
+  x = dpp_add<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]
+  x = dpp_add<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]
+  x = dpp_add<0x141, 0xf>(x);  // row_half_mirror
+  x = dpp_add<0x140, 0xf>(x);  // row_mirror
+  x = dpp_add<0x142, 0xa>(x);  // row_bcast:15 into rows 1, 3
+  x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+
 }  // namespace pscf

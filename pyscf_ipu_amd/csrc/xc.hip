@@ -526,20 +526,19 @@ __constant__ double c_bragg[11] = {0.0,
 
 // Becke weights of all points of all molecules; each point's cell function runs
 // over its own molecule's atoms only.  T: arithmetic (FP64 or FP32).
+// Becke's smoothing step applied three times to nu (an odd function of nu).
 template <typename T>
-__global__ void __launch_bounds__(256) becke_batch_kernel(const BatchMol* __restrict__ mols,
-                                                          const int* __restrict__ ptmol,
-                                                          const int* __restrict__ owner,
-                                                          const double* __restrict__ coords,
-                                                          const double* __restrict__ vol, int64_t G,
-                                                          const int* __restrict__ atm, const double* __restrict__ env,
-                                                          T* __restrict__ w, double* __restrict__ w64) {
-  const int64_t pt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= G) return;
-  const BatchMol Mo = mols[ptmol[pt]];
-  const int a0 = (int)Mo.atm0, na = (int)Mo.natm;
-  const T x = (T)coords[3 * pt], y = (T)coords[3 * pt + 1], z = (T)coords[3 * pt + 2];
-  const int own = owner[pt];
+__device__ inline T becke_f3(T g) {
+  g = (T(3) - g * g) * g * T(0.5);
+  g = (T(3) - g * g) * g * T(0.5);
+  return (T(3) - g * g) * g * T(0.5);
+}
+
+// Weight of one point, every atom pair evaluated from scratch (blocks whose
+// points straddle two molecules, or molecules above BECKE_MAXA atoms).
+template <typename T>
+__device__ inline double becke_point(T x, T y, T z, int a0, int na, int own, const int* __restrict__ atm,
+                                     const double* __restrict__ env) {
   T psum = T(0), pown = T(0);
   for (int i = a0; i < a0 + na; ++i) {
     const double* Ri = env + atm[i * ATM_SLOTS + 1];
@@ -558,13 +557,97 @@ __global__ void __launch_bounds__(256) becke_batch_kernel(const BatchMol* __rest
       aij = aij < T(-0.5) ? T(-0.5) : (aij > T(0.5) ? T(0.5) : aij);
       T gg = (ri - rj) / sqrt(ex * ex + ey * ey + ez * ez);
       gg = gg + aij * (T(1) - gg * gg);
-      gg = (T(3) - gg * gg) * gg * T(0.5);
-      gg = (T(3) - gg * gg) * gg * T(0.5);
-      gg = (T(3) - gg * gg) * gg * T(0.5);
-      pi *= T(0.5) * (T(1) - gg);
+      pi *= T(0.5) * (T(1) - becke_f3(gg));
     }
     psum += pi;
     if (i == own) pown = pi;
+  }
+  return (double)(pown / psum);
+}
+
+// Becke weights of all points (build_grid.py:34-100).  A workgroup takes 256
+// consecutive points; when they belong to one molecule of at most BECKE_MAXA
+// atoms (the common case) the pair constants 1/R_ij and a_ij are tabulated in
+// LDS once, every point keeps its atom distances in registers, and each
+// unordered pair is evaluated once: nu_ji = -nu_ij, so s_ji = 1 - s_ij.
+constexpr int BECKE_MAXA = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(256) becke_batch_kernel(const BatchMol* __restrict__ mols,
+                                                          const int* __restrict__ ptmol,
+                                                          const int* __restrict__ owner,
+                                                          const double* __restrict__ coords,
+                                                          const double* __restrict__ vol, int64_t G,
+                                                          const int* __restrict__ atm, const double* __restrict__ env,
+                                                          T* __restrict__ w, double* __restrict__ w64) {
+  __shared__ T s_inv[BECKE_MAXA * BECKE_MAXA], s_a[BECKE_MAXA * BECKE_MAXA];
+  const int64_t blk0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t last = (blk0 + blockDim.x < G ? blk0 + blockDim.x : G) - 1;
+  const int m0 = ptmol[blk0];
+  const BatchMol Mo = mols[m0];
+  const int a0 = (int)Mo.atm0, na = (int)Mo.natm;
+  const bool fast = ptmol[last] == m0 && na <= BECKE_MAXA;  // uniform over the workgroup
+  const int64_t pt = blk0 + threadIdx.x;
+  if (!fast) {
+    if (pt >= G) return;
+    const BatchMol Mp = mols[ptmol[pt]];
+    const double r = becke_point<T>((T)coords[3 * pt], (T)coords[3 * pt + 1], (T)coords[3 * pt + 2], (int)Mp.atm0,
+                                    (int)Mp.natm, owner[pt], atm, env);
+    const double wt = vol[pt] * r;
+    w[pt] = (T)wt;
+    if (w64) w64[pt] = wt;
+    return;
+  }
+  for (int q = threadIdx.x; q < na * na; q += blockDim.x) {
+    const int i = q / na, j = q - i * na;
+    T inv = T(0), aij = T(0);
+    if (i != j) {
+      const double* Ri = env + atm[(a0 + i) * ATM_SLOTS + 1];
+      const double* Rj = env + atm[(a0 + j) * ATM_SLOTS + 1];
+      const T ex = (T)(Ri[0] - Rj[0]), ey = (T)(Ri[1] - Rj[1]), ez = (T)(Ri[2] - Rj[2]);
+      inv = T(1) / sqrt(ex * ex + ey * ey + ez * ez);
+      const T radi = sqrt((T)c_bragg[atm[(a0 + i) * ATM_SLOTS]]), radj = sqrt((T)c_bragg[atm[(a0 + j) * ATM_SLOTS]]);
+      aij = T(0.25) * (radj / radi - radi / radj);
+      aij = aij < T(-0.5) ? T(-0.5) : (aij > T(0.5) ? T(0.5) : aij);
+    }
+    s_inv[i * BECKE_MAXA + j] = inv;
+    s_a[i * BECKE_MAXA + j] = aij;
+  }
+  __syncthreads();
+  if (pt >= G) return;
+  const T x = (T)coords[3 * pt], y = (T)coords[3 * pt + 1], z = (T)coords[3 * pt + 2];
+  T r[BECKE_MAXA], P[BECKE_MAXA];
+#pragma unroll
+  for (int i = 0; i < BECKE_MAXA; ++i) {
+    P[i] = T(1);
+    r[i] = T(0);
+    if (i < na) {
+      const double* Ri = env + atm[(a0 + i) * ATM_SLOTS + 1];
+      const T dx = x - (T)Ri[0], dy = y - (T)Ri[1], dz = z - (T)Ri[2];
+      r[i] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  // (a uniform branch per i rather than a cut-short loop, so both loops stay
+  // unrolled and r, P in registers)
+#pragma unroll
+  for (int i = 1; i < BECKE_MAXA; ++i) {
+    if (i < na) {
+#pragma unroll
+      for (int j = 0; j < i; ++j) {
+        T nu = (r[i] - r[j]) * s_inv[i * BECKE_MAXA + j];
+        nu = nu + s_a[i * BECKE_MAXA + j] * (T(1) - nu * nu);
+        const T f = becke_f3(nu);
+        P[i] *= T(0.5) * (T(1) - f);
+        P[j] *= T(0.5) * (T(1) + f);
+      }
+    }
+  }
+  const int own = owner[pt] - a0;
+  T psum = T(0), pown = T(0);
+#pragma unroll
+  for (int i = 0; i < BECKE_MAXA; ++i) {
+    if (i < na) psum += P[i];
+    if (i == own) pown = P[i];
   }
   const double wt = vol[pt] * (double)(pown / psum);
   w[pt] = (T)wt;
