@@ -145,6 +145,24 @@ int pscf_scf_step(int dtype, int iteration, int N, int nocc, const double* d_Hco
 int pscf_energy(int dtype, int N, const double* d_D, const double* d_Hcore, const void* d_dJK,
                 const double* d_exc, double e_nuc, double* d_out, void* stream);
 
+/* pscf_rks_run: the plain form's whole SCF loop for one molecule (_nanoDFT,
+ * nanoDFT.py:141-160 over nanoDFT_iteration :40-102), issued from C++:
+ * V_xc/E_xc, dJK and energy of the initial density d_D, then up to max_its
+ * times pscf_scf_step, pscf_xc, pscf_jk_s8 and pscf_energy on `stream`,
+ * stopping when |E_i - E_(i-1)| < conv_tol (a negative conv_tol runs all
+ * iterations, like the reference).  d_Dt is the density in dtype (FP32 runs;
+ * ignored in FP64), d_ao the (4, N, g) AO block, d_state zero-filled
+ * (pscf_scf_state_bytes).  h_pinned: 5 + N doubles of pinned host memory for
+ * the per-iteration read-back; h_log receives max_its rows of
+ * [E, E_core, E_JK, E_xc, E_nuc, |e_LUMO - e_HOMO|]; *h_iterations the rows
+ * written, *h_converged 1 if the stop criterion was met. */
+int pscf_rks_run(int dtype, int N, int nocc, int g, int max_its, double conv_tol, int use_diis, const void* d_ao,
+                 const void* d_weights, const void* d_eri, const double* d_Hcore, const double* d_S,
+                 const double* d_Linv, double* d_D, void* d_Dt, void* d_dJK, void* d_Vxc, double* d_exc, double* d_H,
+                 double* d_mo_energy, double* d_C, void* d_state, double* d_E, void* d_ws_xc, size_t ws_xc_bytes,
+                 void* d_ws_jk, size_t ws_jk_bytes, double e_nuc, double* h_pinned, double* h_log, int* h_iterations,
+                 int* h_converged, void* stream);
+
 /* --------------------------------------------------------- batched form ----
  * Many molecules of uneven size per launch.  d_mols is an int64 [nmol][16]
  * descriptor array (natm, nbas, nao, nocc, atm0, bas0, ns, np, shl0, ngrid,

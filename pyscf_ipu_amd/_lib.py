@@ -47,6 +47,10 @@ _SIGS = {
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
     "pscf_batch_int1e": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
+    "pscf_rks_run": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p,
+                             c_size_t, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pscf_batch_eri_scale": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "pscf_batch_jk_s8": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double,
                                  c_void_p, c_int, c_int, c_int, c_void_p]),

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpyscf_mi355x.so")
 BUILD_DIR = os.path.join(os.path.dirname(HERE), "_build")
-SOURCES = ["runtime.hip", "eri.hip", "jk.hip", "xc.hip", "scf.hip", "mlp.hip", "gemm.hip", "large.hip"]
+SOURCES = ["runtime.hip", "eri.hip", "jk.hip", "xc.hip", "scf.hip", "mlp.hip", "gemm.hip", "large.hip",
+           "rks.hip"]
 ARCH = os.environ.get("PSCF_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wno-unused-command-line-argument"]

@@ -20,6 +20,7 @@ to stop at |dE| < conv_tol (the reference always runs `its` iterations).
 from __future__ import annotations
 
 import argparse
+import ctypes
 import time
 from collections import namedtuple
 from dataclasses import dataclass, field
@@ -146,6 +147,10 @@ class RKS:
     dtype "fp64": every kernel in FP64.  dtype "fp32": ERIs, J/K and XC in FP32;
     SCF step (DIIS, eigh, density) and the energy reduction in FP64."""
 
+    # the library runs the whole loop of this class's own launches (subclasses
+    # that replace a phase set this False)
+    NATIVE_LOOP = True
+
     def __init__(self, mol: Mole, level: int = 0, dtype: str = "fp64", grid: Optional[grids.Grid] = None,
                  events: Optional["KernelEvents"] = None):
         self.mol = mol
@@ -258,11 +263,35 @@ class RKS:
                   device.ptr(self.H), device.ptr(self.mo_e), device.ptr(self.C), device.ptr(self.state), int(diis),
                   device.stream())
 
+    def _run_native(self, its, conv_tol, diis, t0, t1) -> SCFResult:
+        N = self.N
+        h_buf = _pinned(5 + N)
+        log = np.zeros((max(its, 1), 6))
+        n_it, conv = ctypes.c_int(0), ctypes.c_int(0)
+        P = device.ptr
+        _lib.call("pscf_rks_run", self.code, N, self.nocc, self.g, its, float(conv_tol), int(diis), P(self.ao),
+                  P(self.weights), P(self.eri), P(self.Hcore), P(self.S), P(self.L_inv), P(self.D),
+                  P(self.Dt) if self.tdt != torch.float64 else None, P(self.dJK), P(self.Vxc), P(self.exc),
+                  P(self.H), P(self.mo_e), P(self.C), P(self.state), P(self.E), P(self.ws_xc), self.ws_xc_b,
+                  P(self.ws_jk), self.ws_jk_b, float(self.E_nuc), ctypes.c_void_p(h_buf.data_ptr()),
+                  log.ctypes.data_as(ctypes.c_void_p), ctypes.addressof(n_it), ctypes.addressof(conv), device.stream())
+        n = n_it.value
+        torch.cuda.current_stream().synchronize()
+        t2 = time.perf_counter()
+        return SCFResult(float(log[n - 1, 0]) if n else float(h_buf[0]), log[:n, :5].copy(), log[:n, 5].copy(),
+                         self.mo_e.cpu().numpy(), self.C.cpu().numpy(), self.D.cpu().numpy(), n, bool(conv.value),
+                         {"setup_s": t1 - t0, "scf_s": t2 - t1})
+
     def run(self, its: int = 20, conv_tol: float = 1e-8, diis: bool = True, dm0=None) -> SCFResult:
+        """SCF to |E_i - E_(i-1)| < conv_tol or `its` iterations.  The loop runs
+        in the library (pscf_rks_run) unless per-kernel events are requested,
+        in which case it is issued from here, one launch group at a time."""
         t0 = time.perf_counter()
         if not self._setup_done:
             self.setup(dm0)
         t1 = time.perf_counter()
+        if self.events is None and self.NATIVE_LOOP:
+            return self._run_native(its, conv_tol, diis, t0, t1)
         self.fock_parts()
         self.energy_terms()
         E_prev = float(self.E[0].item())
@@ -285,6 +314,10 @@ class RKS:
         log = np.array(log)
         return SCFResult(float(log[-1, 0]), log, np.array(gaps), self.mo_e.cpu().numpy(), self.C.cpu().numpy(),
                          self.D.cpu().numpy(), len(log), converged, {"setup_s": t1 - t0, "scf_s": t2 - t1})
+
+
+def _pinned(n: int) -> torch.Tensor:
+    return torch.empty(n, dtype=torch.float64, pin_memory=True)
 
 
 def run_molecules(mols, streams: int = 4, level: int = 0, dtype: str = "fp64", its: int = 20,

@@ -47,6 +47,8 @@ class LargeRKS(RKS):
     """RKS for large nao.  `gemm` = "<input>/<accumulation>" of the AO-grid
     GEMMs, `eri` = "fp32" or "fp64" (ERI generation and s1 storage)."""
 
+    NATIVE_LOOP = False  # its set-up, Fock phases and step are its own
+
     def __init__(self, mol, level: int = 1, gemm: str = "f64/f64", eri: str = "fp64", grid=None, events=None):
         super().__init__(mol, level=level, dtype="fp64", grid=grid, events=events)
         tin, acc = gemm.split("/")
