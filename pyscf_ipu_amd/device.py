@@ -43,6 +43,23 @@ def to_dev(a, dtype=None) -> torch.Tensor:
     return t.contiguous()
 
 
+def upload_many(*arrays):
+    """Host arrays to the device in one copy: packed (16-byte aligned) into one
+    host buffer, copied once, returned as typed views of the device buffer."""
+    arrs = [np.ascontiguousarray(a) for a in arrays]
+    offs, total = [], 0
+    for a in arrs:
+        total = (total + 15) & ~15
+        offs.append(total)
+        total += a.nbytes
+    buf = np.zeros(max(total, 16), np.uint8)
+    for a, o in zip(arrs, offs):
+        buf[o:o + a.nbytes] = a.reshape(-1).view(np.uint8)
+    d = torch.from_numpy(buf).to(require_gpu())
+    return [d[o:o + a.nbytes].view(torch.from_numpy(a[:0].reshape(-1)).dtype).reshape(a.shape)
+            for a, o in zip(arrs, offs)]
+
+
 def empty(shape, dtype) -> torch.Tensor:
     return torch.empty(shape, dtype=dtype, device=require_gpu())
 

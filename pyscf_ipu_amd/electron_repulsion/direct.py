@@ -183,14 +183,13 @@ def shell_pairs_by_class(mol):
     """Shell pairs (i, j) of all i >= j, grouped ss | ps (p shell first) | pp.
 
     Returns (pairs (npair, 2) int32, counts [n_ss, n_ps, n_pp])."""
-    l = mol._bas[:, ANG_OF]
-    groups = ([], [], [])
-    for i in range(len(l)):
-        for j in range(i + 1):
-            a, b = (i, j) if l[i] >= l[j] else (j, i)
-            groups[int(l[i] + l[j])].append((a, b))
-    pairs = np.array(groups[0] + groups[1] + groups[2], dtype=np.int32).reshape(-1, 2)
-    return pairs, [len(g) for g in groups]
+    l = np.asarray(mol._bas[:, ANG_OF])
+    i, j = np.tril_indices(len(l))            # (i, j), i >= j, in row order
+    first = l[i] >= l[j]
+    ab = np.stack([np.where(first, i, j), np.where(first, j, i)], axis=1).astype(np.int32)
+    cls = l[i] + l[j]
+    order = np.argsort(cls, kind="stable")   # ss, then ps, then pp, each in row order
+    return np.ascontiguousarray(ab[order]), [int(np.sum(cls == c)) for c in range(3)]
 
 
 def s8_size(nao: int) -> int:
@@ -198,17 +197,21 @@ def s8_size(nao: int) -> int:
     return P * (P + 1) // 2
 
 
-def eri_s8(mol, dtype: str = "fp64", out: torch.Tensor = None, dev_mol=None) -> torch.Tensor:
-    """All unique ERIs in PySCF's aosym='s8' packing, on the device."""
+def eri_s8(mol, dtype: str = "fp64", out: torch.Tensor = None, dev_mol=None, dev_pairs=None) -> torch.Tensor:
+    """All unique ERIs in PySCF's aosym='s8' packing, on the device.
+    `dev_pairs` = (device pairs, counts) of shell_pairs_by_class, if uploaded."""
     tdt, code, _ = device.DTYPES[dtype]
     if any(mol._bas[:, ANG_OF] > 1):
         raise _lib.HIPError("eri_s8: only s and p shells are supported")
     atm, bas, env = dev_mol if dev_mol is not None else _mol_device(mol)
-    pairs, counts = shell_pairs_by_class(mol)
     nao = mol.nao_nr()
     if out is None:
         out = device.empty(s8_size(nao), tdt)
-    dp = device.to_dev(pairs)
+    if dev_pairs is None:
+        pairs, counts = shell_pairs_by_class(mol)
+        dp = device.to_dev(pairs)
+    else:
+        dp, counts = dev_pairs
     cc = (ctypes.c_int * 3)(*counts)
     _lib.call("pscf_eri_s8", code, device.ptr(atm), len(mol._atm), device.ptr(bas), len(mol._bas), device.ptr(env),
               device.ptr(dp), cc, nao, device.ptr(out), device.stream())

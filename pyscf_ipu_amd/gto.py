@@ -229,11 +229,9 @@ class Mole:
     def energy_nuc(self) -> float:
         z = self.atom_charges().astype(np.float64)
         r = self.atom_coords()
-        e = 0.0
-        for i in range(self.natm):
-            for j in range(i):
-                e += z[i] * z[j] / np.linalg.norm(r[i] - r[j])
-        return float(e)
+        i, j = np.tril_indices(self.natm, -1)
+        d = np.sqrt(((r[i] - r[j]) ** 2).sum(axis=1))
+        return float(np.sum(z[i] * z[j] / d))
 
     def shell_arrays(self):
         """Per-shell (centre, l, exps, coefs incl. the angular factor) lists."""

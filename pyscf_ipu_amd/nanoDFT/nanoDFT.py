@@ -168,9 +168,24 @@ class RKS:
         dev = device.require_gpu()
         self.N = N = mol.nao_nr()
         self.nocc = mol.nelectron // 2
-        atm, bas, env = direct._mol_device(mol)
-        self.dev_mol = (atm, bas, env)
         f64 = torch.float64
+        # every host input in one upload (molecule, descriptor, grid, shell
+        # pair lists): each separate copy from pageable memory costs ~0.1 ms
+        g = self.grid if self.grid is not None else grids.atomic_grids(mol, self.level)
+        self.g = g.size
+        pairs, counts = direct.shell_pairs_by_class(mol)
+        desc = np.zeros((1, 16), np.int64)
+        desc[0, :4] = (mol.natm, mol.nbas, N, self.nocc)
+        host = [mol._atm.astype(np.int32), mol._bas.astype(np.int32), mol._env.astype(np.float64), desc,
+                np.asarray(g.coords, np.float64), pairs, mol.ao_loc_nr()[:-1].astype(np.int32)]
+        if g.weights is None:
+            host += [g.owner.astype(np.int32), np.asarray(g.vol, np.float64),
+                     np.asarray(mol.atom_coords(), np.float64), grids.becke_adjust_table(mol.atom_charges())]
+        else:
+            host += [np.asarray(g.weights, np.float64)]
+        up = device.upload_many(*host)
+        atm, bas, env, self.d_desc, self.coords, d_pairs, shell_ao = up[:7]
+        self.dev_mol = (atm, bas, env)
         S, T, V = device.empty((N, N), f64), device.empty((N, N), f64), device.empty((N, N), f64)
         _lib.call("pscf_int1e", device.ptr(atm), mol.natm, device.ptr(bas), mol.nbas, device.ptr(env), N,
                   device.ptr(S), device.ptr(T), device.ptr(V), device.stream())
@@ -178,27 +193,19 @@ class RKS:
         self.Hcore = (T + V).contiguous()
         # overlap Cholesky -> L^-1 and the atomic-density guess on the device
         # (the batched kernels over a one-molecule descriptor)
-        desc = np.zeros((1, 16), np.int64)
-        desc[0, :4] = (mol.natm, mol.nbas, N, self.nocc)
-        self.d_desc = device.to_dev(desc)
         ctl = device.zeros((1, 8), f64)
         self.L_inv = device.empty((N, N), f64)
         _lib.call("pscf_batch_chol", device.ptr(self.d_desc), 1, N, device.ptr(S), device.ptr(self.L_inv),
                   device.ptr(ctl), device.stream())
         self._ctl1 = ctl
         # grid: atom-centred templates on the host, Becke weights on the device
-        g = self.grid if self.grid is not None else grids.atomic_grids(mol, self.level)
-        self.g = g.size
-        self.coords = device.to_dev(g.coords)
         if g.weights is None:
-            owner, vol = device.to_dev(g.owner.astype(np.int32)), device.to_dev(g.vol)
-            axyz = device.to_dev(mol.atom_coords())
-            adj = device.to_dev(grids.becke_adjust_table(mol.atom_charges()))
+            owner, vol, axyz, adj = up[7:11]
             w64 = device.empty(g.size, f64)
             _lib.call("pscf_becke_weights", device.ptr(self.coords), device.ptr(owner), device.ptr(vol), g.size,
                       device.ptr(axyz), device.ptr(adj), mol.natm, device.ptr(w64), device.stream())
         else:
-            w64 = device.to_dev(g.weights)
+            w64 = up[7]
         self.weights64 = w64
         self.weights = w64.to(self.tdt)
         self.ao = device.empty((4, N, g.size), self.tdt)
@@ -206,11 +213,10 @@ class RKS:
                   device.ptr(self.coords), g.size, device.ptr(self.ao), device.stream())
         # ERIs, packed s8
         with _timed(self.events, "eri_s8"):
-            self.eri = direct.eri_s8(mol, self.dtype, dev_mol=self.dev_mol)
+            self.eri = direct.eri_s8(mol, self.dtype, dev_mol=self.dev_mol, dev_pairs=(d_pairs, counts))
         # work buffers
         if dm0 is None:
             self.D = device.empty((N, N), f64)
-            shell_ao = device.to_dev(mol.ao_loc_nr()[:-1].astype(np.int32))
             _lib.call("pscf_batch_guess", _lib.F64, device.ptr(self.d_desc), 1, device.ptr(atm), device.ptr(bas),
                       device.ptr(shell_ao), device.ptr(S), device.ptr(self.D), None, device.stream())
             self._shell_ao = shell_ao
