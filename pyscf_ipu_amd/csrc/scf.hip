@@ -172,6 +172,14 @@ __device__ inline void wave_jacobi(double* a, double* v, int n, int* rp, int* rq
   }
 }
 
+// 1 / sqrt(x) for x >= 1 in FP64: the hardware estimate refined by two
+// Newton steps y <- y (3 - x y^2) / 2 (each doubles the correct bits).
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+
 // Parallel cyclic Jacobi on A (LDS, N x N, leading dimension ld; on exit its
 // diagonal holds the eigenvalues), accumulating rotations into V (LDS, same
 // leading dimension, caller initialised).  Round-robin pairing gives N/2
@@ -203,9 +211,11 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
         if (k == 0) {
           p = m - 1;
           q = r;
-        } else {
-          p = (r + k) % (m - 1);
-          q = (r - k + (m - 1)) % (m - 1);
+        } else {  // (r + k) mod (m - 1), (r - k) mod (m - 1) without a division
+          p = r + k;
+          if (p >= m - 1) p -= m - 1;
+          q = r - k;
+          if (q < 0) q += m - 1;
         }
         if (p > q) {
           const int t = p;
@@ -216,10 +226,12 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
         if (q < N) {
           const double apq = A[p * ld + q];
           const double app = A[p * ld + p], aqq = A[q * ld + q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app * aqq))) {
-            const double th = (aqq - app) / (2.0 * apq);
-            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
+          if (fabs(apq) > 1e-300 && apq * apq > 1e-36 * fabs(app * aqq)) {
+            // t = sign(th) / (|th| + sqrt(th^2 + 1)) with th = d / e, d = aqq - app,
+            // e = 2 apq, written with one division: t = sign(d) e / (|d| + sqrt(d^2 + e^2))
+            const double d = aqq - app, e = 2.0 * apq;
+            const double t = (d >= 0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+            c = rsqrt_nr(1.0 + t * t);
             s = t * c;
           }
         }
