@@ -92,6 +92,14 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// 1 / sqrt(x) for x >= 1 in FP64: the hardware estimate refined by two
+// Newton steps y <- y (3 - x y^2) / 2 (each doubles the correct bits).
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+
 // Cyclic Jacobi eigen-decomposition of a small symmetric n x n matrix
 // (n <= 16, LDS, row-major) by ONE wavefront: round-robin pairing gives n/2
 // disjoint rotations per round, applied as 2x2 block updates.  On exit the
@@ -117,9 +125,11 @@ __device__ inline void wave_jacobi(double* a, double* v, int n, int* rp, int* rq
         if (lane == 0) {
           p = m - 1;
           q = r;
-        } else {
-          p = (r + lane) % (m - 1);
-          q = (r - lane + (m - 1)) % (m - 1);
+        } else {  // (r + lane) mod (m - 1), (r - lane) mod (m - 1) without a division
+          p = r + lane;
+          if (p >= m - 1) p -= m - 1;
+          q = r - lane;
+          if (q < 0) q += m - 1;
         }
         if (p > q) {
           const int t = p;
@@ -129,10 +139,10 @@ __device__ inline void wave_jacobi(double* a, double* v, int n, int* rp, int* rq
         double c = 1.0, s = 0.0;
         if (q < n) {
           const double apq = a[p * n + q];
-          if (fabs(apq) >= 1e-300) {
-            const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
-            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
+          if (fabs(apq) >= 1e-300) {  // the rotation of parallel_jacobi (one division)
+            const double d = a[q * n + q] - a[p * n + p], e = 2.0 * apq;
+            const double t = (d >= 0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+            c = rsqrt_nr(1.0 + t * t);
             s = t * c;
           }
         }
@@ -170,14 +180,6 @@ __device__ inline void wave_jacobi(double* a, double* v, int n, int* rp, int* rq
       wave_sync();
     }
   }
-}
-
-// 1 / sqrt(x) for x >= 1 in FP64: the hardware estimate refined by two
-// Newton steps y <- y (3 - x y^2) / 2 (each doubles the correct bits).
-__device__ inline double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  y = y * (1.5 - 0.5 * x * y * y);
-  return y * (1.5 - 0.5 * x * y * y);
 }
 
 // Parallel cyclic Jacobi on A (LDS, N x N, leading dimension ld; on exit its
@@ -407,13 +409,30 @@ __device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, c
 #pragma unroll
     for (int k = 0; k < DIIS_N; ++k) part[k] = 0.0;
     double* err = Vh + (size_t)head * NN;
-    for (int x = tid; x < NN; x += blockDim.x) {
-      const int i = x / N, j = x - i * N;
-      const double e = A[i * ld + j] - A[j * ld + i];
-      err[x] = e;
+    // two elements per pass, all history loads issued before the error
+    // vector's store (the store may alias those loads in the compiler's view)
+    for (int x0 = tid; x0 < NN; x0 += 2 * blockDim.x) {
+      double e[2], h[2][DIIS_N];
 #pragma unroll
-      for (int k = 0; k < DIIS_N; ++k)
-        if (k < nvalid) part[k] += (k == head ? e : Vh[(size_t)k * NN + x]) * e;
+      for (int u = 0; u < 2; ++u) {
+        const int x = x0 + u * blockDim.x;
+        e[u] = 0.0;
+        if (x < NN) {
+          const int i = x / N, j = x - i * N;
+          e[u] = A[i * ld + j] - A[j * ld + i];
+        }
+#pragma unroll
+        for (int k = 0; k < DIIS_N; ++k)
+          h[u][k] = (x < NN && k < nvalid && k != head) ? Vh[(size_t)k * NN + x] : e[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+#pragma unroll
+        for (int k = 0; k < DIIS_N; ++k)
+          if (k < nvalid) part[k] += h[u][k] * e[u];
+        const int x = x0 + u * blockDim.x;
+        if (x < NN) err[x] = e[u];
+      }
     }
 #pragma unroll
     for (int k = 0; k < DIIS_N; ++k) {
