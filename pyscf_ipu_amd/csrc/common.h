@@ -64,6 +64,14 @@ __device__ inline double dexp<double>(double x) { return exp(x); }
 template <>
 __device__ inline float dexp<float>(float x) { return expf(x); }
 
+// 1 / x: the hardware reciprocal in FP32 (1 ulp), a division in FP64
+template <typename T>
+__device__ inline T drcp(T x);
+template <>
+__device__ inline double drcp<double>(double x) { return 1.0 / x; }
+template <>
+__device__ inline float drcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename T>
 __device__ inline T dsqrt(T x);
 template <>

@@ -125,13 +125,19 @@ __device__ constexpr int cexp(int f, int d) {
 // ------------------------------------------------------- shell-quartet ERIs ---
 // acc[((fi*NJ + fj)*NK + fk)*NL + fl] += (ij|kl), contracted, spherical (l <= 1).
 // T: arithmetic of the recurrence/accumulation; BT: Boys-table precision.
-template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL>
-__device__ inline void shell_quartet(const ShellD& A, const ShellD& B, const ShellD& C, const ShellD& D,
-                                     const BT* __restrict__ tab, T* acc) {
+// NPC > 0: every shell has NPC primitives (STO-3G: 3), so the primitive loops
+// unroll and the ket-pair data (exponent sum, its reciprocal, the Gaussian
+// product prefactor with its exp) is computed once per contracted quartet
+// instead of once per bra primitive pair; NPC = 0 reads the counts at run time.
+template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL, int NPC>
+__device__ inline void shell_quartet_np(const ShellD& A, const ShellD& B, const ShellD& C, const ShellD& D,
+                                        const BT* __restrict__ tab, T* acc) {
   constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
   constexpr int NMAX = LI + LJ, MMAX = LK + LL;
   constexpr int NI = NComp<LI>::v, NJ = NComp<LJ>::v, NK = NComp<LK>::v, NL = NComp<LL>::v;
   constexpr int NC = NI * NJ * NK * NL;
+  constexpr bool SSSS = LI + LJ + LK + LL == 0;  // value = prefactor * F_0: no root needed
+  constexpr int NKP = NPC > 0 ? NPC * NPC : 1;
   const T angc = (T)((LI ? ANG_P : ANG_S) * (LJ ? ANG_P : ANG_S) * (LK ? ANG_P : ANG_S) * (LL ? ANG_P : ANG_S));
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = T(0);
@@ -143,11 +149,32 @@ __device__ inline void shell_quartet(const ShellD& A, const ShellD& B, const She
   }
   const T rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
   const T rcd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
-  for (int pa = 0; pa < A.np; ++pa) {
+  // ket pairs, hoisted (NPC > 0)
+  T k_akl[NKP], k_r[NKP], k_al[NKP], k_kcd[NKP];
+  if constexpr (NPC > 0) {
+#pragma unroll
+    for (int pc = 0; pc < NPC; ++pc)
+#pragma unroll
+      for (int pd = 0; pd < NPC; ++pd) {
+        const int q = pc * NPC + pd;
+        const T ak = (T)C.e[pc], al = (T)D.e[pd];
+        k_akl[q] = ak + al;
+        k_r[q] = drcp<T>(k_akl[q]);
+        k_al[q] = al * k_r[q];
+        k_kcd[q] = dexp<T>(-ak * k_al[q] * rcd2) * (T)C.c[pc] * (T)D.c[pd];
+      }
+  }
+  const int npa = NPC > 0 ? NPC : A.np, npb = NPC > 0 ? NPC : B.np;
+  const int npc = NPC > 0 ? NPC : C.np, npd = NPC > 0 ? NPC : D.np;
+  // bra primitive pairs in a rolled loop (code size); ket pairs unrolled when
+  // NPC > 0, so the hoisted arrays are indexed at compile time
+#pragma unroll 1
+  for (int pa = 0; pa < npa; ++pa) {
     const T ai = (T)A.e[pa], ca = (T)A.c[pa];
-    for (int pb = 0; pb < B.np; ++pb) {
+#pragma unroll 1
+    for (int pb = 0; pb < npb; ++pb) {
       const T aj = (T)B.e[pb];
-      const T aij = ai + aj, r_aij = T(1) / aij;
+      const T aij = ai + aj, r_aij = drcp<T>(aij);
       const T kab = dexp<T>(-ai * aj * r_aij * rab2) * ca * (T)B.c[pb];
       T PA[3], P[3];
 #pragma unroll
@@ -155,23 +182,41 @@ __device__ inline void shell_quartet(const ShellD& A, const ShellD& B, const She
         PA[d] = -aj * r_aij * AB[d];  // P - A
         P[d] = (T)A.r[d] + PA[d];
       }
-      for (int pc = 0; pc < C.np; ++pc) {
-        const T ak = (T)C.e[pc], cc = (T)C.c[pc];
-        for (int pd = 0; pd < D.np; ++pd) {
-          const T al = (T)D.e[pd];
-          const T akl = ak + al, r_akl = T(1) / akl;
-          const T kcd = dexp<T>(-ak * al * r_akl * rcd2) * cc * (T)D.c[pd];
+#pragma unroll
+      for (int pc = 0; pc < npc; ++pc) {
+#pragma unroll
+        for (int pd = 0; pd < npd; ++pd) {
+          T akl, r_akl, alr, kcd;
+          if constexpr (NPC > 0) {
+            const int q = pc * NPC + pd;
+            akl = k_akl[q];
+            r_akl = k_r[q];
+            alr = k_al[q];
+            kcd = k_kcd[q];
+          } else {
+            const T ak = (T)C.e[pc], al = (T)D.e[pd];
+            akl = ak + al;
+            r_akl = drcp<T>(akl);
+            alr = al * r_akl;
+            kcd = dexp<T>(-ak * alr * rcd2) * (T)C.c[pc] * (T)D.c[pd];
+          }
           T QC[3], PQ[3];
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            QC[d] = -al * r_akl * CD[d];  // Q - C
+            QC[d] = -alr * CD[d];  // Q - C
             PQ[d] = P[d] - ((T)C.r[d] + QC[d]);
           }
-          const T s = aij + akl, r_s = T(1) / s;
+          const T r_s = drcp<T>(aij + akl);
           const T rho = aij * akl * r_s;
           const T tt = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
           // 2 pi^(5/2) / (aij akl sqrt(aij + akl)) * K_ab * K_cd * angular factors
           const T pref = T(34.986836655249725) * r_aij * r_akl * dsqrt<T>(r_s) * kab * kcd * angc;
+          if constexpr (SSSS) {
+            BT f[1];
+            boys<BT, 0, NTERMS>((BT)tt, tab, f);
+            acc[0] += pref * (T)f[0];
+            continue;
+          }
           BT f[2 * NR];
           boys<BT, 2 * NR - 1, NTERMS>((BT)tt, tab, f);
           T u[NR], w[NR];
@@ -242,6 +287,20 @@ __device__ inline void shell_quartet(const ShellD& A, const ShellD& B, const She
       }
     }
   }
+}
+
+// Dispatch on the primitive counts: in FP32 and the one-root classes ((ss|ss),
+// (ps|ss), the most numerous) four shells with 3 primitives (every STO-3G
+// shell) take the unrolled, ket-hoisted form; the multi-root classes and FP64
+// keep rolled loops (unrolled, they need more registers than a lane has).
+template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL>
+__device__ inline void shell_quartet(const ShellD& A, const ShellD& B, const ShellD& C, const ShellD& D,
+                                     const BT* __restrict__ tab, T* acc) {
+  constexpr bool ONE_ROOT = LI + LJ + LK + LL <= 1 && sizeof(T) == 4;  // FP32 (FP64 would need 2x the registers)
+  if (ONE_ROOT && A.np == 3 && B.np == 3 && C.np == 3 && D.np == 3)
+    shell_quartet_np<T, BT, NTERMS, LI, LJ, LK, LL, ONE_ROOT ? 3 : 0>(A, B, C, D, tab, acc);
+  else
+    shell_quartet_np<T, BT, NTERMS, LI, LJ, LK, LL, 0>(A, B, C, D, tab, acc);
 }
 
 template <typename BT>
