@@ -229,6 +229,55 @@ def test_scf_step_matches_oracle_iteration(hip):
         dD = device.to_dev(D)  # continue both from the oracle density
 
 
+@pytest.mark.parametrize("N", [2, 3, 9, 33, 63, 64, 65, 70])
+def test_scf_step_sizes_deterministic(hip, N):
+    """Synthetic Fock problems at the eigensolver's edge sizes (odd N pairs
+    with a dummy index; N <= 64 takes the one-barrier Jacobi, above it the
+    two-phase form): three DIIS steps equal the oracle's DIIS + eigh, and two
+    runs from identical inputs agree bit for bit."""
+    from pyscf_ipu_amd import _lib
+    rng = np.random.default_rng(100 + N)
+    X = rng.normal(size=(N, N)) * 0.1
+    S = np.eye(N) + X @ X.T / N
+    Li = np.linalg.inv(np.linalg.cholesky(S))
+    Hc = rng.normal(size=(N, N))
+    Hc = Hc + Hc.T - 4.0 * np.diag(np.arange(N, dtype=float)[::-1]) / N
+    nocc = max(1, N // 3)
+    # oracle sequence: inputs (D, dJK) and outputs (H, eigenvalues, D) per step
+    st, D, steps = rks.diis_init(N), np.zeros((N, N)), []
+    for it in range(3):
+        dJK = np.random.default_rng(7 * it + N).normal(size=(N, N)) * 0.1
+        dJK = dJK + dJK.T
+        H = rks.diis(it, Hc + dJK, D, S, st)
+        w, v = np.linalg.eigh(Li @ H @ Li.T)
+        C = Li.T @ v
+        Dn = 2 * C[:, :nocc] @ C[:, :nocc].T
+        steps.append((D, dJK, H, w, Dn))
+        D = Dn
+    runs = []
+    zero = device.to_dev(np.zeros((N, N)))
+    fixed = [device.to_dev(x) for x in (Hc, S, Li)]
+    for rep in range(2):
+        state = device.zeros(_lib.lib().pscf_scf_state_bytes(N) // 8, torch.float64)
+        Hd, Cd, mo = (device.empty((N, N), torch.float64), device.empty((N, N), torch.float64),
+                      device.empty(N, torch.float64))
+        out = []
+        for it, (Din, dJK, H, w, Dn) in enumerate(steps):
+            dD, dj = device.to_dev(Din), device.to_dev(dJK)
+            _lib.call("pscf_scf_step", _lib.F64, it, N, nocc, device.ptr(fixed[0]), device.ptr(fixed[1]),
+                      device.ptr(fixed[2]), device.ptr(dj), device.ptr(zero), device.ptr(dD), device.ptr(Hd),
+                      device.ptr(mo), device.ptr(Cd), device.ptr(state), 1, device.stream())
+            torch.cuda.synchronize()
+            out.append((_np(Hd), _np(mo), _np(dD)))
+            assert np.abs(out[-1][0] - H).max() < 1e-10, (rep, it)
+            assert np.abs(out[-1][1] - w).max() < 1e-10, (rep, it)
+            assert np.abs(out[-1][2] - Dn).max() < 1e-9, (rep, it)
+        runs.append(out)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("name", ["methane", "gen3"])
 def test_full_scf_fp64_matches_oracle(hip, name):
     from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS
@@ -280,14 +329,21 @@ def test_native_scf_loop_equals_python_loop(hip, dtype):
     the per-launch Python loop: same energies per iteration, same count."""
     from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS, KernelEvents
     # FP32: a fixed 12 iterations; its J/K sums run in LDS-atomic order, so two
-    # FP32 runs differ by ~1e-5 Ha after a dozen iterations
-    its, conv, tol = (30, 1e-9, 1e-10) if dtype == "fp64" else (12, -1.0, 1e-4)
+    # FP32 runs differ by ~1e-5 Ha after a dozen iterations.
+    # FP64: the total energy to 1e-10.  Its terms to 1e-7: the Jacobi stops at a
+    # relative off-diagonal norm of 1e-9, a discrete decision that the last-bit
+    # order of the J/K atomics can move by one sweep (measured on methane, with
+    # its degenerate t2 shells: two runs of the same loop differ by 1.5e-8 in
+    # E_core or by 5e-12), so D and each term (first order in D) move within the
+    # eigensolver's tolerance while the total, stationary in D, stays at ~1e-12.
+    its, conv, tol, tol_terms = (30, 1e-9, 1e-10, 1e-7) if dtype == "fp64" else (12, -1.0, 1e-4, 1e-4)
     for s in (METHANE, MOLS["gen3"]):
         mol = build_mol(s)
         nat = RKS(mol, level=0, dtype=dtype).run(its=its, conv_tol=conv)
         py = RKS(mol, level=0, dtype=dtype, events=KernelEvents()).run(its=its, conv_tol=conv)
         assert nat.iterations == py.iterations
         assert nat.converged == py.converged
-        assert np.abs(nat.energies - py.energies).max() < tol
-        assert np.abs(nat.hlgaps - py.hlgaps).max() < tol
-        assert np.abs(nat.dm - py.dm).max() < 1e3 * tol
+        assert np.abs(nat.energies[:, 0] - py.energies[:, 0]).max() < tol
+        assert np.abs(nat.energies - py.energies).max() < tol_terms
+        assert np.abs(nat.hlgaps - py.hlgaps).max() < tol_terms
+        assert np.abs(nat.dm - py.dm).max() < 1e2 * tol_terms
