@@ -182,14 +182,154 @@ __device__ inline void wave_jacobi(double* a, double* v, int n, int* rp, int* rq
   }
 }
 
+// Pair k of round r of the round-robin over m (even) players, p < q: pair 0 is
+// (r, m - 1), pair k > 0 is {(r + k) mod (m - 1), (r - k) mod (m - 1)}.
+__device__ inline void rr_pair(int r, int k, int m, int& p, int& q) {
+  const int M1 = m - 1;
+  if (k == 0) {
+    p = r;
+    q = M1;
+    return;
+  }
+  int a = r + k, b = r - k;
+  if (a >= M1) a -= M1;
+  if (b < 0) b += M1;
+  p = a < b ? a : b;
+  q = a < b ? b : a;
+}
+
+// Rotation (c, s) of pair k of round r from the current A (index N is the
+// dummy of odd N: identity).  t = sign(th) / (|th| + sqrt(th^2 + 1)) with
+// th = d / e, d = aqq - app, e = 2 apq, written with one division:
+// t = sign(d) e / (|d| + sqrt(d^2 + e^2)); entries below 1e-18 of the
+// diagonal scale are left alone.
+__device__ inline void jacobi_pair_rotation(const double* A, int ld, int N, int m, int r, int k, double* rc,
+                                            double* rs) {
+  int p, q;
+  rr_pair(r, k, m, p, q);
+  double c = 1.0, s = 0.0;
+  if (q < N) {
+    const double apq = A[p * ld + q];
+    const double app = A[p * ld + p], aqq = A[q * ld + q];
+    if (fabs(apq) > 1e-300 && apq * apq > 1e-36 * fabs(app * aqq)) {
+      const double d = aqq - app, e = 2.0 * apq;
+      const double t = (d >= 0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+      c = rsqrt_nr(1.0 + t * t);
+      s = t * c;
+    }
+  }
+  rc[k] = c;
+  rs[k] = s;
+}
+
+// One V column rotation of round r: row i of columns (p, q) of pair lp.
+struct VItem {
+  int a, b;  // LDS indices of V[i][p], V[i][q]
+  bool on;
+  double vp, vq, c, s;
+};
+
+__device__ inline VItem v_item_load(const double* V, const double* rc, const double* rs, int N, int ld, int m,
+                                    int half, int r, int lp, int i) {
+  VItem it;
+  int p, q;
+  rr_pair(r, lp < half ? lp : 0, m, p, q);
+  it.on = lp < half && i < N && q < N;
+  it.a = it.on ? i * ld + p : 0;
+  it.b = it.on ? i * ld + q : 0;
+  it.vp = V[it.a];
+  it.vq = V[it.b];
+  it.c = rc[lp < half ? lp : 0];
+  it.s = rs[lp < half ? lp : 0];
+  return it;
+}
+
+__device__ inline void v_item_store(double* V, const VItem& it) {
+  if (it.on) {
+    V[it.a] = it.c * it.vp - it.s * it.vq;
+    V[it.b] = it.s * it.vp + it.c * it.vq;
+  }
+}
+
+// Parallel cyclic Jacobi for N <= 64 by exactly 1024 threads, two barriers a
+// round with the rotations off the critical path.  Round r:
+//   phase 1: thread (k, l) = (tid / 32, tid % 32) updates the 2x2 block of A
+//            (pair k rows, pair l columns); every wave also rotates the V
+//            columns of pair w = tid / 64 (lane = row), wave 15 also pair 31;
+//   phase 2: wave 15 computes round r + 1's rotations from the updated A while
+//            waves 0..14 rotate the V columns of pairs 16..30 (V never feeds A).
+// Rotation tables rc, rs hold two rounds (32 pairs each, by round parity).
+// Pair indices come from the round-robin formula.  Every thread issues its
+// loads before its stores (A blocks and V items are disjoint), so the
+// arithmetic and the result are those of the two-phase form below.
+__device__ inline int jacobi_pipelined(double* A, double* V, int N, int ld, double* rc, double* rs, double* red) {
+  const int m = N + (N & 1), half = m / 2;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int k = tid >> 5, l = tid & 31;
+  int sweep = 0;
+  for (; sweep < MAX_SWEEPS; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      const double v = A[i * ld + j] * A[i * ld + j];
+      tot += v;
+      if (i != j) off += v;
+    }
+    off = block_sum(off, red);
+    tot = block_sum(tot, red);
+    if (off <= 1e-18 * tot) break;
+    if (tid < half) jacobi_pair_rotation(A, ld, N, m, 0, tid, rc, rs);
+    __syncthreads();
+    for (int r = 0; r < m - 1; ++r) {
+      const int cur = (r & 1) * 32, nxt = 32 - cur;
+      {  // phase 1
+        const bool blk = k < half && l < half;
+        int pk, qk, pl, ql;
+        rr_pair(r, blk ? k : 0, m, pk, qk);
+        rr_pair(r, blk ? l : 0, m, pl, ql);
+        const bool qkv = blk && qk < N, qlv = blk && ql < N;
+        const int rq_ = qkv ? qk : pk, cq_ = qlv ? ql : pl;
+        const int i00 = pk * ld + pl, i01 = pk * ld + cq_, i10 = rq_ * ld + pl, i11 = rq_ * ld + cq_;
+        const double x00 = A[i00], x01r = A[i01], x10r = A[i10], x11r = A[i11];
+        const double ck = rc[cur + k], sk = rs[cur + k], cl = rc[cur + l], sl = rs[cur + l];
+        const VItem v1 = v_item_load(V, rc + cur, rs + cur, N, ld, m, half, r, w, lane);
+        const VItem v2 = v_item_load(V, rc + cur, rs + cur, N, ld, m, half, r, w == 15 ? 31 : 32, lane);
+        const double x01 = qlv ? x01r : 0.0, x10 = qkv ? x10r : 0.0, x11 = (qkv && qlv) ? x11r : 0.0;
+        // columns: [x.0, x.1] R_l; rows: R_k^T [t0.; t1.]
+        const double t00 = cl * x00 - sl * x01, t01 = sl * x00 + cl * x01;
+        const double t10 = cl * x10 - sl * x11, t11 = sl * x10 + cl * x11;
+        if (blk) {
+          A[i00] = ck * t00 - sk * t10;
+          if (qlv) A[i01] = ck * t01 - sk * t11;
+          if (qkv) A[i10] = sk * t00 + ck * t10;
+          if (qkv && qlv) A[i11] = sk * t01 + ck * t11;
+        }
+        v_item_store(V, v1);
+        v_item_store(V, v2);
+      }
+      __syncthreads();
+      if (w == 15) {  // phase 2
+        if (r + 1 < m - 1 && lane < half) jacobi_pair_rotation(A, ld, N, m, r + 1, lane, rc + nxt, rs + nxt);
+      } else {
+        const VItem v3 = v_item_load(V, rc + cur, rs + cur, N, ld, m, half, r, 16 + w, lane);
+        v_item_store(V, v3);
+      }
+      __syncthreads();
+    }
+  }
+  return sweep;
+}
+
 // Parallel cyclic Jacobi on A (LDS, N x N, leading dimension ld; on exit its
 // diagonal holds the eigenvalues), accumulating rotations into V (LDS, same
 // leading dimension, caller initialised).  Round-robin pairing gives N/2
 // disjoint rotations per round; the two-sided update A <- J^T A J is one pass
 // over the 2x2 blocks (pair k rows, pair l columns): Y = R_k^T X R_l with
-// R = [[c, s], [-s, c]].  Returns the number of sweeps.
+// R = [[c, s], [-s, c]].  N <= 64 with 1024 threads takes jacobi_pipelined,
+// which needs rc and rs of 64 entries.  Returns the number of sweeps.
 __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* rp, int* rq, double* rc, double* rs,
                                       double* red) {
+  if (N <= 64 && blockDim.x == 1024) return jacobi_pipelined(A, V, N, ld, rc, rs, red);
   const int m = N + (N & 1);  // even number of players; index N is a dummy
   const int half = m / 2;
   int sweep = 0;
@@ -289,8 +429,9 @@ struct ScfLds {
 
 constexpr int SCF_RED = (SCF_THREADS / 64) * DIIS_N;
 
-// rotation tables hold N/2 + 1 pairs, and at least 8 for the 10 x 10 DIIS system
-__host__ __device__ inline int scf_pairs(int N) { return N / 2 + 1 > 8 ? N / 2 + 1 : 8; }
+// rotation tables hold N/2 + 1 pairs, at least 8 for the 10 x 10 DIIS system,
+// and two rounds of 32 (jacobi_pipelined) for N <= 64
+__host__ __device__ inline int scf_pairs(int N) { return N <= 64 ? 64 : N / 2 + 1; }
 
 __host__ __device__ inline size_t scf_lds_bytes(int N) {
   const int h = scf_pairs(N), ld = lds_ld(N);
@@ -870,7 +1011,11 @@ __global__ void __launch_bounds__(256) energy_batch_kernel(const BatchMol* __res
   }
 }
 
-static size_t scf_lds(int N) { return scf_lds_bytes(N); }
+// LDS for every molecule of nao <= N (a batch launch carves by each molecule's nao)
+static size_t scf_lds(int N) {
+  const size_t a = scf_lds_bytes(N), b = scf_lds_bytes(N < 64 ? N : 64);
+  return a > b ? a : b;
+}
 
 }  // namespace pscf
 
