@@ -63,6 +63,17 @@ def test_oracle_features_closed_forms():
     assert mask[k].sum() == 5  # 5 AOs of C x 1 AO of H
 
 
+def test_rerun_mask_selects_unconverged_and_small_gap():
+    """Re-run from MINAO: not converged (unless failed), or converged with a
+    gap below the judge; a converged molecule at or above it is kept."""
+    conv = np.array([True, True, False, False, True, True])
+    failed = np.array([False, False, False, True, False, True])
+    gap = np.array([0.20, 0.03, 0.01, 0.30, lg.GAP_JUDGE, 0.01])
+    redo, judged = lg.rerun_mask(conv, failed, gap)
+    assert redo.tolist() == [False, True, True, False, False, False]
+    assert judged.tolist() == [False, True, False, False, False, False]
+
+
 def test_flat_layout_round_trip():
     flat = lg.init_params(1)
     parts = lg.unflatten(flat)
