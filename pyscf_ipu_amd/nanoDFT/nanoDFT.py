@@ -237,7 +237,8 @@ class RKS:
         self.ws_jk = device.workspace(self.ws_jk_b)
         self.E_nuc = mol.energy_nuc()
         self._setup_done = True
-        torch.cuda.synchronize()
+        # this stream only: molecules set up on other streams run on meanwhile
+        torch.cuda.current_stream().synchronize()
         return time.perf_counter() - t0
 
     # -- kernels of one iteration ---------------------------------------------------
@@ -315,7 +316,7 @@ class RKS:
                 converged = True
                 break
             E_prev = e[0]
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         t2 = time.perf_counter()
         log = np.array(log)
         return SCFResult(float(log[-1, 0]), log, np.array(gaps), self.mo_e.cpu().numpy(), self.C.cpu().numpy(),
@@ -332,7 +333,9 @@ def run_molecules(mols, streams: int = 4, level: int = 0, dtype: str = "fp64", i
     molecule per launch, as nanoDFT runs them), up to `streams` molecules in
     flight on their own HIP streams from as many host threads.  A molecule's
     SCF is latency-bound (its eigensolver is one workgroup), so concurrent
-    molecules fill otherwise idle CUs; 4 matches the box's hardware queues.
+    molecules fill otherwise idle CUs.  Streams beyond the process's hardware
+    queues (GPU_MAX_HW_QUEUES, HIP's default 4, read at HIP start-up) share
+    queues and run one after another.
     Results come back in input order and equal the one-at-a-time results."""
     import threading
     out = [None] * len(mols)
