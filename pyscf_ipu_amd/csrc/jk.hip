@@ -13,6 +13,7 @@
 //     fold into one accumulator M with dJK = M + M^T (6 scatters per ERI).
 #include "batch.h"
 #include "common.h"
+#include "reduce.h"
 #include "wave.h"
 
 namespace pscf {
@@ -509,6 +510,12 @@ static int jk_s8_blocks(int N) {
   return (int)(nb > JK_MAX_BLOCKS ? JK_MAX_BLOCKS : (nb < 1 ? 1 : nb));
 }
 
+// workspace: nb partial matrices, then the chunk sums of reduce_parts
+static size_t jk_s8_ws_bytes(size_t es, int N) {
+  const int nb = jk_s8_blocks(N);
+  return red_align(es * (size_t)nb * N * N) + es * (size_t)red_chunks(nb) * N * N;
+}
+
 template <typename T>
 static size_t jk_s8_lds(int N) {
   return sizeof(T) * 2 * (size_t)N * N;
@@ -521,13 +528,12 @@ static int jk_s8_impl(const T* eri, int N, const T* dm, double hyb, T* out, void
   PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_jk_s8: N=%d needs %zu B of LDS (max %d)", N, lds,
                max_lds_bytes());
   const int nb = jk_s8_blocks(N);
-  PSCF_REQUIRE(ws && ws_bytes >= sizeof(T) * (size_t)nb * N * N, "pscf_jk_s8: workspace too small");
+  PSCF_REQUIRE(ws && ws_bytes >= jk_s8_ws_bytes(sizeof(T), N), "pscf_jk_s8: workspace too small");
   PSCF_SET_LDS(jk_s8_kernel<T>, lds);
   jk_s8_kernel<T><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
   PSCF_HIP(hipGetLastError());
-  jk_reduce_kernel<T, true><<<(N * N + 3) / 4, 256, 0, st>>>((const T*)ws, nb, N, out);
-  PSCF_HIP(hipGetLastError());
-  return PSCF_OK;
+  T* lvl = (T*)((char*)ws + red_align(sizeof(T) * (size_t)nb * N * N));
+  return reduce_parts<T, true>((const T*)ws, nb, N, lvl, out, nullptr, nullptr, st);
 }
 
 }  // namespace pscf
@@ -555,8 +561,7 @@ extern "C" int pscf_jk_sparse(int dtype, const void* d_eri, const int16_t* d_idx
 }
 
 extern "C" size_t pscf_jk_s8_workspace_bytes(int dtype, int N) {
-  const size_t es = dtype == PSCF_F64 ? 8 : 4;
-  return es * (size_t)jk_s8_blocks(N) * N * N;
+  return jk_s8_ws_bytes(dtype == PSCF_F64 ? 8 : 4, N);
 }
 
 extern "C" int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_dm, double hyb, void* d_out,

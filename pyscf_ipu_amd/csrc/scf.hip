@@ -726,7 +726,7 @@ __global__ void energy_kernel(int N, const double* __restrict__ D, const double*
                               const T* __restrict__ dJK, const double* __restrict__ exc, double e_nuc,
                               double* __restrict__ out) {
   __shared__ double red[16];
-  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;  // Neumaier sums
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;  // Neumaier sums (1024 threads: a few terms each)
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
     const double a = D[x] * Hcore[x];
     double t = s1 + a;
@@ -1080,9 +1080,9 @@ extern "C" int pscf_energy(int dtype, int N, const double* d_D, const double* d_
   PSCF_REQUIRE(N > 0 && d_D && d_Hcore && d_dJK && d_exc && d_out, "pscf_energy: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   if (dtype == PSCF_F64)
-    energy_kernel<double><<<1, 256, 0, st>>>(N, d_D, d_Hcore, (const double*)d_dJK, d_exc, e_nuc, d_out);
+    energy_kernel<double><<<1, 1024, 0, st>>>(N, d_D, d_Hcore, (const double*)d_dJK, d_exc, e_nuc, d_out);
   else
-    energy_kernel<float><<<1, 256, 0, st>>>(N, d_D, d_Hcore, (const float*)d_dJK, d_exc, e_nuc, d_out);
+    energy_kernel<float><<<1, 1024, 0, st>>>(N, d_D, d_Hcore, (const float*)d_dJK, d_exc, e_nuc, d_out);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }

@@ -15,6 +15,7 @@
 #include "b3lyp.h"
 #include "batch.h"
 #include "common.h"
+#include "reduce.h"
 #include "wave.h"
 
 namespace pscf {
@@ -707,33 +708,6 @@ __global__ void __launch_bounds__(256) ao_batch_kernel(const BatchMol* __restric
   }
 }
 
-// V_xc = sum_b (part[b] + part[b]^T), one wave per element (lanes over the
-// partials, fixed shuffle tree); the last block also sums the energy partials.
-template <typename T>
-__global__ void xc_reduce_kernel(const T* __restrict__ part_v, const double* __restrict__ part_e, int nb, int N,
-                                 T* __restrict__ vxc, double* __restrict__ exc) {
-  const int NN = N * N;
-  const int x = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (x < NN) {
-    const int i = x / N, j = x - i * N;
-    T s = T(0);
-    for (int b = lane; b < nb; b += 64) s += part_v[(int64_t)b * NN + x] + part_v[(int64_t)b * NN + j * N + i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) vxc[x] = s;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s = 0.0, c = 0.0;  // Neumaier
-    for (int b = 0; b < nb; ++b) {
-      const double v = part_e[b], t = s + v;
-      c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
-      s = t;
-    }
-    *exc = s + c;
-  }
-}
-
 template <typename T, int XT, int NP>
 static size_t xc_lds(int N) {
   const size_t main = sizeof(T) * XTile<XT, NP>::lds_elems;
@@ -744,6 +718,21 @@ static size_t xc_lds(int N) {
 static int xc_blocks(int g) {
   const int ntiles = (g + XC_TILE - 1) / XC_TILE;
   return ntiles < XC_MAX_BLOCKS ? (ntiles < 1 ? 1 : ntiles) : XC_MAX_BLOCKS;
+}
+
+// workspace: nb partial V_xc matrices | nb FP64 energy partials | the chunk
+// sums of reduce_parts (each part 256-byte aligned)
+struct XcWs {
+  size_t e_off, lvl_off, bytes;
+};
+
+static XcWs xc_ws(size_t es, int N, int g) {
+  const int nb = xc_blocks(g);
+  XcWs w;
+  w.e_off = red_align(es * (size_t)nb * N * N);
+  w.lvl_off = w.e_off + red_align(sizeof(double) * (size_t)nb);
+  w.bytes = w.lvl_off + es * (size_t)red_chunks(nb) * N * N;
+  return w;
 }
 
 template <typename T, int NP>
@@ -763,8 +752,8 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
   PSCF_REQUIRE(3 * sc + (int64_t)(N - 1) * smu + (int64_t)(g - 1) * sp < ((int64_t)1 << 31),
                "pscf_xc: AO block of %lld elements needs 32-bit offsets", (long long)(4 * sc));
   const int nb = xc_blocks(g);
-  const size_t need = sizeof(T) * (size_t)nb * N * N + sizeof(double) * nb;
-  PSCF_REQUIRE(ws && ws_bytes >= need, "pscf_xc: workspace too small (%zu < %zu)", ws_bytes, need);
+  const XcWs lay = xc_ws(sizeof(T), N, g);
+  PSCF_REQUIRE(ws && ws_bytes >= lay.bytes, "pscf_xc: workspace too small (%zu < %zu)", ws_bytes, lay.bytes);
   XCArgs<T> a;
   a.ao = ao;
   a.sc = sc;
@@ -776,7 +765,7 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
   a.w = w;
   a.eps = (T)eps;
   a.part_v = (T*)ws;
-  a.part_e = (double*)((char*)ws + sizeof(T) * (size_t)nb * N * N);
+  a.part_e = (double*)((char*)ws + lay.e_off);
   int rc;
   switch (xm_np(N)) {
     case 16: rc = xc_fused_launch<T, 16>(a, nb, st); break;
@@ -786,9 +775,7 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
     default: rc = xc_fused_launch<T, 80>(a, nb, st); break;
   }
   if (rc) return rc;
-  xc_reduce_kernel<T><<<(N * N + 3) / 4, 256, 0, st>>>(a.part_v, a.part_e, nb, N, vxc, exc);
-  PSCF_HIP(hipGetLastError());
-  return PSCF_OK;
+  return reduce_parts<T, true>(a.part_v, nb, N, (T*)((char*)ws + lay.lvl_off), vxc, a.part_e, exc, st);
 }
 
 }  // namespace pscf
@@ -852,9 +839,7 @@ extern "C" int pscf_b3lyp_host(const double* h_rho, int g, double eps, double* h
 }
 
 extern "C" size_t pscf_xc_workspace_bytes(int dtype, int N, int g) {
-  const size_t es = dtype == PSCF_F64 ? 8 : 4;
-  const size_t nb = (size_t)xc_blocks(g);
-  return es * nb * N * N + sizeof(double) * nb;
+  return xc_ws(dtype == PSCF_F64 ? 8 : 4, N, g).bytes;
 }
 
 extern "C" int pscf_xc(int dtype, const void* d_ao, int64_t ao_sc, int64_t ao_smu, int64_t ao_sp, int g, int N,

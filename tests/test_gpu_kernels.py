@@ -191,6 +191,27 @@ def test_exchange_correlation_dropin(hip, mol, dtype, tol):
     assert np.abs(V - V_ref).max() < tol * max(1.0, np.abs(V_ref).max())
 
 
+@pytest.mark.parametrize("npts", [1, 64, 65, 1100, 16500])
+def test_exchange_correlation_partial_counts(hip, npts):
+    """Grids of 1..258 workgroup partials (64-point tiles): the partial V_xc
+    matrices and energies are summed exactly once each (1, 1, 2, 18 and the
+    256-workgroup cap)."""
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import exchange_correlation
+    mol = build_mol(MOLS["gen3"])
+    g = grids.build_grid(mol, 1)
+    assert g.size >= npts
+    sel = np.linspace(0, g.size - 1, npts).astype(int)
+    coords, w = g.coords[sel], g.weights[sel]
+    ao = oracle_ao.eval_ao_deriv1(mol, coords)
+    S, _, _ = cint.int1e(mol)
+    from pyscf_ipu_amd import guess
+    dm = guess.minao(mol, S)
+    E_ref, V_ref, _ = oracle_xc.exchange_correlation(dm, ao, w)
+    E, V = exchange_correlation(dm, ao, w)
+    assert abs(E - E_ref) < 1e-10 * max(1.0, abs(E_ref))
+    assert np.abs(V - V_ref).max() < 1e-10 * max(1.0, np.abs(V_ref).max())
+
+
 def test_scf_step_matches_oracle_iteration(hip):
     """Two SCF steps (DIIS + eigh + density) from the same Fock inputs."""
     from pyscf_ipu_amd import _lib
