@@ -36,7 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # (HIP's default is 4 queues per process; 8 measured best for 32 molecules of
 # nao 54-65: 378 mol/s against 346 at 4 and 315 at 16)
 PLAIN_STREAMS = 8
-os.environ.setdefault("GPU_MAX_HW_QUEUES", str(PLAIN_STREAMS))
+# read once when HIP starts (before torch is imported); raised, never lowered
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < PLAIN_STREAMS:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(PLAIN_STREAMS)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 
@@ -60,6 +62,22 @@ def _cpu_baseline(mol_strs, budget_s: float = 25.0):
     return {"value": done / dt, "unit": "molecules/s", "cores": cores, "kind": "port",
             "sample": f"{done} of the benchmark's 9-heavy-atom molecules, FP64 oracle (C Rys ERIs with OpenMP over "
                       f"{cores} threads + numpy SCF), {dt:.1f} s"}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_bench.md")
+N_CU = 256
+
+
+def _profile_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == kernel:
+            return int(cells[4].replace(",", ""))
+    return None
 
 
 def _roof(bound, achieved, peak, unit, ms, model):
@@ -251,9 +269,16 @@ def main(argv=None):
     else:
         achieved = work["flops"] / mean_s / 1e12
         peak, unit = PEAK_TFLOPS[a.dtype], "TFLOP/s"
+    kname = {"scf_step": "scf_step_kernel", "jk_s8": "jk_s8_kernel", "xc": "xc_fused_kernel"}.get(dominant)
+    tdt = "double" if a.dtype == "fp64" else "float"
+    traffic = _profile_traffic(f"{kname}<{tdt}>") if kname else None
     roof = {"kernel": dominant, "bound": work["bound"], "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None, "mean_launch_ms": ks[dominant]["mean_ms"],
-            "work_model": work["model"]}
+            "frac": achieved / peak, "traffic": traffic, "mean_launch_ms": ks[dominant]["mean_ms"],
+            "work_model": work["model"],
+            "traffic_source": "profiles/hbm_traffic_bench.md: FETCH_SIZE + WRITE_SIZE bytes per dispatch, each "
+                              "scaled by the copy-kernel calibration (8 molecules of the same generator)"}
+    if dominant == "scf_step":  # one workgroup: the CU it runs on bounds it
+        roof.update({"peak_one_cu": peak / N_CU, "frac_one_cu": achieved / (peak / N_CU)})
     kernel_rooflines = {}
     for k, v in ks.items():
         try:
