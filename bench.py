@@ -189,6 +189,42 @@ def _copy_calibration(n: int = 1 << 27, reps: int = 3) -> int:
     return 0
 
 
+def _probes(n: int = 1 << 27, reps: int = 5) -> dict:
+    """Measured machine constants beside the published peaks: HBM stream rate
+    of the library's copy kernel (1 GiB read + 1 GiB written per launch) and a
+    BF16 GEMM through torch (rocBLAS/hipBLASLt), both timed with HIP events on
+    the launch stream."""
+    import torch
+    from pyscf_ipu_amd import _lib, device
+    st = torch.cuda.current_stream()
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    copy = lambda: _lib.call("pscf_cast", _lib.F64, device.ptr(x), _lib.F64, device.ptr(y), n, device.stream())
+    copy()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for _ in range(reps):
+        copy()
+    b.record(st)
+    b.synchronize()
+    copy_gbs = 16.0 * n * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del x, y
+    m = 8192
+    A = torch.randn(m, m, device="cuda", dtype=torch.bfloat16)
+    B = torch.randn(m, m, device="cuda", dtype=torch.bfloat16)
+    C = A @ B
+    a.record(st)
+    for _ in range(reps):
+        C = A @ B
+    b.record(st)
+    b.synchronize()
+    gemm_tf = 2.0 * m ** 3 * reps / (a.elapsed_time(b) * 1e-3) / 1e12
+    del A, B, C
+    torch.cuda.empty_cache()
+    return {"hbm_copy_gbs": copy_gbs, "hbm_copy_frac_of_8000": copy_gbs / HBM_PEAK_GBS,
+            "bf16_gemm_8192_tflops": gemm_tf, "bf16_gemm_frac_of_2500": gemm_tf / 2500.0}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +349,7 @@ def main(argv=None):
         "serial_one_stream": {"value": passed / serial_wall, "ms_per_step": serial_wall * 1e3},
         "kernel_ms_per_step": {k: v["total_ms"] for k, v in ks.items()},
         "roofline": roof, "kernel_rooflines": kernel_rooflines, "cpu_baseline": cpu, "forms": forms,
+        "machine_probes": _probes(),
     }
     print(json.dumps(line), flush=True)
     return 0
