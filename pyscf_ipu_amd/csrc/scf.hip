@@ -28,13 +28,53 @@ __host__ __device__ inline size_t state_doubles(int N) {
 // walk down a column (stride ld doubles) touches distinct LDS bank pairs.
 __host__ __device__ inline int lds_ld(int N) { return N | 1; }
 
+// C = op(X) op(Y) for N <= 64 by 16 waves on FP64 MFMA (16x16x4): wave w
+// computes output tile (w / nt, w % nt), nt = ceil(N / 16).  Lane l supplies
+// A[row = l % 16][k = l / 16] and B[k = l / 16][col = l % 16] and receives
+// C[l / 16 + 4 j][l % 16], j = 0..3.  Operands outside N x N are zero (their
+// LDS reads are clamped in bounds).  Each operand element is read once per
+// tile, an eighth of the row-walk form's LDS traffic.
+template <bool TX, bool TY>
+__device__ inline void lds_matmul_mfma(const double* __restrict__ X, const double* __restrict__ Y,
+                                       double* __restrict__ C, int N, int ld, int ldc) {
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  const int nt = (N + 15) >> 4;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w >= nt * nt) return;  // wave-uniform
+  const int I = w / nt, J = w - I * nt;
+  const int li = lane & 15, lg = lane >> 4;
+  const int row = I * 16 + li, col = J * 16 + li;
+  const bool rv = row < N, cv = col < N;
+  const int rowc = rv ? row : N - 1, colc = cv ? col : N - 1;
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k0 = 0; k0 < N; k0 += 4) {
+    const int k = k0 + lg;
+    const bool kv = k < N;
+    const int kc = kv ? k : N - 1;
+    const double a = TX ? X[kc * ld + rowc] : X[rowc * ld + kc];
+    const double b = TY ? Y[colc * ld + kc] : Y[kc * ld + colc];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64((rv && kv) ? a : 0.0, (cv && kv) ? b : 0.0, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = I * 16 + lg + 4 * j;
+    if (r < N && cv) C[r * ldc + col] = acc[j];
+  }
+}
+
 // C = op(X) op(Y), N x N, X and Y in LDS (leading dimension ld), C in LDS or
-// global (leading dimension ldc), never aliasing X or Y.  Each thread owns
-// rows {i, i + h} x columns {j, j + h} (h = ceil(N / 2)), so the lanes of a
-// wave walk consecutive columns and share rows.
+// global (leading dimension ldc), never aliasing X or Y.  N <= 64 with 1024
+// threads takes the MFMA form above.  Otherwise each thread owns rows
+// {i, i + h} x columns {j, j + h} (h = ceil(N / 2)), so the lanes of a wave walk
+// consecutive columns and share rows.
 template <bool TX, bool TY>
 __device__ inline void lds_matmul(const double* __restrict__ X, const double* __restrict__ Y, double* __restrict__ C,
                                   int N, int ld, int ldc) {
+  if (N <= 64 && blockDim.x == 1024) {
+    lds_matmul_mfma<TX, TY>(X, Y, C, N, ld, ldc);
+    return;
+  }
   const int h = (N + 1) >> 1;
   for (int b = threadIdx.x; b < h * h; b += blockDim.x) {
     const int i = b / h, j = b - i * h;
