@@ -203,6 +203,17 @@ struct XTile {
   static constexpr size_t lds_elems = (size_t)4 * XT * XS + (size_t)XT * XD + (size_t)NP * XD + 256 + 4 * XT;
 };
 
+// FP32 with 32-point tiles and nao <= 64 keeps GEMM 1's D operand in registers
+// (NJ * NP / 4 values per lane: D does not change between tiles) instead of
+// LDS, which lowers the LDS of a workgroup from 67 KB to 46 KB at NP = 64, so
+// three workgroups share a CU instead of two.  FP64 and wider shapes would need
+// more registers than they have spare and keep D in LDS.
+template <typename T, int XT, int NP>
+struct XDReg {
+  static constexpr bool on = sizeof(T) == 4 && XT == 32 && NP <= 64;
+  static constexpr size_t lds_elems = XTile<XT, NP>::lds_elems - (on ? (size_t)NP * XTile<XT, NP>::XD : 0);
+};
+
 template <typename T, int XT, int NP>
 using XVacc = typename XMTraits<T>::v4[XTile<XT, NP>::NBW];
 
@@ -214,19 +225,36 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
   constexpr int PB = L::PB, NSLOT = L::NSLOT, NB = L::NB, XS = L::XS, XD = L::XD, NVB = L::NVB, NBW = L::NBW;
   constexpr int RS = XMTraits<T>::RSTEP;
   const int N = a.N;
-  T* Phi = reinterpret_cast<T*>(smem);  // [4][XT][XS]
-  T* Y = Phi + 4 * XT * XS;             // [XT][XD]
-  T* Dm = Y + XT * XD;                  // [NP][XD]
-  T* rpart = Dm + NP * XD;              // [NSLOT][4][XT]
-  T* coef = rpart + 256;                // [4][XT]
+  constexpr bool DREG = XDReg<T, XT, NP>::on;
+  constexpr int NK = NP / 4;
+  T* Phi = reinterpret_cast<T*>(smem);        // [4][XT][XS]
+  T* Y = Phi + 4 * XT * XS;                   // [XT][XD]
+  T* Dm = Y + XT * XD;                        // [NP][XD] (not when DREG)
+  T* rpart = Dm + (DREG ? 0 : NP * XD);       // [NSLOT][4][XT]
+  T* coef = rpart + 256;                      // [4][XT]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
   const int pb = wave % PB, slot = wave / PB;
+  // GEMM 1's B operand of this lane: D[k0 + lg][column block of jb, li]
+  T dreg[DREG ? L::NJ * NK : 1];
   __syncthreads();
-  for (int x = tid; x < NP * NP; x += blockDim.x) {
-    const int i = x / NP, j = x - i * NP;
-    Dm[i * XD + j] = (i < N && j < N) ? a.dm[i * N + j] : T(0);
+  if (DREG) {
+#pragma unroll
+    for (int jb = 0; jb < L::NJ; ++jb) {
+      const int cbk = slot + jb * NSLOT < NB ? slot + jb * NSLOT : NB - 1;  // as in GEMM 1
+      const int col = cbk * 16 + li;
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) {
+        const int row = kk * 4 + lg;
+        dreg[DREG ? jb * NK + kk : 0] = (row < N && col < N) ? a.dm[row * N + col] : T(0);
+      }
+    }
+  } else {
+    for (int x = tid; x < NP * NP; x += blockDim.x) {
+      const int i = x / NP, j = x - i * NP;
+      Dm[i * XD + j] = (i < N && j < N) ? a.dm[i * N + j] : T(0);
+    }
   }
   // padding columns nu in [N, NP) of Phi stay zero for every tile
   for (int x = tid; x < 4 * XT * (NP - N); x += blockDim.x) {
@@ -292,7 +320,8 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
         const T scale = on ? T(1) : T(0);
         v4 acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int k0 = 0; k0 < NP; k0 += 4) acc = XMTraits<T>::mfma(Phi[a1 + k0], Dm[b1 + k0 * XD + cb], acc);
+        for (int k0 = 0; k0 < NP; k0 += 4)
+          acc = XMTraits<T>::mfma(Phi[a1 + k0], DREG ? dreg[DREG ? jb * NK + k0 / 4 : 0] : Dm[b1 + k0 * XD + cb], acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] *= scale;
 #pragma unroll
@@ -710,7 +739,7 @@ __global__ void __launch_bounds__(256) ao_batch_kernel(const BatchMol* __restric
 
 template <typename T, int XT, int NP>
 static size_t xc_lds(int N) {
-  const size_t main = sizeof(T) * XTile<XT, NP>::lds_elems;
+  const size_t main = sizeof(T) * XDReg<T, XT, NP>::lds_elems;
   const size_t tail = sizeof(T) * (size_t)N * N + 8 * sizeof(double);  // V + energy partials after the tiles
   return (main > tail ? main : tail) + 8 * sizeof(double);
 }
