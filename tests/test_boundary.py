@@ -47,3 +47,13 @@ def test_host_b3lyp_matches_oracle_autograd():
     b = b3lyp_host(rho, 1e-20)
     for x, y in zip(a, b):
         assert np.max(np.abs(x - y) / (np.abs(x) + 1e-300)) < 1e-11
+
+
+def test_pool_refuses_after_gpu_use(monkeypatch):
+    """The pool's workers are started by fork + exec, which must not follow GPU
+    use in the starting process."""
+    import torch
+    from pyscf_ipu_amd.nanoDFT.pool import MoleculePool
+    monkeypatch.setattr(torch.cuda, "is_initialized", lambda: True)
+    with pytest.raises(RuntimeError, match="before this process uses the GPU"):
+        MoleculePool(workers=1)

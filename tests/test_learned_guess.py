@@ -101,3 +101,4 @@ def test_pair_lists_match_oracle():
         for (a, na_, c, nc_) in info[sel][:, 1:].tolist():
             assert na_ in (1, 5) and nc_ in (1, 5)
             assert 0 <= a and a + na_ <= b.nao[m] and c + nc_ <= b.nao[m]
+
