@@ -37,6 +37,11 @@ HBM_PEAK_GBS = 8000.0
 # 8 host threads of one process on the same box: the threads serialise on the
 # interpreter lock during each molecule's set-up)
 PLAIN_WORKERS = 8
+# when worker processes cannot be started (a profiler attached to this process):
+# host threads of this process, each on its own HIP stream and hardware queue
+PLAIN_STREAMS = 8
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < PLAIN_STREAMS:  # read when HIP starts
+    os.environ["GPU_MAX_HW_QUEUES"] = str(PLAIN_STREAMS)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 
@@ -248,13 +253,27 @@ def main(argv=None):
     from pyscf_ipu_amd import molgen
     from pyscf_ipu_amd.gto import build_mol
     from pyscf_ipu_amd.nanoDFT.nanoDFT import KernelEvents, run_molecules
-    from pyscf_ipu_amd.nanoDFT.pool import MoleculePool
+    from pyscf_ipu_amd.nanoDFT.pool import MoleculePool, start_blocker
     from pyscf_ipu_amd import roofline
 
     # the worker processes start before this process uses the GPU, and this
     # process stays off the GPU while they run (its own queues would share the
     # hardware queue slots with theirs)
-    pool = MoleculePool(workers=PLAIN_WORKERS, dtype=a.dtype)
+    blocker = start_blocker()
+    pool = MoleculePool(workers=PLAIN_WORKERS, dtype=a.dtype) if blocker is None else None
+    if pool is None:
+        torch.cuda.set_device(0)
+        in_flight = f"{PLAIN_STREAMS} host threads on {PLAIN_STREAMS} HIP streams ({blocker})"
+    else:
+        in_flight = f"{PLAIN_WORKERS} worker processes"
+
+    def plain_run():
+        if pool is not None:
+            return pool.run(mols, its=a.its, conv_tol=conv_tol)
+        out = run_molecules(mols, streams=PLAIN_STREAMS, dtype=a.dtype, its=a.its, conv_tol=conv_tol)
+        torch.cuda.synchronize()
+        return out
+
     mol_strs = molgen.plain_set(a.nmol, 9, seed=0)
     mols = [build_mol(s) for s in mol_strs]
     golden = None
@@ -271,15 +290,16 @@ def main(argv=None):
     # A result arrives after its worker has synchronised its stream, so the
     # last result of a step marks the end of that step's GPU work.
     for _ in range(a.warmup):
-        pool.run(mols, its=a.its, conv_tol=conv_tol)
+        plain_run()
     t0 = time.perf_counter()
     results = []
     for _ in range(a.steps):
-        results = pool.run(mols, its=a.its, conv_tol=conv_tol)
+        results = plain_run()
     t1 = time.perf_counter()
     wall = t1 - t0
-    pool.close()
-    torch.cuda.set_device(0)
+    if pool is not None:
+        pool.close()
+        torch.cuda.set_device(0)
     energies = np.array([r.energy for r in results])
     iters = np.array([r.iterations for r in results])
     if golden is not None:
@@ -348,7 +368,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": "plain one-molecule-per-launch RKS B3LYP SCF: 32 seeded 9-heavy-atom C/N/O "
                                "molecules, STO-3G, grid level 0, |dE|<1e-8 stop, <=20 iterations, "
-                               f"{PLAIN_WORKERS} molecules in flight in {PLAIN_WORKERS} worker processes",
+                               f"molecules in flight: {in_flight}",
                    "n_molecules": a.nmol, "basis": "sto-3g", "grid_level": 0, "form": "plain",
                    "nao_range": [int(min(m.nao_nr() for m in mols)), int(max(m.nao_nr() for m in mols))]},
         "mean_scf_iters": float(iters.mean()), "passed_1mha": passed, "max_abs_dE_ha": max_dE,

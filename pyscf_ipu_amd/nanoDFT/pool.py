@@ -16,9 +16,23 @@ Tasks go through one shared queue, so the workers balance themselves.
 from __future__ import annotations
 
 import multiprocessing as mp
+import os
 import queue as _queue
 import traceback
 from typing import List, Optional, Sequence
+
+
+def start_blocker() -> Optional[str]:
+    """Why this process must not start worker processes, or None.  Workers are
+    started by fork + exec, which must not follow GPU initialisation here: by
+    torch, or by a profiler's preloaded library (it initialises the GPU before
+    the program starts)."""
+    import torch
+    if torch.cuda.is_initialized():
+        return "this process already uses the GPU"
+    if any(k.startswith("ROCPROF") for k in os.environ) or "rocprofiler" in os.environ.get("LD_PRELOAD", ""):
+        return "a profiler is attached to this process"
+    return None
 
 
 def _worker(task_q, result_q, level: int, dtype: str):
@@ -48,10 +62,10 @@ class MoleculePool:
     launches.  `run(mols)` returns the `SCFResult`s in input order."""
 
     def __init__(self, workers: int = 8, level: int = 0, dtype: str = "fp64", start_timeout: float = 600.0):
-        import torch
-        if torch.cuda.is_initialized():
-            raise RuntimeError("MoleculePool: create the pool before this process uses the GPU (its workers are "
-                               "started by fork + exec)")
+        reason = start_blocker()
+        if reason is not None:
+            raise RuntimeError(f"MoleculePool: {reason}; create the pool before this process uses the GPU "
+                               "(its workers are started by fork + exec)")
         if workers < 1:
             raise ValueError("MoleculePool: workers must be >= 1")
         ctx = mp.get_context("spawn")

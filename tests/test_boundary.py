@@ -57,3 +57,16 @@ def test_pool_refuses_after_gpu_use(monkeypatch):
     monkeypatch.setattr(torch.cuda, "is_initialized", lambda: True)
     with pytest.raises(RuntimeError, match="before this process uses the GPU"):
         MoleculePool(workers=1)
+
+
+def test_pool_refuses_under_a_profiler(monkeypatch):
+    """A profiler's preloaded library initialises the GPU before the program
+    starts, so no workers may be forked from such a process."""
+    from pyscf_ipu_amd.nanoDFT import pool
+    monkeypatch.setenv("ROCPROF_KERNEL_TRACE", "1")
+    assert "profiler" in pool.start_blocker()
+    with pytest.raises(RuntimeError, match="profiler"):
+        pool.MoleculePool(workers=1)
+    monkeypatch.delenv("ROCPROF_KERNEL_TRACE")
+    monkeypatch.setenv("LD_PRELOAD", "/opt/rocm/lib/librocprofiler-sdk-tool.so")
+    assert "profiler" in pool.start_blocker()
