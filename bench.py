@@ -133,8 +133,8 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
                      "max_abs_dgap_vs_fp64_ha": float(np.abs(res.hlgap - r64.hlgap)[ok].max()) if ok.any() else None,
                      "mean_scf_iters": float(g.iterations_total.mean()),
                      "fallback_to_minao": int(g.fallback.sum()),
-                     "fallback_not_converged": int((g.fallback & ~g.judged).sum()),
-                     "fallback_small_gap": int(g.judged.sum()),
+                     "fallback_not_converged": int((g.fallback & ~g.small_gap).sum()),
+                     "fallback_small_gap": int(g.small_gap.sum()),
                      "failed_1mha_with_converged_reference": int(((dE > 1e-3) & r64.converged).sum()),
                      "converged": int(res.converged.sum()),
                      "autotune_ms_per_launch": {k: {a: round(b, 3) for a, b in v.items()}

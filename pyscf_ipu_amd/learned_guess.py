@@ -15,7 +15,7 @@ within 5 Angstrom:
 The assembled matrix is symmetrised and purified on the GPU
 (`pscf_batch_purify`: exactly nocc doubly occupied natural orbitals, D S D = 2D).
 Molecules that do not converge within 15 iterations from the learned guess,
-or converge with a HOMO-LUMO gap below GAP_JUDGE (where the start can decide
+or converge with a HOMO-LUMO gap below GAP_RECHECK (where the start can decide
 which SCF solution is reached), are re-run from the MINAO guess (`solve`).  Every kernel -- features, the four
 GEMMs forward and backward, loss, bias gradients, Adam, target extraction,
 assembly, purification -- is the library's own HIP code; PyTorch only holds
@@ -257,38 +257,38 @@ def learned_guess(model: PairMLP):
 # which one the iterations reach depends on the start: from the learned guess,
 # 26 of 4096 generator molecules (seed 1) converged to a solution 1-18 mHa away
 # from the MINAO-started one (above it for some, below for others), every one
-# with a converged gap below 0.06 Ha.  The converged result is therefore judged:
-# a gap below GAP_JUDGE sends the molecule back to the MINAO start, the
+# with a converged gap below 0.06 Ha.  The converged result is therefore checked:
+# a gap below GAP_RECHECK sends the molecule back to the MINAO start, the
 # reference's own guess (pyscf_ipu/nanoDFT/nanoDFT.py:233), so that the state
 # reached is the reference's.  0.065 Ha leaves a margin over the largest gap
 # seen among those cases; 17% of the molecules of that set fall below it.
-GAP_JUDGE = 0.065
+GAP_RECHECK = 0.065
 
 
 @dataclass
 class GuessedResult:
     result: BatchResult          # merged per-molecule result
-    fallback: np.ndarray         # (M,) bool: re-run from MINAO (not converged or judged)
+    fallback: np.ndarray         # (M,) bool: re-run from MINAO (not converged, or a small gap)
     iterations_total: np.ndarray  # learned-guess iterations + fallback iterations
     tuning: dict = None          # autotuner record of the main batch (BatchRKS.tuning)
     setup_ms: dict = None        # set-up phase times of the main batch (BatchRKS.setup_timer)
-    judged: np.ndarray = None    # (M,) bool: converged from the learned guess with gap < GAP_JUDGE
+    small_gap: np.ndarray = None  # (M,) bool: converged from the learned guess with gap < GAP_RECHECK
 
 
-def rerun_mask(converged: np.ndarray, failed: np.ndarray, hlgap: np.ndarray, gap_judge: float = GAP_JUDGE):
+def rerun_mask(converged: np.ndarray, failed: np.ndarray, hlgap: np.ndarray, gap_recheck: float = GAP_RECHECK):
     """Molecules to re-run from MINAO after the learned-guess SCF: those not
-    converged (and not failed, e.g. over-size), and converged ones judged by a
-    gap below `gap_judge`.  Returns (re-run mask, judged mask)."""
-    judged = converged & ~failed & (hlgap < gap_judge)
-    return (~converged & ~failed) | judged, judged
+    converged (and not failed, e.g. over-size), and converged ones with a gap
+    below `gap_recheck`.  Returns (re-run mask, small-gap mask)."""
+    small_gap = converged & ~failed & (hlgap < gap_recheck)
+    return (~converged & ~failed) | small_gap, small_gap
 
 
 def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_its: int = 20,
           learned_its: int = 15, conv_tol: float = 1e-7, graph: bool = True, tune: bool = False,
-          gap_judge: float = GAP_JUDGE) -> GuessedResult:
+          gap_recheck: float = GAP_RECHECK) -> GuessedResult:
     """Batched SCF from the learned guess; molecules not converged after
     `learned_its` iterations, and converged ones whose HOMO-LUMO gap is below
-    `gap_judge` (Ha), are re-run from MINAO with `max_its` iterations."""
+    `gap_recheck` (Ha), are re-run from MINAO with `max_its` iterations."""
     b = BatchRKS(molecules, dtype=dtype)
     if model is not None:
         b.setup(guess=learned_guess(model))
@@ -299,7 +299,7 @@ def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_i
         none = np.zeros(len(molecules), bool)
         return GuessedResult(r, none, r.iterations.copy(), b.tuning, b.setup_timer.ms(), none.copy())
     tuning, setup_ms = b.tuning, b.setup_timer.ms()
-    redo_mask, judged = rerun_mask(r.converged, r.failed, r.hlgap, gap_judge)
+    redo_mask, small_gap = rerun_mask(r.converged, r.failed, r.hlgap, gap_recheck)
     redo = np.where(redo_mask)[0]
     total = r.iterations.copy()
     fb = np.zeros(len(molecules), bool)
@@ -317,4 +317,4 @@ def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_i
             total[i] += r2.iterations[k]
         fb[redo] = True
         r.n_iter_launches += r2.n_iter_launches
-    return GuessedResult(r, fb, total, tuning, setup_ms, judged)
+    return GuessedResult(r, fb, total, tuning, setup_ms, small_gap)

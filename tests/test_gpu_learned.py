@@ -192,10 +192,10 @@ def test_training_reduces_loss(hip):
 @pytest.mark.skipif(not os.path.exists(lg.DEFAULT_WEIGHTS), reason="trained weights not present")
 def test_trained_guess_gains_over_minao(hip):
     """Held-out molecules (seeds disjoint from training and from the set the
-    gap judge was measured on): the trained network's purified guess is closer
+    gap threshold was measured on): the trained network's purified guess is closer
     to the converged FP64 density than the MINAO guess (mean Frobenius distance
     at least 20% lower), and the batched SCF from it (MINAO fallback after 15
-    iterations or below the judged gap) reaches the MINAO-started energy of
+    iterations or below the gap threshold) reaches the MINAO-started energy of
     every molecule within 1 mHa.  Mean SCF iterations of both starts are printed."""
     mols = batch.parse(molgen.batched_set(64, (1, 9), seed=424242))
     conv = batch.BatchRKS(mols, dtype="fp64")
@@ -213,7 +213,7 @@ def test_trained_guess_gains_over_minao(hip):
     dE = np.abs(learned.result.energy - rc.energy)[ok]
     print(f"|D_guess - D_conv|_F minao {err_m:.3f} learned {err_l:.3f}; mean iterations minao "
           f"{base.iterations_total.mean():.2f} learned {learned.iterations_total.mean():.2f} "
-          f"(fallback {int(learned.fallback.sum())}, of which small gap {int(learned.judged.sum())}); "
+          f"(fallback {int(learned.fallback.sum())}, of which small gap {int(learned.small_gap.sum())}); "
           f"within 1 mHa {int((dE < 1e-3).sum())}/{len(ok)}")
     assert err_l < 0.8 * err_m
     assert (dE < 1e-3).all(), (np.where(dE >= 1e-3)[0], dE[dE >= 1e-3])

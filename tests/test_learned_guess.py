@@ -65,13 +65,13 @@ def test_oracle_features_closed_forms():
 
 def test_rerun_mask_selects_unconverged_and_small_gap():
     """Re-run from MINAO: not converged (unless failed), or converged with a
-    gap below the judge; a converged molecule at or above it is kept."""
+    gap below the threshold; a converged molecule at or above it is kept."""
     conv = np.array([True, True, False, False, True, True])
     failed = np.array([False, False, False, True, False, True])
-    gap = np.array([0.20, 0.03, 0.01, 0.30, lg.GAP_JUDGE, 0.01])
-    redo, judged = lg.rerun_mask(conv, failed, gap)
+    gap = np.array([0.20, 0.03, 0.01, 0.30, lg.GAP_RECHECK, 0.01])
+    redo, small_gap = lg.rerun_mask(conv, failed, gap)
     assert redo.tolist() == [False, True, True, False, False, False]
-    assert judged.tolist() == [False, True, False, False, False, False]
+    assert small_gap.tolist() == [False, True, False, False, False, False]
 
 
 def test_flat_layout_round_trip():
