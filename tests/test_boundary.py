@@ -70,3 +70,19 @@ def test_pool_refuses_under_a_profiler(monkeypatch):
     monkeypatch.delenv("ROCPROF_KERNEL_TRACE")
     monkeypatch.setenv("LD_PRELOAD", "/opt/rocm/lib/librocprofiler-sdk-tool.so")
     assert "profiler" in pool.start_blocker()
+
+
+def test_bench_reads_committed_traffic_profile():
+    """bench.py's roofline `traffic` comes from the committed counter summary:
+    the dominant plain kernel's row parses to HBM bytes per dispatch, and the
+    summary states the copy-kernel calibration it was scaled by."""
+    import importlib.util
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    b = bench._profile_traffic("scf_step_kernel<double>")
+    assert isinstance(b, int) and 1e5 < b < 1e8
+    assert bench._profile_traffic("no_such_kernel") is None
+    text = open(bench.TRAFFIC_PROFILE).read()
+    assert "copy-calibration" in text and "FETCH_SIZE is scaled by" in text
