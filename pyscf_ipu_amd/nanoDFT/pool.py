@@ -93,27 +93,46 @@ class MoleculePool:
                 waited += 1.0
                 dead = [p.exitcode for p in self._procs if not p.is_alive()]
                 if dead:
+                    try:  # a message the worker sent before it exited
+                        return self._results.get(timeout=1.0)
+                    except _queue.Empty:
+                        pass
                     raise RuntimeError(f"MoleculePool: a worker process exited (exit codes {dead})") from None
                 if timeout is not None and waited >= timeout:
                     raise RuntimeError("MoleculePool: no message from the workers within the time limit") from None
 
     def run(self, mols: Sequence, its: int = 20, conv_tol: float = 1e-8, diis: bool = True,
             timeout: Optional[float] = 600.0) -> List:
+        """Raises on the first failed molecule, dead worker or time-out, and the
+        pool is closed then: the batch's other molecules are still in flight, and
+        their results must not reach a later `run`."""
+        if not self._procs:
+            raise RuntimeError("MoleculePool: the pool is closed")
         for i, m in enumerate(mols):
             self._tasks.put((i, m, its, conv_tol, diis))
         out: List = [None] * len(mols)
-        for _ in range(len(mols)):
-            kind, idx, payload = self._next(timeout)
-            if kind != "ok":
-                raise RuntimeError(f"MoleculePool: molecule {idx} failed in a worker:\n{payload}")
-            out[idx] = payload
+        try:
+            for _ in range(len(mols)):
+                kind, idx, payload = self._next(timeout)
+                if kind != "ok":
+                    raise RuntimeError(f"MoleculePool: molecule {idx} failed in a worker:\n{payload}")
+                out[idx] = payload
+        except BaseException:
+            self.close()
+            raise
         return out
 
     def close(self):
-        for _ in self._procs:
-            self._tasks.put(None)
+        """Stops the workers: each finishes its molecule, then exits; one still
+        running after a minute is terminated."""
+        for p in self._procs:
+            if p.is_alive():
+                self._tasks.put(None)
         for p in self._procs:
             p.join(timeout=60)
+            if p.is_alive():
+                p.terminate()
+                p.join(timeout=10)
         self._procs = []
 
     def __enter__(self):

@@ -72,6 +72,21 @@ def test_pool_refuses_under_a_profiler(monkeypatch):
     assert "profiler" in pool.start_blocker()
 
 
+def test_pool_reports_a_worker_that_cannot_start():
+    """Without a GPU a worker fails to start: the pool raises with the worker's
+    error, leaves no process behind and refuses later use."""
+    import torch
+    from pyscf_ipu_amd.nanoDFT import pool
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present: workers would start")
+    with pytest.raises(RuntimeError, match="failed to start"):
+        pool.MoleculePool(workers=1, start_timeout=120)
+    p = object.__new__(pool.MoleculePool)
+    p._procs = []
+    with pytest.raises(RuntimeError, match="closed"):
+        p.run([])
+
+
 def test_bench_reads_committed_traffic_profile():
     """bench.py's roofline `traffic` comes from the committed counter summary:
     the dominant plain kernel's row parses to HBM bytes per dispatch, and the
