@@ -136,6 +136,11 @@ def _batched(nmol: int, steps: int, warmup: int, its: int):
                      "fallback_not_converged": int((g.fallback & ~g.small_gap).sum()),
                      "fallback_small_gap": int(g.small_gap.sum()),
                      "failed_1mha_with_converged_reference": int(((dE > 1e-3) & r64.converged).sum()),
+                     "failures": [{"molecule": int(i), "abs_dE_ha": float(dE[i]), "converged": bool(res.converged[i]),
+                                   "fallback": bool(g.fallback[i]), "small_gap": bool(g.small_gap[i]),
+                                   "gap_ha": float(res.hlgap[i]), "gap_fp64_ha": float(r64.hlgap[i]),
+                                   "iterations": int(g.iterations_total[i])}
+                                  for i in np.where((dE > 1e-3) & r64.converged)[0][:8]],
                      "converged": int(res.converged.sum()),
                      "autotune_ms_per_launch": {k: {a: round(b, 3) for a, b in v.items()}
                                                 for k, v in (g.tuning or {}).get("ms", {}).items()},

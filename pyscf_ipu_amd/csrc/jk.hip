@@ -155,9 +155,9 @@ __device__ inline void tri_split(int c, int& r, int& s) {
 // for it, and the group's reads would become a chain of round trips.
 constexpr int S8_GROUP = 4;
 
-template <typename T, bool PRESCALED>
+template <typename T, bool PRESCALED, typename TA = T>
 __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
-                                       T* __restrict__ acc, int N, T hhalf, int lane) {
+                                       TA* __restrict__ acc, int N, T hhalf, int lane) {
   const T dpq = dm[p * N + q];
   const T fpq = (p == q && !PRESCALED) ? T(0.5) : T(1);
   const T* dmp = dm + p * N;
@@ -197,9 +197,9 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
       const T wk = -hhalf * w;
       jrow += w2 * d_rs[u];
       if (valid) {
-        atomicAdd(acc + r[u] * N + s[u], w2 * dpq);
-        atomicAdd(acc + p * N + s[u], wk * d_qr[u]);
-        atomicAdd(acc + q * N + s[u], wk * d_pr[u]);
+        atomicAdd(acc + r[u] * N + s[u], (TA)(w2 * dpq));
+        atomicAdd(acc + p * N + s[u], (TA)(wk * d_qr[u]));
+        atomicAdd(acc + q * N + s[u], (TA)(wk * d_pr[u]));
       }
       T a = wk * d_qs[u], b = wk * d_ps[u];
       const int key = valid ? r[u] : -1;
@@ -207,14 +207,14 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
       // the run of this lane's r ends here: the next position starts a new r,
       // or the chunk or the row ends
       if (valid && (s[u] == r[u] || lane == 63 || rs == pq)) {
-        atomicAdd(acc + p * N + r[u], a);
-        atomicAdd(acc + q * N + r[u], b);
+        atomicAdd(acc + p * N + r[u], (TA)a);
+        atomicAdd(acc + q * N + r[u], (TA)b);
       }
     }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) jrow += __shfl_xor(jrow, off, 64);
-  if (lane == 0) atomicAdd(acc + p * N + q, jrow);
+  if (lane == 0) atomicAdd(acc + p * N + q, (TA)jrow);
 }
 
 // One wavefront per row pq of the packed lower triangle; lanes walk rs <= pq.
@@ -253,8 +253,11 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
 // with ACC, added to an FP64 accumulator (incremental build from density
 // changes: the FP32 rounding then scales with |dD| instead of |D|).  The ERIs
 // come divided by their multiplicity (pscf_batch_eri_scale).  LDS holds
-// dm and M of one molecule, 2 nao^2 elements, sized by the launch for the
-// largest molecule it covers; 256 or 512 threads share them.
+// M (FP64) and dm of one molecule, nao^2 elements each, sized by the launch
+// for the largest molecule it covers; 256 or 512 threads share them.  M is
+// FP64 whatever T is: the waves' atomic adds into it come in an order that
+// changes from run to run, which shows at 1e-7 of the sum in FP32 and at
+// 1e-16 in FP64 (ds_add_f64, no slower here than the FP32 adds).
 constexpr int JKB_MAX_THREADS = 512;
 
 template <typename T, typename TO, bool ACC>
@@ -277,11 +280,11 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const Batc
   }
   const T* eri = eri_all + Mo.eri0;
   const T* dm_g = dm_all + Mo.mat0;
-  T* dm = reinterpret_cast<T*>(smem);
-  T* acc = dm + NN;
+  double* acc = reinterpret_cast<double*>(smem);
+  T* dm = reinterpret_cast<T*>(acc + NN);
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     dm[x] = dm_g[x];
-    acc[x] = T(0);
+    acc[x] = 0.0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const Batc
   for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
     int p, q;
     tri_split(pq, p, q);
-    s8_row<T, true>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
+    s8_row<T, true, double>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
   const bool atomic = gridDim.y > 1;
@@ -347,11 +350,11 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
   }
   const T* eri = eri_all + Mo.eri0;
   const T* dm_g = dm_all + Mo.mat0;
-  T* dm = reinterpret_cast<T*>(smem);
-  T* acc = dm + NN;
+  double* acc = reinterpret_cast<double*>(smem);
+  T* dm = reinterpret_cast<T*>(acc + NN);
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     dm[x] = dm_g[x];
-    acc[x] = T(0);
+    acc[x] = 0.0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -417,19 +420,19 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
       }
       const T tot = transpose_reduce16(t, lane) * -hhalf;
       const int u = lane & 7;
-      if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, tot);
+      if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, (double)tot);
     }
     jrow = wave_sum_last(jrow) * T(2);
     if (lane < N) {
-      atomicAdd(acc + p * N + lane, kp * -hhalf);
-      atomicAdd(acc + q * N + lane, kq * -hhalf);
+      atomicAdd(acc + p * N + lane, (double)(kp * -hhalf));
+      atomicAdd(acc + q * N + lane, (double)(kq * -hhalf));
     }
-    if (lane == 63) atomicAdd(acc + p * N + q, jrow);
+    if (lane == 63) atomicAdd(acc + p * N + q, (double)jrow);
   }
   // (predicated rather than cut short, so the loop stays unrolled and jacc in registers)
 #pragma unroll
   for (int r = 0; r < JKC_MAXN; ++r)
-    if (r < N && lane <= r) atomicAdd(acc + r * N + lane, jacc[r]);
+    if (r < N && lane <= r) atomicAdd(acc + r * N + lane, (double)jacc[r]);
   __syncthreads();
   const bool atomic = gridDim.y > 1;
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
@@ -599,7 +602,7 @@ extern "C" int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_orde
                "pscf_batch_jk_s8: bad arguments");
   PSCF_REQUIRE(split >= 1 && split <= 64, "pscf_batch_jk_s8: bad split %d", split);
   PSCF_REQUIRE(threads == 256 || threads == JKB_MAX_THREADS, "pscf_batch_jk_s8: threads %d not 256 or 512", threads);
-  const size_t lds = dtype == PSCF_F64 ? jk_s8_lds<double>(max_nao) : jk_s8_lds<float>(max_nao);
+  const size_t lds = (size_t)max_nao * max_nao * ((dtype == PSCF_F64 ? 8 : 4) + sizeof(double));
   PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_batch_jk_s8: nao %d too large for LDS", max_nao);
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(nmol, split);
