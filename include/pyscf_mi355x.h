@@ -232,7 +232,9 @@ int pscf_batch_eri_scale(int dtype, const void* d_mols, int nmol, int max_nao, v
  * LDS; a larger molecule gets NaN).  accumulate = 0: d_out (dtype) =
  * dJK(d_dm); accumulate = 1: d_out (FP64) += dJK(d_dm), the incremental build
  * from density changes.  split workgroups of `threads` (256 or 512) threads
- * share each molecule's ERI rows (atomic accumulation). */
+ * share each molecule's ERI rows (atomic accumulation).  Each workgroup holds
+ * dm (dtype) and an FP64 accumulator in LDS: max_nao^2 * (sizeof(dtype) + 8)
+ * bytes, at most 160 KiB (max_nao <= 116 in FP32, 101 in FP64). */
 int pscf_batch_jk_s8(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                      const void* d_eri, const void* d_dm, double hyb, void* d_out, int accumulate, int split,
                      int threads, void* stream);
