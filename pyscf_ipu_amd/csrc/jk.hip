@@ -218,18 +218,22 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
 }
 
 // One wavefront per row pq of the packed lower triangle; lanes walk rs <= pq.
-template <typename T>
+// The LDS accumulator (TA, first in the LDS, then dm) is FP64 unless
+// nao^2 (sizeof(T) + 8) bytes do not fit: the waves' atomic adds come in an
+// order that changes from run to run, which shows at 1e-7 of the sum in FP32
+// and at 1e-16 in FP64 (ds_add_f64, no slower here than the FP32 adds).
+template <typename T, typename TA>
 __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__ eri, int N,
                                                            const T* __restrict__ dm_g, T hhalf,
                                                            T* __restrict__ part) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NN = N * N;
   const int P = N * (N + 1) / 2;
-  T* dm = reinterpret_cast<T*>(smem);
-  T* acc = dm + NN;
+  TA* acc = reinterpret_cast<TA*>(smem);
+  T* dm = reinterpret_cast<T*>(acc + NN);
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {
     dm[x] = dm_g[x];
-    acc[x] = T(0);
+    acc[x] = TA(0);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -238,11 +242,11 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
   for (int pq = wave; pq < P; pq += nwaves) {
     int p, q;
     tri_split(pq, p, q);
-    s8_row<T, false>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
+    s8_row<T, false, TA>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
   T* dst = part + (int64_t)blockIdx.x * NN;
-  for (int x = threadIdx.x; x < NN; x += blockDim.x) dst[x] = acc[x];
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) dst[x] = (T)acc[x];
 }
 
 // ------------------------------------------------------------- batched ----
@@ -519,21 +523,28 @@ static size_t jk_s8_ws_bytes(size_t es, int N) {
   return red_align(es * (size_t)nb * N * N) + es * (size_t)red_chunks(nb) * N * N;
 }
 
-template <typename T>
+template <typename T, typename TA>
 static size_t jk_s8_lds(int N) {
-  return sizeof(T) * 2 * (size_t)N * N;
+  return (sizeof(T) + sizeof(TA)) * (size_t)N * N;
 }
 
 template <typename T>
 static int jk_s8_impl(const T* eri, int N, const T* dm, double hyb, T* out, void* ws, size_t ws_bytes,
                       hipStream_t st) {
-  const size_t lds = jk_s8_lds<T>(N);
+  // FP64 accumulator when it fits, else one of T (FP32 molecules up to nao 143)
+  const bool acc64 = (int)jk_s8_lds<T, double>(N) <= max_lds_bytes();
+  const size_t lds = acc64 ? jk_s8_lds<T, double>(N) : jk_s8_lds<T, T>(N);
   PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_jk_s8: N=%d needs %zu B of LDS (max %d)", N, lds,
                max_lds_bytes());
   const int nb = jk_s8_blocks(N);
   PSCF_REQUIRE(ws && ws_bytes >= jk_s8_ws_bytes(sizeof(T), N), "pscf_jk_s8: workspace too small");
-  PSCF_SET_LDS(jk_s8_kernel<T>, lds);
-  jk_s8_kernel<T><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
+  if (acc64) {
+    PSCF_SET_LDS((jk_s8_kernel<T, double>), lds);
+    jk_s8_kernel<T, double><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
+  } else {
+    PSCF_SET_LDS((jk_s8_kernel<T, T>), lds);
+    jk_s8_kernel<T, T><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
+  }
   PSCF_HIP(hipGetLastError());
   T* lvl = (T*)((char*)ws + red_align(sizeof(T) * (size_t)nb * N * N));
   return reduce_parts<T, true>((const T*)ws, nb, N, lvl, out, nullptr, nullptr, st);

@@ -127,6 +127,40 @@ def test_jk_s8(hip, mol, dtype, tol):
     assert np.abs(got - ref).max() < tol * max(1.0, np.abs(ref).max())
 
 
+def test_jk_s8_fp32_large_nao(hip):
+    """nao 120 in FP32: dm and an FP64 accumulator exceed the LDS, so the
+    kernel accumulates in FP32.  A few hundred nonzero s8 ERIs, checked against
+    J_kl = sum (ij|kl) D_ij, K_il = sum (ij|kl) D_jk over their 8-fold images
+    (dJK = J - 0.1 K for B3LYP)."""
+    from pyscf_ipu_amd.nanoDFT import sparse_symmetric_ERI as ssE
+    N = 120
+    P = N * (N + 1) // 2
+    rng = np.random.default_rng(5)
+    e8 = np.zeros(P * (P + 1) // 2, np.float32)
+    dm = rng.normal(size=(N, N))
+    dm = dm + dm.T
+    J = np.zeros((N, N))
+    K = np.zeros((N, N))
+    for _ in range(300):
+        i, k = sorted(rng.integers(0, N, 2), reverse=True)
+        j, l = rng.integers(0, i + 1), rng.integers(0, k + 1)
+        pq, rs = i * (i + 1) // 2 + j, k * (k + 1) // 2 + l
+        if pq < rs:
+            (i, j, pq), (k, l, rs) = (k, l, rs), (i, j, pq)
+        pos = pq * (pq + 1) // 2 + rs
+        if e8[pos] != 0:
+            continue
+        v = np.float32(rng.normal())
+        e8[pos] = v
+        for a, b, c, d in {(i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                           (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)}:
+            J[c, d] += float(v) * dm[a, b]
+            K[a, d] += float(v) * dm[b, c]
+    ref = J - 0.1 * K
+    got = _np(ssE.jk_s8(device.to_dev(e8, torch.float32), device.to_dev(dm, torch.float32)))
+    assert np.abs(got - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())
+
+
 @pytest.mark.parametrize("dtype,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
 def test_ao_eval(hip, mol, dtype, tol):
     from pyscf_ipu_amd import _lib
