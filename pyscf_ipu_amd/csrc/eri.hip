@@ -332,24 +332,20 @@ struct BoysType<float, 1> {
   using type = float;
 };
 
+// Quartet q of class (X, Y) of a molecule's s8 array: pair a of class X and
+// pair b of class Y (a >= b within one class), pairs listed from x0 / y0.
 template <typename T, int X, int Y>
-__global__ void __launch_bounds__(256) eri_s8_kernel(const int* __restrict__ atm, const int* __restrict__ bas,
-                                                     const double* __restrict__ env, int nbas,
-                                                     const int* __restrict__ pairs, int x0, int nx, int y0, int ny,
-                                                     T* __restrict__ out) {
+__device__ inline void s8_quartet(const int* __restrict__ atm, const int* __restrict__ bas,
+                                  const double* __restrict__ env, const int* __restrict__ ao_loc,
+                                  const int* __restrict__ pairs, int x0, int nx, int y0, int ny, int64_t q,
+                                  const double* tab64, const float* tab32, T* __restrict__ out) {
   constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
   constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
   using BT = typename BoysType<T, NR>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
-  BT* tab = reinterpret_cast<BT*>(smem);
-  int* ao_loc = reinterpret_cast<int*>(smem + sizeof(BT) * BOYS_NT * BOYS_NM);
-  for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) tab[i] = (BT)g_boys[i];
-  stage_ao_loc(ao_loc, bas, nbas);
-  __syncthreads();
-
+  const BT* tab;
+  if constexpr (sizeof(BT) == 8) tab = tab64; else tab = tab32;
   const bool tri_launch = (X == Y);
   const int64_t total = tri_launch ? tri((int64_t)nx) : (int64_t)nx * ny;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= total) return;
   int64_t a, b;
   if (tri_launch) {
@@ -386,6 +382,46 @@ __global__ void __launch_bounds__(256) eri_s8_kernel(const int* __restrict__ atm
           const int64_t idx = pq >= rs ? tri(pq) + rs : tri(rs) + pq;
           out[idx] = acc[((fi * NJ + fj) * NK + fk) * NL + fl];
         }
+}
+
+// The six shell-pair classes of one molecule in one launch, so they run side
+// by side rather than one after another: blocks [b0[c], b0[c + 1]) take class
+// S8_ORDER[c], the heaviest first.  off / cnt: each pair class's first pair
+// and count in `pairs` (s-s, p-s, p-p).
+constexpr int S8_CLASSES = 6;
+struct S8Launch {
+  int off[3], cnt[3];
+  int b0[S8_CLASSES + 1];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) eri_s8_kernel(const int* __restrict__ atm, const int* __restrict__ bas,
+                                                     const double* __restrict__ env, int nbas,
+                                                     const int* __restrict__ pairs, S8Launch L,
+                                                     T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* tab64 = reinterpret_cast<double*>(smem);
+  float* tab32 = reinterpret_cast<float*>(smem + sizeof(double) * BOYS_NT * BOYS_NM);
+  int* ao_loc = reinterpret_cast<int*>(smem + (sizeof(double) + sizeof(float)) * BOYS_NT * BOYS_NM);
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) {
+    tab64[i] = g_boys[i];
+    if constexpr (sizeof(T) == 4) tab32[i] = (float)g_boys[i];
+  }
+  stage_ao_loc(ao_loc, bas, nbas);
+  __syncthreads();
+  const int b = blockIdx.x;
+  const int c = b < L.b0[1] ? 0 : b < L.b0[2] ? 1 : b < L.b0[3] ? 2 : b < L.b0[4] ? 3 : b < L.b0[5] ? 4 : 5;
+  const int64_t q = (int64_t)(b - L.b0[c]) * blockDim.x + threadIdx.x;
+  const int* o = L.off;
+  const int* n = L.cnt;
+  switch (c) {  // S8_ORDER: (2,2), (2,1), (2,0), (1,1), (1,0), (0,0)
+    case 0: s8_quartet<T, 2, 2>(atm, bas, env, ao_loc, pairs, o[2], n[2], o[2], n[2], q, tab64, tab32, out); break;
+    case 1: s8_quartet<T, 2, 1>(atm, bas, env, ao_loc, pairs, o[2], n[2], o[1], n[1], q, tab64, tab32, out); break;
+    case 2: s8_quartet<T, 2, 0>(atm, bas, env, ao_loc, pairs, o[2], n[2], o[0], n[0], q, tab64, tab32, out); break;
+    case 3: s8_quartet<T, 1, 1>(atm, bas, env, ao_loc, pairs, o[1], n[1], o[1], n[1], q, tab64, tab32, out); break;
+    case 4: s8_quartet<T, 1, 0>(atm, bas, env, ao_loc, pairs, o[1], n[1], o[0], n[0], q, tab64, tab32, out); break;
+    default: s8_quartet<T, 0, 0>(atm, bas, env, ao_loc, pairs, o[0], n[0], o[0], n[0], q, tab64, tab32, out); break;
+  }
 }
 
 // Quartets of an explicit list; each lane canonicalises its quartet to a class
@@ -728,34 +764,33 @@ __global__ void __launch_bounds__(256) int1e_batch_kernel(const BatchMol* __rest
   }
 }
 
-template <typename T, int X, int Y>
-static int launch_s8(const int* atm, const int* bas, const double* env, int nbas, const int* pairs, const int* off,
-                     const int* cnt, T* out, hipStream_t st) {
-  const int nx = cnt[X], ny = cnt[Y];
-  if (nx == 0 || ny == 0) return PSCF_OK;
-  const int64_t total = (X == Y) ? tri((int64_t)nx) : (int64_t)nx * ny;
-  constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
-  constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
-  using BT = typename BoysType<T, NR>::type;
-  const size_t shm = sizeof(BT) * BOYS_NT * BOYS_NM + sizeof(int) * (nbas + 1);
-  const int64_t blocks = (total + 255) / 256;
-  PSCF_REQUIRE(blocks < (1ll << 31), "pscf_eri_s8: too many quartets");
-  eri_s8_kernel<T, X, Y><<<(unsigned)blocks, 256, shm, st>>>(atm, bas, env, nbas, pairs, off[X], nx, off[Y], ny, out);
-  PSCF_HIP(hipGetLastError());
-  return PSCF_OK;
+// Quartets of pair classes (X, Y): all pairs of X against all of Y, or the
+// lower triangle when X == Y.
+static int64_t s8_class_quartets(int X, int Y, const int* cnt) {
+  return X == Y ? tri((int64_t)cnt[X]) : (int64_t)cnt[X] * cnt[Y];
 }
 
 template <typename T>
 static int eri_s8_all(const int* atm, const int* bas, const double* env, int nbas, const int* pairs, const int* cnt,
                       T* out, hipStream_t st) {
-  int off[3] = {0, cnt[0], cnt[0] + cnt[1]};
-  int rc;
-  if ((rc = launch_s8<T, 0, 0>(atm, bas, env, nbas, pairs, off, cnt, out, st))) return rc;
-  if ((rc = launch_s8<T, 1, 0>(atm, bas, env, nbas, pairs, off, cnt, out, st))) return rc;
-  if ((rc = launch_s8<T, 1, 1>(atm, bas, env, nbas, pairs, off, cnt, out, st))) return rc;
-  if ((rc = launch_s8<T, 2, 0>(atm, bas, env, nbas, pairs, off, cnt, out, st))) return rc;
-  if ((rc = launch_s8<T, 2, 1>(atm, bas, env, nbas, pairs, off, cnt, out, st))) return rc;
-  if ((rc = launch_s8<T, 2, 2>(atm, bas, env, nbas, pairs, off, cnt, out, st))) return rc;
+  static const int order[S8_CLASSES][2] = {{2, 2}, {2, 1}, {2, 0}, {1, 1}, {1, 0}, {0, 0}};
+  S8Launch L;
+  L.off[0] = 0;
+  L.off[1] = cnt[0];
+  L.off[2] = cnt[0] + cnt[1];
+  for (int x = 0; x < 3; ++x) L.cnt[x] = cnt[x];
+  int64_t blocks = 0;
+  for (int c = 0; c < S8_CLASSES; ++c) {
+    L.b0[c] = (int)blocks;
+    blocks += (s8_class_quartets(order[c][0], order[c][1], cnt) + 255) / 256;
+    PSCF_REQUIRE(blocks < (1ll << 31), "pscf_eri_s8: too many quartets");
+  }
+  L.b0[S8_CLASSES] = (int)blocks;
+  if (blocks == 0) return PSCF_OK;
+  const size_t shm = (sizeof(double) + sizeof(float)) * BOYS_NT * BOYS_NM + sizeof(int) * (nbas + 1);
+  PSCF_SET_LDS(eri_s8_kernel<T>, shm);
+  eri_s8_kernel<T><<<(unsigned)blocks, 256, shm, st>>>(atm, bas, env, nbas, pairs, L, out);
+  PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
 
