@@ -57,6 +57,18 @@ def test_eri_s8(hip, mol, dtype, tol):
     assert err.max() < tol * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+def test_eri_s8_s_shells_only(hip, dtype, tol):
+    """H2 + H2: s shells only, so the launch's block ranges of the five classes
+    with p shells are empty and every block takes the (ss|ss) class."""
+    from pyscf_ipu_amd.electron_repulsion import direct
+    mol = build_mol("H 0 0 0; H 0 0 0.74; H 2.5 0 0; H 2.5 0.3 0.7")
+    got = _np(direct.eri_s8(mol, dtype)).astype(np.float64)
+    ref = cint.eri_s8(mol)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() < tol * max(1.0, np.abs(ref).max())
+
+
 def test_eri_quartets_block_layout(hip):
     from pyscf_ipu_amd.electron_repulsion import direct
     mol = build_mol(MOLS["hcno"])
