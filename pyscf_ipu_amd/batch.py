@@ -444,6 +444,36 @@ class BatchRKS:
         return BatchResult(E[:, 0].copy(), E, ctl[:, 6].astype(int), ctl[:, 2] > 0, ctl[:, 2] < 0, ctl[:, 5].copy(),
                            launches)
 
+    def restart(self, molecules: np.ndarray):
+        """Prepares molecules (indices) for another `run` from the atomic guess,
+        in place: their density, dJK accumulator, DIIS / warm-start state and
+        control words are reset, and every other molecule stays inactive.
+        Grids, integrals, ERIs and the tuned tiling are kept."""
+        idx = np.unique(np.asarray(molecules, dtype=np.int64))
+        if len(idx) == 0:
+            return
+        P, st = device.ptr, device.stream()
+        nm = self.sizes["mat"]
+        D0, Dt0 = device.empty(nm, torch.float64), device.empty(nm, self.tdt)
+        _lib.call("pscf_batch_guess", self.code, P(self.d_desc), self.M, P(self.d_atm), P(self.d_bas),
+                  P(self.d_shell_ao), P(self.S), P(D0), P(Dt0), st)
+
+        def spans(start, length):  # element indices of [start, start + length) per molecule
+            length = length.astype(np.int64)
+            return np.repeat(start - (np.cumsum(length) - length), length) + np.arange(length.sum())
+
+        e = device.to_dev(spans(self.desc[idx, 12], self.nao[idx] ** 2))
+        self.D[e] = D0[e]
+        self.Dt[e] = Dt0[e]
+        self.dJK[e] = 0.0
+        self.state[device.to_dev(spans(self.desc[idx, 15], state_doubles(self.nao[idx])))] = 0.0
+        ctl = self.ctl.cpu().numpy()
+        ctl[:, 1] = 0.0
+        ctl[idx, :] = 0.0
+        ctl[idx, 1] = 1.0
+        self.ctl.copy_(torch.from_numpy(ctl))
+        torch.cuda.synchronize()
+
     # ---------------------------------------------------------------- accessors ---
     def matrix(self, buf: torch.Tensor, m: int) -> np.ndarray:
         N, o = int(self.nao[m]), int(self.desc[m, 12])

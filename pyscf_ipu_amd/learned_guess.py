@@ -304,17 +304,16 @@ def solve(molecules: Sequence, model: PairMLP = None, dtype: str = "fp32", max_i
     total = r.iterations.copy()
     fb = np.zeros(len(molecules), bool)
     if len(redo):
-        del b
-        b2 = BatchRKS([molecules[i] for i in redo], dtype=dtype)
-        r2 = b2.run(max_its=max_its, conv_tol=conv_tol, graph=graph)
-        for k, i in enumerate(redo):
-            r.energy[i] = r2.energy[k]
-            r.terms[i] = r2.terms[k]
-            r.converged[i] = r2.converged[k]
-            r.failed[i] = r2.failed[k]
-            r.hlgap[i] = r2.hlgap[k]
-            r.iterations[i] = r2.iterations[k]
-            total[i] += r2.iterations[k]
+        # restarted in place: grids, integrals and ERIs of the first run are kept
+        b.restart(redo)
+        r2 = b.run(max_its=max_its, conv_tol=conv_tol, graph=graph)
+        r.energy[redo] = r2.energy[redo]
+        r.terms[redo] = r2.terms[redo]
+        r.converged[redo] = r2.converged[redo]
+        r.failed[redo] = r2.failed[redo]
+        r.hlgap[redo] = r2.hlgap[redo]
+        r.iterations[redo] = r2.iterations[redo]
+        total[redo] += r2.iterations[redo]
         fb[redo] = True
         r.n_iter_launches += r2.n_iter_launches
     return GuessedResult(r, fb, total, tuning, setup_ms, small_gap)

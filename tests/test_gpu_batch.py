@@ -57,6 +57,24 @@ def test_batch_graph_replay_equals_eager_fp64(hip):
     assert np.array_equal(graphed.iterations, eager.iterations)
 
 
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-8), ("fp32", 1e-4)])
+def test_batch_restart_in_place(hip, dtype, tol):
+    """Molecules restarted in place from the atomic guess (after a first run cut
+    short) reach what a fresh batch reaches; the others keep their first-run
+    state and stay inactive."""
+    mols = batch.parse(STRS)
+    fresh = batch.BatchRKS(mols, dtype=dtype).run(max_its=30, conv_tol=1e-9 if dtype == "fp64" else 1e-6)
+    b = batch.BatchRKS(mols, dtype=dtype)
+    first = b.run(max_its=3, conv_tol=1e-9)
+    redo = np.array([0, 3, 6])
+    b.restart(redo)
+    again = b.run(max_its=30, conv_tol=1e-9 if dtype == "fp64" else 1e-6)
+    assert again.converged[redo].all()
+    assert np.abs(again.energy[redo] - fresh.energy[redo]).max() < tol
+    others = np.setdiff1d(np.arange(len(mols)), redo)
+    assert np.array_equal(again.iterations[others], first.iterations[others])
+
+
 def test_batch_matches_plain_path(hip):
     from pyscf_ipu_amd.nanoDFT.nanoDFT import RKS
     s = STRS[3]
