@@ -72,11 +72,14 @@ def class_pairs(X: int, ns: np.ndarray, npp: np.ndarray) -> np.ndarray:
 
 def _chunks(counts: np.ndarray, size: int) -> np.ndarray:
     """(owner, start) pairs covering range(counts[i]) in steps of `size`."""
-    n = (counts + size - 1) // size
-    owner = np.repeat(np.arange(len(counts)), n)
-    first = np.repeat(np.cumsum(n) - n, n)
-    start = (np.arange(n.sum()) - first) * size
-    return np.stack([owner, start], axis=1).astype(np.int32)
+    n = (counts.astype(np.int64) + size - 1) // size
+    total = int(n.sum())
+    out = np.empty((total, 2), np.int32)  # written in place, int32 throughout (2 M rows at 4096 molecules)
+    out[:, 0] = np.repeat(np.arange(len(counts), dtype=np.int32), n)
+    first = np.repeat((np.cumsum(n) - n).astype(np.int32), n)
+    np.subtract(np.arange(total, dtype=np.int32), first, out=out[:, 1])
+    out[:, 1] *= size
+    return out
 
 
 def parse(mol_strs: Sequence[str]) -> List[Tuple[np.ndarray, np.ndarray]]:
