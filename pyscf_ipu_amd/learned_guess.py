@@ -62,24 +62,35 @@ def pair_lists(b: BatchRKS, molecules: np.ndarray = None) -> Tuple[np.ndarray, n
     `molecules` restricts the list to those molecule indices."""
     R = b.env[int(b.atm[0, 1]):].reshape(-1, 3) if b.A else np.zeros((0, 3))
     Z = b.atm[:, 0]
-    pairs, info = [], []
     mol_sel = np.ones(b.M, bool) if molecules is None else np.isin(np.arange(b.M), molecules)
-    atm0 = np.cumsum(b.natm) - b.natm
+    atm0 = (np.cumsum(b.natm) - b.natm).astype(np.int32)
+    ao0, nao_a = b.atom_ao0.astype(np.int32), b.atom_nao.astype(np.int32)
+    pieces = []  # per atom count: int32 rows (a, b, molecule, AO0 a, nAO a, AO0 b, nAO b)
     for na in np.unique(b.natm):
-        ms = np.where((b.natm == na) & mol_sel)[0]
+        ms = np.where((b.natm == na) & mol_sel)[0].astype(np.int32)
         if len(ms) == 0:
             continue
-        idx = atm0[ms][:, None] + np.arange(na)[None, :]               # (k, na)
+        idx = atm0[ms][:, None] + np.arange(na, dtype=np.int32)[None, :]  # (k, na)
         r = R[idx]
-        d = np.linalg.norm(r[:, :, None] - r[:, None, :], axis=-1)      # (k, na, na)
-        k, ia, ib = np.nonzero(d < RCUT_BOHR)
-        ga, gb = idx[k, ia], idx[k, ib]
-        pairs.append(np.stack([ga, gb], axis=1))
-        info.append(np.stack([ms[k], b.atom_ao0[ga], b.atom_nao[ga], b.atom_ao0[gb], b.atom_nao[gb]], axis=1))
-    if not pairs:
+        # |r_a - r_b| summed as (dx^2 + dy^2) + dz^2, then the square root
+        dx = r[:, :, None, 0] - r[:, None, :, 0]
+        dy = r[:, :, None, 1] - r[:, None, :, 1]
+        dz = r[:, :, None, 2] - r[:, None, :, 2]
+        k, ia, ib = np.nonzero(np.sqrt(dx * dx + dy * dy + dz * dz) < RCUT_BOHR)
+        out = np.empty((len(k), 7), np.int32)
+        out[:, 0] = idx[k, ia]
+        out[:, 1] = idx[k, ib]
+        out[:, 2] = ms[k]
+        out[:, 3] = ao0[out[:, 0]]
+        out[:, 4] = nao_a[out[:, 0]]
+        out[:, 5] = ao0[out[:, 1]]
+        out[:, 6] = nao_a[out[:, 1]]
+        pieces.append(out)
+    if not pieces:
         return np.zeros((0, 2), np.int32), np.zeros((0, 5), np.int32)
     assert set(np.unique(Z).tolist()) <= set(AO_PER_ATOM), "learned guess covers H, C, N, O"
-    return np.concatenate(pairs).astype(np.int32), np.concatenate(info).astype(np.int32)
+    rows = np.concatenate(pieces)
+    return np.ascontiguousarray(rows[:, :2]), np.ascontiguousarray(rows[:, 2:])
 
 
 def features(b: BatchRKS, pairs: np.ndarray, fp32_copy: bool = False):
