@@ -84,6 +84,24 @@ size_t pscf_jk_s8_workspace_bytes(int dtype, int N);
 int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_dm, double hyb, void* d_out, void* d_ws,
                size_t ws_bytes, void* stream);
 
+/* pscf_jk_blocks: vj and vk of ipu_einsum(__out3, dm, mol)
+ * (pyscf_ipu/electron_repulsion/direct.py:489-504, vertices
+ * poplar_direct_s1*, int2e_sph.cpp:7598-8176): vj[k][l] = sum (ij|kl) dm[j][i],
+ * vk[i][l] = sum (ij|kl) dm[j][k] over the ERI blocks of the shell-quartet list
+ * (electron_repulsion_integrals' output concatenated: nclass classes of
+ * uniform block size h_class_size[c], class c's elements starting at
+ * h_class_elem0[c], its blocks at h_class_blk0[c]; h_class_elem0[nclass] =
+ * total elements).  d_blocks: int32 [nblk][4] per block = (di | dj << 4 |
+ * dk << 8 | dl << 12 | permutation mask << 16, i0 | j0 << 16, k0 | l0 << 16,
+ * 0); block elements in libcint's column-major (di, dj, dk, dl) order; mask bit
+ * p applies the p-th index image of the reference's do_list
+ * (direct.py:1029-1110).  dm need not be symmetric.  LDS holds dm, vj and vk
+ * (3 N^2 elements of dtype, at most 160 KiB). */
+size_t pscf_jk_blocks_workspace_bytes(int dtype, int N, int64_t nelem);
+int pscf_jk_blocks(int dtype, const void* d_eri, const int* d_blocks, int nclass, const int64_t* h_class_elem0,
+                   const int* h_class_blk0, const int* h_class_size, const void* d_dm, int N, void* d_vj, void* d_vk,
+                   void* d_ws, size_t ws_bytes, void* stream);
+
 /* ------------------------------------------------------------------- XC ----
  * pscf_ao_eval: AO values and gradients on grid points, the role of
  * mol.eval_gto('GTOval_sph_deriv1', coords, 4) (nanoDFT.py:219-220).
@@ -248,10 +266,13 @@ int pscf_batch_scf_step(int dtype, const void* d_mols, const int* d_order, int n
                         double* d_C, double* d_state, int use_diis, void* stream);
 /* Energies (d_E [nmol][5], nanoDFT.py:20-38) and convergence control: init
  * records E_prev; otherwise a molecule converges when |E - E_prev| < tol and
- * stops at max_its iterations.  dJK in FP64. */
+ * stops at max_its iterations.  dJK in FP64.  d_hist (optional, [nmol][5]
+ * doubles): ring of the last five iteration energies, slot = iteration mod 5
+ * (the initial energy is iteration 0), for the convergence spread the QM1B
+ * rows record (std of the last five energies, pyscf_ipu/dft.py:631). */
 int pscf_batch_energy(int dtype, const void* d_mols, int nmol, const double* d_e_nuc, const double* d_D,
                       const double* d_Hcore, const double* d_dJK, const double* d_exc, const double* d_mo_energy,
-                      double* d_ctl, double* d_E, int init, double tol, int max_its, void* stream);
+                      double* d_ctl, double* d_E, double* d_hist, int init, double tol, int max_its, void* stream);
 
 /* ---- large molecules and the precision study ---------------------------------
  * For 6-31G molecules (nao 130-200) whose matrices exceed the LDS-resident

@@ -49,3 +49,27 @@ def shell_quartets_bruteforce(nbas: int):
                     if i >= j and k >= l and i * j >= k * l:
                         out.append((i, j, k, l))
     return out
+
+
+def einsum_inputs_bruteforce(bas: np.ndarray):
+    """(indices (num_calls, 8) = di, dj, dk, dl, i0, j0, k0, l0 and do_lists
+    (num_calls, 8)) of the shell-quartet loop of
+    pyscf_ipu/electron_repulsion/direct.py:1029-1110: an index image of a
+    block's origin is applied by the first quartet (in loop order) that
+    produces it, so every s1 element is counted once."""
+    nbas = bas.shape[0]
+    ao_loc = np.cumsum(np.concatenate([[0], (bas[:, 1] * 2 + 1) * bas[:, 3]])).astype(int)
+    seen = set()
+    idx, dol = [], []
+    for (i, j, k, l) in shell_quartets_bruteforce(nbas):
+        d = [ao_loc[s + 1] - ao_loc[s] for s in (i, j, k, l)]
+        i0, j0, k0, l0 = (ao_loc[s] for s in (i, j, k, l))
+        images = [(i0, j0, k0, l0), (i0, j0, l0, k0), (j0, i0, k0, l0), (j0, i0, l0, k0),
+                  (k0, l0, i0, j0), (k0, l0, j0, i0), (l0, k0, i0, j0), (l0, k0, j0, i0)]
+        do = []
+        for key in images:
+            do.append(key not in seen)
+            seen.add(key)
+        idx.append(d + [i0, j0, k0, l0])
+        dol.append(do)
+    return np.array(idx, dtype=np.int32), np.array(dol, dtype=np.int32)

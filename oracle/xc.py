@@ -10,10 +10,16 @@ reference takes them from jax.value_and_grad.
 rho[1:4] is half the density gradient, as the reference's exchange_correlation
 produces it (nanoDFT.py:109-112); it is doubled here like b3lyp.py:15.
 
-Pins: the LDA part against the closed-form Slater exchange; B88 against its
-gamma -> 0 limit (pure Slater); VWN and LYP derivatives against central finite
-differences; the whole against the GPU kernel's independently hand-derived
-derivatives.  Absolute parity with XCFun/PySCF values is unpinned (neither is
+Pins (tests/test_xc_pins.py): the LDA part against the closed-form Slater
+constant (3/4)(3/pi)^(1/3); B88 at zero gradient against spin-resolved Slater
+exchange and its gradient correction against Becke's published form in spin
+variables; VWN5 against the Perdew-Zunger fit of the same Ceperley-Alder data
+(within 0.5 mHa for r_s >= 1) and its high-density slope (1 - ln 2)/pi^2; LYP at
+zero gradient against its closed form; the mixing against a0 = 0.2, ax = 0.72,
+ac = 0.81; v_rho and v_norms of the whole and of each piece against central
+finite differences.  The GPU kernel's independently hand-derived derivatives
+are compared with this module (tests/test_boundary.py, tests/test_gpu_kernels.py).
+Pointwise agreement with XCFun/PySCF themselves is not checked (neither is
 installed here).
 """
 from __future__ import annotations

@@ -27,6 +27,9 @@ _SIGS = {
     "pscf_jk_sparse": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_void_p, c_int, c_double, c_void_p, c_void_p,
                                c_size_t, c_void_p]),
     "pscf_jk_s8_workspace_bytes": (c_size_t, [c_int, c_int]),
+    "pscf_jk_blocks_workspace_bytes": (c_size_t, [c_int, c_int, c_int64]),
+    "pscf_jk_blocks": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                               c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pscf_jk_s8": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pscf_ao_eval": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
                              c_void_p]),
@@ -70,7 +73,7 @@ _SIGS = {
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_int, c_void_p]),
     "pscf_batch_energy": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
     # large molecules and the precision study
     "pscf_eri_s8_to_s1": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "pscf_jk_s1": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p]),

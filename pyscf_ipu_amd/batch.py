@@ -29,6 +29,7 @@ from . import _lib, device, grids, gto
 
 DIIS_N = 9
 CTL = 8
+ENERGY_HIST = 5  # iteration energies kept per molecule (pscf_batch_energy's ring)
 SPLITS = (1, 2, 4, 8)   # workgroups per molecule tried by the autotuner
 MAX_SPLIT = max(SPLITS)
 MIN_WORKGROUPS = 512  # a small class of molecules is split further, to fill the 256 CUs
@@ -328,6 +329,8 @@ class BatchRKS:
         self.H, self.C = device.empty(nm, f64), device.empty(nm, f64)
         self.mo_e = device.empty(S["vec"], f64)
         self.E = device.empty((self.M, 5), f64)
+        # the last ENERGY_HIST iteration energies of each molecule (ring, slot = iteration % ENERGY_HIST)
+        self.E_hist = device.zeros((self.M, ENERGY_HIST), f64)
         self.state = device.zeros(S["state"], f64)
         self._keep = [gw, ae, ap0, am, toff, tn, tg] + works
         torch.cuda.synchronize()
@@ -403,8 +406,8 @@ class BatchRKS:
     def _energy(self, init, tol, max_its):
         P, st = device.ptr, device.stream()
         _lib.call("pscf_batch_energy", self.code, P(self.d_desc), self.M, P(self.d_enuc), P(self.D), P(self.Hc),
-                  P(self.dJK), P(self.exc), P(self.mo_e), P(self.ctl), P(self.E), int(init), float(tol), int(max_its),
-                  st)
+                  P(self.dJK), P(self.exc), P(self.mo_e), P(self.ctl), P(self.E), P(self.E_hist), int(init), float(tol),
+                  int(max_its), st)
 
     def _iteration(self, tol, max_its, diis=True):
         P, st = device.ptr, device.stream()
@@ -451,10 +454,15 @@ class BatchRKS:
         """Prepares molecules (indices) for another `run` from the atomic guess,
         in place: their density, dJK accumulator, DIIS / warm-start state and
         control words are reset, and every other molecule stays inactive.
-        Grids, integrals, ERIs and the tuned tiling are kept."""
+        Grids, integrals, ERIs and the tuned tiling are kept.  A molecule whose
+        set-up failed (ctl[m, 2] < 0: S not positive definite, or over-size)
+        has no valid L^-1 and cannot be restarted: ValueError."""
         idx = np.unique(np.asarray(molecules, dtype=np.int64))
         if len(idx) == 0:
             return
+        failed = idx[self.ctl[device.to_dev(idx), 2].cpu().numpy() < 0]
+        if len(failed):
+            raise ValueError(f"restart: molecules {failed.tolist()[:8]} failed set-up and cannot be restarted")
         P, st = device.ptr, device.stream()
         nm = self.sizes["mat"]
         D0, Dt0 = device.empty(nm, torch.float64), device.empty(nm, self.tdt)
@@ -469,6 +477,7 @@ class BatchRKS:
         self.D[e] = D0[e]
         self.Dt[e] = Dt0[e]
         self.dJK[e] = 0.0
+        self.E_hist[device.to_dev(idx)] = 0.0
         self.state[device.to_dev(spans(self.desc[idx, 15], state_doubles(self.nao[idx])))] = 0.0
         ctl = self.ctl.cpu().numpy()
         ctl[:, 1] = 0.0
@@ -478,6 +487,18 @@ class BatchRKS:
         torch.cuda.synchronize()
 
     # ---------------------------------------------------------------- accessors ---
+    def energy_history(self) -> np.ndarray:
+        """(M, ENERGY_HIST) the last iteration energies of each molecule in
+        iteration order (oldest first); NaN where a molecule ran fewer."""
+        h = self.E_hist.cpu().numpy()
+        its = self.ctl[:, 0].cpu().numpy().astype(np.int64)
+        out = np.full_like(h, np.nan)
+        for k in range(ENERGY_HIST):
+            it = its - (ENERGY_HIST - 1 - k)  # iteration number held in column k
+            ok = it >= 0
+            out[ok, k] = h[np.where(ok)[0], it[ok] % ENERGY_HIST]
+        return out
+
     def matrix(self, buf: torch.Tensor, m: int) -> np.ndarray:
         N, o = int(self.nao[m]), int(self.desc[m, 12])
         return buf[o:o + N * N].reshape(N, N).cpu().numpy()

@@ -29,5 +29,7 @@ struct BatchMol {
 // [0] iteration, [1] active, [2] converged, [3] E_prev, [4] E_last, [5] gap,
 // [6] iterations used, [7] spare.
 constexpr int CTL = 8;
+// iteration energies kept per molecule by pscf_batch_energy (the last ones)
+constexpr int BATCH_HIST = 5;
 
 }  // namespace pscf

@@ -510,6 +510,136 @@ static int jk_sparse_impl(const T* eri, const int16_t* idx, int64_t nnz, const T
   return PSCF_OK;
 }
 
+// ------------------------------------------------------ block-wise direct ---
+// vj, vk of ipu_einsum (pyscf_ipu/electron_repulsion/direct.py:489-504; the
+// poplar_direct_s1* vertices, int2e_sph.cpp:7598-8176): the ERI blocks of the
+// shell-quartet list, grouped by block size as electron_repulsion_integrals
+// returns them, each block (di, dj, dk, dl) in libcint's column-major order
+// with origin (i0, j0, k0, l0) and an 8-bit mask of the index permutations it
+// applies (the first occurrence of each permuted origin, so every s1 element
+// is counted once).  Element-parallel: thread e finds its block from the
+// uniform-size classes, decodes (ci, cj, ck, cl) and adds its up to 8 J and 8 K
+// contributions into LDS accumulators; per-workgroup partials are summed in a
+// fixed order (reduce_parts).
+constexpr int JKD_MAX_CLASSES = 8;
+struct BlockClasses {
+  int64_t elem0[JKD_MAX_CLASSES + 1];  // first element of class c; elem0[n] = total
+  int blk0[JKD_MAX_CLASSES];           // first block of class c
+  int size[JKD_MAX_CLASSES];           // elements per block of class c
+  int n;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(JK_THREADS) jk_blocks_kernel(const T* __restrict__ eri,
+                                                               const int4* __restrict__ blk, BlockClasses cls,
+                                                               const T* __restrict__ dm_g, int N,
+                                                               T* __restrict__ part_j, T* __restrict__ part_k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int NN = N * N;
+  T* dm = reinterpret_cast<T*>(smem);
+  T* vj = dm + NN;
+  T* vk = vj + NN;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+    dm[x] = dm_g[x];
+    vj[x] = T(0);
+    vk[x] = T(0);
+  }
+  __syncthreads();
+  const int64_t total = cls.elem0[cls.n];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    int c = 0;
+    while (c + 1 < cls.n && e >= cls.elem0[c + 1]) ++c;
+    const int size = cls.size[c];
+    const int64_t rel = e - cls.elem0[c];
+    const int64_t q = rel / size;
+    int loc = (int)(rel - q * size);
+    const int4 d = blk[cls.blk0[c] + q];
+    const int di = d.x & 15, dj = (d.x >> 4) & 15, dk = (d.x >> 8) & 15;
+    const int mask = (d.x >> 16) & 255;
+    const int ci = loc % di;
+    loc /= di;
+    const int cj = loc % dj;
+    loc /= dj;
+    const int ck = loc % dk;
+    const int cl = loc / dk;
+    const int i = (d.y & 0xffff) + ci, j = (int)((unsigned)d.y >> 16) + cj;
+    const int k = (d.z & 0xffff) + ck, l = (int)((unsigned)d.z >> 16) + cl;
+    const T v = eri[e];
+    // the eight index images (i j | k l) -> targets as the reference's vertex
+    if (mask & 1) {
+      atomicAdd(vj + k * N + l, dm[j * N + i] * v);
+      atomicAdd(vk + i * N + l, dm[j * N + k] * v);
+    }
+    if (mask & 2) {
+      atomicAdd(vj + l * N + k, dm[j * N + i] * v);
+      atomicAdd(vk + i * N + k, dm[j * N + l] * v);
+    }
+    if (mask & 4) {
+      atomicAdd(vj + k * N + l, dm[i * N + j] * v);
+      atomicAdd(vk + j * N + l, dm[i * N + k] * v);
+    }
+    if (mask & 8) {
+      atomicAdd(vj + l * N + k, dm[i * N + j] * v);
+      atomicAdd(vk + j * N + k, dm[i * N + l] * v);
+    }
+    if (mask & 16) {
+      atomicAdd(vj + i * N + j, dm[l * N + k] * v);
+      atomicAdd(vk + k * N + j, dm[l * N + i] * v);
+    }
+    if (mask & 32) {
+      atomicAdd(vj + j * N + i, dm[l * N + k] * v);
+      atomicAdd(vk + k * N + i, dm[l * N + j] * v);
+    }
+    if (mask & 64) {
+      atomicAdd(vj + i * N + j, dm[k * N + l] * v);
+      atomicAdd(vk + l * N + j, dm[k * N + i] * v);
+    }
+    if (mask & 128) {
+      atomicAdd(vj + j * N + i, dm[k * N + l] * v);
+      atomicAdd(vk + l * N + i, dm[k * N + j] * v);
+    }
+  }
+  __syncthreads();
+  T* dj_ = part_j + (int64_t)blockIdx.x * NN;
+  T* dk_ = part_k + (int64_t)blockIdx.x * NN;
+  for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+    dj_[x] = vj[x];
+    dk_[x] = vk[x];
+  }
+}
+
+static int jk_blocks_nb(int64_t nelem) {
+  const int64_t want = (nelem + JK_THREADS * 16 - 1) / (JK_THREADS * 16);
+  return (int)(want < 1 ? 1 : (want > JK_MAX_BLOCKS ? JK_MAX_BLOCKS : want));
+}
+
+// workspace: nb vj partials, nb vk partials, then the chunk sums of reduce_parts
+static size_t jk_blocks_ws_bytes(size_t es, int N, int64_t nelem) {
+  const int nb = jk_blocks_nb(nelem);
+  return 2 * red_align(es * (size_t)nb * N * N) + es * (size_t)red_chunks(nb) * N * N;
+}
+
+template <typename T>
+static int jk_blocks_impl(const T* eri, const int4* blk, const BlockClasses& cls, const T* dm, int N, T* vj, T* vk,
+                          void* ws, size_t ws_bytes, hipStream_t st) {
+  const int64_t NN = (int64_t)N * N;
+  const int64_t nelem = cls.elem0[cls.n];
+  const int nb = jk_blocks_nb(nelem);
+  const size_t lds = 3 * sizeof(T) * (size_t)NN;
+  PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_jk_blocks: nao %d too large for the LDS accumulators", N);
+  PSCF_REQUIRE(ws && ws_bytes >= jk_blocks_ws_bytes(sizeof(T), N, nelem), "pscf_jk_blocks: workspace too small");
+  T* pj = (T*)ws;
+  T* pk = (T*)((char*)ws + red_align(sizeof(T) * (size_t)nb * NN));
+  T* lvl = (T*)((char*)ws + 2 * red_align(sizeof(T) * (size_t)nb * NN));
+  PSCF_SET_LDS((jk_blocks_kernel<T>), lds);
+  jk_blocks_kernel<T><<<nb, JK_THREADS, lds, st>>>(eri, blk, cls, dm, N, pj, pk);
+  PSCF_HIP(hipGetLastError());
+  int rc = reduce_parts<T, false>(pj, nb, N, lvl, vj, nullptr, nullptr, st);
+  if (rc != PSCF_OK) return rc;
+  return reduce_parts<T, false>(pk, nb, N, lvl, vk, nullptr, nullptr, st);
+}
+
 static int jk_s8_blocks(int N) {
   const int64_t P = (int64_t)N * (N + 1) / 2;
   int64_t waves = P;  // at most one wave per row
@@ -572,6 +702,40 @@ extern "C" int pscf_jk_sparse(int dtype, const void* d_eri, const int16_t* d_idx
                                   ws_bytes, st);
   return jk_sparse_impl<float>((const float*)d_eri, d_idx, nnz, (const float*)d_dm, N, hyb, (float*)d_out, d_ws,
                                ws_bytes, st);
+}
+
+extern "C" size_t pscf_jk_blocks_workspace_bytes(int dtype, int N, int64_t nelem) {
+  return jk_blocks_ws_bytes(dtype == PSCF_F64 ? 8 : 4, N, nelem);
+}
+
+extern "C" int pscf_jk_blocks(int dtype, const void* d_eri, const int* d_blocks, int nclass, const int64_t* h_class_elem0,
+                              const int* h_class_blk0, const int* h_class_size, const void* d_dm, int N, void* d_vj,
+                              void* d_vk, void* d_ws, size_t ws_bytes, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_jk_blocks: bad dtype %d", dtype);
+  PSCF_REQUIRE(N > 0 && N <= 0xffff, "pscf_jk_blocks: N=%d out of range", N);
+  PSCF_REQUIRE(nclass >= 1 && nclass <= JKD_MAX_CLASSES, "pscf_jk_blocks: %d block classes (1..%d)", nclass,
+               JKD_MAX_CLASSES);
+  PSCF_REQUIRE(h_class_elem0 && h_class_blk0 && h_class_size && d_dm && d_vj && d_vk, "pscf_jk_blocks: null pointer");
+  BlockClasses cls{};
+  cls.n = nclass;
+  for (int c = 0; c < nclass; ++c) {
+    PSCF_REQUIRE(h_class_size[c] >= 1 && h_class_size[c] <= 15 * 15 * 15 * 15 && h_class_blk0[c] >= 0 &&
+                     h_class_elem0[c + 1] >= h_class_elem0[c] && h_class_elem0[0] == 0,
+                 "pscf_jk_blocks: bad class %d", c);
+    PSCF_REQUIRE((h_class_elem0[c + 1] - h_class_elem0[c]) % h_class_size[c] == 0,
+                 "pscf_jk_blocks: class %d is not a whole number of blocks", c);
+    cls.elem0[c] = h_class_elem0[c];
+    cls.blk0[c] = h_class_blk0[c];
+    cls.size[c] = h_class_size[c];
+  }
+  cls.elem0[nclass] = h_class_elem0[nclass];
+  PSCF_REQUIRE(cls.elem0[nclass] == 0 || (d_eri && d_blocks), "pscf_jk_blocks: null ERI or block table");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    return jk_blocks_impl<double>((const double*)d_eri, (const int4*)d_blocks, cls, (const double*)d_dm, N,
+                                  (double*)d_vj, (double*)d_vk, d_ws, ws_bytes, st);
+  return jk_blocks_impl<float>((const float*)d_eri, (const int4*)d_blocks, cls, (const float*)d_dm, N, (float*)d_vj,
+                               (float*)d_vk, d_ws, ws_bytes, st);
 }
 
 extern "C" size_t pscf_jk_s8_workspace_bytes(int dtype, int N) {

@@ -999,7 +999,8 @@ __global__ void __launch_bounds__(256) energy_batch_kernel(const BatchMol* __res
                                                            const double* __restrict__ dJK,
                                                            const double* __restrict__ exc,
                                                            const double* __restrict__ mo_e, double* __restrict__ ctl,
-                                                           double* __restrict__ E, int init, double tol, int max_its) {
+                                                           double* __restrict__ E, double* __restrict__ hist, int init,
+                                                           double tol, int max_its) {
   __shared__ double red[16];
   const int m = blockIdx.x;
   if (!init && ctl[m * CTL + 1] == 0.0) return;
@@ -1030,6 +1031,10 @@ __global__ void __launch_bounds__(256) energy_batch_kernel(const BatchMol* __res
     Em[3] = ex;
     Em[4] = e_nuc[m];
     double* c = ctl + m * CTL;
+    // ring of the last BATCH_HIST iteration energies (slot = iteration mod
+    // BATCH_HIST; the initial energy is iteration 0; an init call on a
+    // molecule that has already iterated leaves its ring alone)
+    if (hist != nullptr && (!init || c[0] == 0.0)) hist[(int64_t)m * BATCH_HIST + ((init ? 0 : (int)c[0] + 1) % BATCH_HIST)] = tot;
     if (init) {
       c[3] = tot;
       c[4] = tot;
@@ -1207,18 +1212,18 @@ extern "C" int pscf_batch_purify(int dtype, const void* d_mols, const int* d_ord
 
 extern "C" int pscf_batch_energy(int dtype, const void* d_mols, int nmol, const double* d_e_nuc, const double* d_D,
                                  const double* d_Hcore, const double* d_dJK, const double* d_exc,
-                                 const double* d_mo_energy, double* d_ctl, double* d_E, int init, double tol,
-                                 int max_its, void* stream) {
+                                 const double* d_mo_energy, double* d_ctl, double* d_E, double* d_hist, int init,
+                                 double tol, int max_its, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_energy: bad dtype");
   PSCF_REQUIRE(nmol > 0 && d_mols && d_e_nuc && d_D && d_Hcore && d_dJK && d_exc && d_mo_energy && d_ctl && d_E,
                "pscf_batch_energy: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   if (dtype == PSCF_F64)
     energy_batch_kernel<double><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_e_nuc, d_D, d_Hcore, d_dJK, d_exc,
-                                                      d_mo_energy, d_ctl, d_E, init, tol, max_its);
+                                                      d_mo_energy, d_ctl, d_E, d_hist, init, tol, max_its);
   else
     energy_batch_kernel<float><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_e_nuc, d_D, d_Hcore, d_dJK, d_exc,
-                                                     d_mo_energy, d_ctl, d_E, init, tol, max_its);
+                                                     d_mo_energy, d_ctl, d_E, d_hist, init, tol, max_its);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }

@@ -4,6 +4,8 @@ Kept signatures (pyscf_ipu/electron_repulsion/direct.py):
   prepare_int_floats(mol)                                   :710-739
   prepare_integrals_2_inputs(mol) == prepare_electron_repulsion_integrals  :741-946
   electron_repulsion_integrals(input_floats, input_ints, mol, num_threads=3, v=1)  :513-515
+  prepare_einsum_inputs(mol)                                :1029-1110
+  ipu_einsum(__out3, dm, mol, threads=1, v=2) -> (vj, vk)   :500-504
 plus the packed-s8 fast path used by the SCF driver (`eri_s8`).
 
 The integer work list of `prepare_integrals_2_inputs` is reproduced exactly,
@@ -177,6 +179,89 @@ def electron_repulsion_integrals(input_floats, input_ints, mol, num_threads: int
         out.append(flat[s:s + size * count].view(count, size))
         s += size * count
     return out
+
+
+def prepare_einsum_inputs(mol):
+    """(tuple_indices, tuple_do_lists, N, num_calls) as the reference
+    (direct.py:1029-1110): per quartet of `quartet_list` (loop order)
+    (di, dj, dk, dl, i0, j0, k0, l0) and the 8 flags of the index images of the
+    block origin that this quartet applies (the first quartet, in loop order,
+    and the first image within it, that produces an origin applies it)."""
+    ind, do = _einsum_tables(mol._bas)
+    return tuple(map(tuple, ind.tolist())), tuple(map(tuple, do.tolist())), mol.nao_nr(), len(ind)
+
+
+def _einsum_tables(bas: np.ndarray):
+    ao_loc = _ao_loc(bas).astype(np.int64)
+    q = quartet_list(bas.shape[0]).astype(np.int64)
+    d = ao_loc[q + 1] - ao_loc[q]
+    o = ao_loc[q]
+    i0, j0, k0, l0 = o[:, 0], o[:, 1], o[:, 2], o[:, 3]
+    images = [(i0, j0, k0, l0), (i0, j0, l0, k0), (j0, i0, k0, l0), (j0, i0, l0, k0),
+              (k0, l0, i0, j0), (k0, l0, j0, i0), (l0, k0, i0, j0), (l0, k0, j0, i0)]
+    # origins < 2^15: one int64 key per image; flattened in (quartet, image) order
+    keys = np.stack([(a << 45) | (b << 30) | (c << 15) | e for a, b, c, e in images], axis=1).reshape(-1)
+    _, first = np.unique(keys, return_index=True)
+    do = np.zeros(keys.shape[0], np.int32)
+    do[first] = 1
+    return np.concatenate([d, o], axis=1).astype(np.int32), do.reshape(-1, 8)
+
+
+def ipu_einsum(__out3, dm, mol, threads: int = 1, v: int = 2):
+    """(vj, vk) with vj[k, l] = sum (ij|kl) dm[j, i] and vk[i, l] = sum (ij|kl)
+    dm[j, k] (the einsums 'ijkl,ji->kl' / 'ijkl,jk->il'), from the ERI blocks of
+    `electron_repulsion_integrals` (one (count, size) array per block size, in
+    the order of prepare_electron_repulsion_integrals), block-wise in one launch
+    (`pscf_jk_blocks`).  `threads` and `v` select IPU tile variants in the
+    reference and are accepted for signature compatibility.  Arithmetic in the
+    ERI blocks' dtype; returns device tensors, or numpy arrays for a numpy dm."""
+    is_np = not isinstance(dm, torch.Tensor)
+    N = mol.nao_nr()
+    if N > 0xffff:
+        raise ValueError("ipu_einsum: nao above 65535")
+    ind, do = _einsum_tables(mol._bas)
+    *_, indxs, _, _ = prepare_electron_repulsion_integrals(mol)
+    ind, do = ind[indxs], do[indxs]
+    sizes = np.prod(ind[:, :4].astype(np.int64), axis=1)
+    parts = [p if isinstance(p, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(p)) for p in __out3]
+    dt = torch.float64 if parts[0].dtype == torch.float64 else torch.float32
+    code = _lib.F64 if dt == torch.float64 else _lib.F32
+    counts = [int(p.shape[0]) for p in parts]
+    if sum(counts) != len(ind):
+        raise ValueError(f"ipu_einsum: {sum(counts)} ERI blocks given, the molecule has {len(ind)}")
+    cls_size, cls_blk0, cls_elem0, b0, e0 = [], [], [0], 0, 0
+    for p, c in zip(parts, counts):
+        if c == 0:
+            continue
+        sz = int(sizes[b0])
+        if not np.all(sizes[b0:b0 + c] == sz) or p.numel() != c * sz:
+            raise ValueError("ipu_einsum: ERI blocks do not follow the molecule's block-size classes")
+        cls_size.append(sz)
+        cls_blk0.append(b0)
+        b0 += c
+        e0 += c * sz
+        cls_elem0.append(e0)
+    eri = torch.cat([device.to_dev(p.reshape(-1), dt) for p in parts if p.shape[0] > 0])
+    mask = (do << np.arange(8)).sum(axis=1).astype(np.int64)
+    d = ind[:, :4].astype(np.int64)
+    o = ind[:, 4:].astype(np.int64)
+    tab = np.zeros((len(ind), 4), np.int64)
+    tab[:, 0] = d[:, 0] | (d[:, 1] << 4) | (d[:, 2] << 8) | (d[:, 3] << 12) | (mask << 16)
+    tab[:, 1] = o[:, 0] | (o[:, 1] << 16)
+    tab[:, 2] = o[:, 2] | (o[:, 3] << 16)
+    tab = device.to_dev(tab.astype(np.uint32).view(np.int32))
+    dmt = device.to_dev(dm if not is_np else np.asarray(dm), dt).reshape(N, N).contiguous()
+    vj, vk = device.empty((N, N), dt), device.empty((N, N), dt)
+    nc = len(cls_size)
+    wsb = _lib.lib().pscf_jk_blocks_workspace_bytes(code, N, e0)
+    ws = device.workspace(wsb)
+    _lib.call("pscf_jk_blocks", code, device.ptr(eri), device.ptr(tab), nc,
+              (ctypes.c_int64 * (nc + 1))(*cls_elem0), (ctypes.c_int * nc)(*cls_blk0),
+              (ctypes.c_int * nc)(*cls_size), device.ptr(dmt), N, device.ptr(vj), device.ptr(vk), device.ptr(ws),
+              wsb, device.stream())
+    if is_np:
+        return vj.cpu().numpy(), vk.cpu().numpy()
+    return vj, vk
 
 
 def shell_pairs_by_class(mol):

@@ -8,8 +8,11 @@ ANO minimal basis, projected onto the molecular basis.  The ANO basis data is
 not available here, so `minao` keeps the occupation rule and replaces the ANO
 functions by the molecule's own atom-centred shells, orthonormalised within each
 atom and angular momentum (Gram-Schmidt in basis order).  The projection step
-is then the identity.  Only SCF iteration counts depend on the guess; the
-converged energy does not.  Parity with PySCF's MINAO density is unpinned.
+is then the identity.  Parity with PySCF's MINAO density is unpinned.  For
+most molecules the guess changes only the SCF iteration count; a molecule with
+a small HOMO-LUMO gap can have several SCF solutions (measured: 1-31 mHa apart),
+and there the start decides which one is reached, so which solution such a
+molecule reaches from PySCF's MINAO start is not pinned either.
 """
 from __future__ import annotations
 

@@ -269,10 +269,13 @@ def learned_guess(model: PairMLP):
 # 26 of 4096 generator molecules (seed 1) converged to a solution 1-18 mHa away
 # from the MINAO-started one (above it for some, below for others), every one
 # with a converged gap below 0.06 Ha.  The converged result is therefore checked:
-# a gap below GAP_RECHECK sends the molecule back to the MINAO start, the
-# reference's own guess (pyscf_ipu/nanoDFT/nanoDFT.py:233), so that the state
-# reached is the reference's.  0.065 Ha leaves a margin over the largest gap
-# seen among those cases; 17% of the molecules of that set fall below it.
+# a gap below GAP_RECHECK sends the molecule back to the atomic-density start
+# (guess.minao, the stand-in for the reference's MINAO guess,
+# pyscf_ipu/nanoDFT/nanoDFT.py:233), so that learned and atomic starts reach the
+# same state.  That state is the one the stand-in start leads to; whether PySCF's
+# own MINAO start reaches the same solution for such molecules is not known here.
+# 0.065 Ha leaves a margin over the largest gap seen among those cases; 17% of
+# the molecules of that set fall below it.
 GAP_RECHECK = 0.065
 
 

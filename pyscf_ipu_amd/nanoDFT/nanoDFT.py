@@ -11,11 +11,12 @@ Kept names and meanings (pyscf_ipu/nanoDFT/nanoDFT.py):
   main()                                                        :620-643
 
 Per molecule everything after the host set-up runs in the library's kernels:
-int1e, ERIs (packed s8), AO values, Becke weights, and per iteration the fused
-XC kernel, the s8 J/K kernel, the one-workgroup SCF step (DIIS + Jacobi eigh
-+ density) and the FP64 energy reduction.  The host keeps the grid templates,
-the Cholesky of S and the initial guess, and reads one energy per iteration
-to stop at |dE| < conv_tol (the reference always runs `its` iterations).
+int1e, the Cholesky of S and L^-1, the atomic-density guess, ERIs (packed s8),
+AO values, Becke weights, and per iteration the fused XC kernel, the s8 J/K
+kernel, the one-workgroup SCF step (DIIS + Jacobi eigh + density) and the FP64
+energy reduction.  The host keeps the grid templates and reads one energy per
+iteration to stop at |dE| < conv_tol (the reference always runs `its`
+iterations).
 """
 from __future__ import annotations
 
@@ -77,12 +78,30 @@ def exchange_correlation(density_matrix, grid_AO, grid_weights, eps: float = EPS
 
 
 def get_JK(density_matrix, ERI, dense_ERI=False, backend="mi355x"):
-    """dJK = J - K/2 * HYB_B3LYP.  ERI: dense (N,N,N,N) when dense_ERI, else
-    (nonzero_distinct_ERI, nonzero_indices) as nanoDFT prepares them."""
+    """dJK = J - K/2 * HYB_B3LYP (nanoDFT.py:125-139).  dense_ERI: ERI is the
+    full (N, N, N, N) tensor (optionally with a leading axis of 1, as nanoDFT
+    stacks it) and J = einsum('ijkl,ji->kl'), K = einsum('ijkl,jk->il') run in
+    one streaming pass over it (`pscf_jk_s1`, FP64 accumulation; uses the 8-fold
+    symmetry of real ERIs, any dm).  Otherwise ERI = (nonzero_distinct_ERI,
+    nonzero_indices) as nanoDFT prepares them (`sparse_symmetric_einsum`).
+    Returns (N, N) in the density's type: a device tensor, or numpy for numpy."""
     N = density_matrix.shape[0]
     if dense_ERI:
-        raise NotImplementedError("dense_ERI: the MI355X path keeps ERIs in packed s8 form; "
-                                  "pass (values, indices) or use the s8 driver")
+        is_np = not isinstance(density_matrix, torch.Tensor)
+        eri = ERI if isinstance(ERI, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(ERI))
+        if eri.numel() != N ** 4:
+            raise ValueError(f"dense ERI has {eri.numel()} elements, expected N^4 = {N ** 4}")
+        dt = _dt(eri)
+        code = _lib.F64 if dt == torch.float64 else _lib.F32
+        eri = device.to_dev(eri.reshape(-1), dt)
+        dm = device.to_dev(density_matrix if not is_np else np.asarray(density_matrix), torch.float64).reshape(N, N)
+        out = device.empty((N, N), torch.float64)
+        _lib.call("pscf_jk_s1", code, device.ptr(eri), N, device.ptr(dm), HYB_B3LYP, device.ptr(out),
+                  device.stream())
+        odt = np.asarray(density_matrix).dtype if is_np else density_matrix.dtype
+        if is_np:
+            return out.cpu().numpy().astype(odt)
+        return out.to(odt)
     out = ssE.sparse_symmetric_einsum(ERI[0], ERI[1], density_matrix, backend)
     return out.reshape(N, N)
 
@@ -323,6 +342,41 @@ class RKS:
                          self.D.cpu().numpy(), len(log), converged, {"setup_s": t1 - t0, "scf_s": t2 - t1})
 
 
+class DenseRKS(RKS):
+    """RKS with the dense s1 ERI tensor (nanoDFT's `--dense_ERI`,
+    nanoDFT.py:275-281, get_JK :128-131): the packed s8 ERIs are expanded once
+    to N^4 values (`pscf_eri_s8_to_s1`) and dJK is one streaming pass over them
+    per iteration (`pscf_jk_s1`, FP64 accumulation).  Everything else is RKS."""
+
+    NATIVE_LOOP = False
+
+    def setup(self, dm0=None):
+        t = super().setup(dm0)
+        N = self.N
+        self.s1 = device.empty(N ** 4, self.tdt)
+        _lib.call("pscf_eri_s8_to_s1", self.code, device.ptr(self.eri), self.code, device.ptr(self.s1), N,
+                  device.stream())
+        self.dJK64 = self.dJK if self.tdt == torch.float64 else device.empty((N, N), torch.float64)
+        torch.cuda.current_stream().synchronize()
+        return t
+
+    def fock_parts(self):
+        N = self.N
+        if self.tdt != torch.float64:
+            _lib.call("pscf_cast", _lib.F64, device.ptr(self.D), self.code, device.ptr(self.Dt), N * N,
+                      device.stream())
+        with _timed(self.events, "xc"):
+            _lib.call("pscf_xc", self.code, device.ptr(self.ao), N * self.g, self.g, 1, self.g, N,
+                      device.ptr(self.Dt), device.ptr(self.weights), EPSILON_B3LYP, device.ptr(self.exc),
+                      device.ptr(self.Vxc), device.ptr(self.ws_xc), self.ws_xc_b, device.stream())
+        with _timed(self.events, "jk_s1"):
+            _lib.call("pscf_jk_s1", self.code, device.ptr(self.s1), N, device.ptr(self.D), HYB_B3LYP,
+                      device.ptr(self.dJK64), device.stream())
+            if self.dJK64 is not self.dJK:
+                _lib.call("pscf_cast", _lib.F64, device.ptr(self.dJK64), self.code, device.ptr(self.dJK), N * N,
+                          device.stream())
+
+
 def _pinned(n: int) -> torch.Tensor:
     return torch.empty(n, dtype=torch.float64, pin_memory=True)
 
@@ -373,11 +427,13 @@ def run_molecules(mols, streams: int = 4, level: int = 0, dtype: str = "fp64", i
 
 
 def nanoDFT(mol, opts):
-    """Reference-shaped entry point: returns (energies (n, 5) in eV,
-    (logged_energies eV, hlgaps eV, mo_energy, mo_coeff, grid_coords, grid_weights))."""
+    """Reference-shaped entry point (nanoDFT.py:259-336): returns (energies
+    (n, 5) in eV, (logged_energies eV, hlgaps eV, mo_energy, mo_coeff,
+    grid_coords, grid_weights)); n = opts.its unless opts.conv_tol stops
+    earlier.  opts.dense_ERI selects the s1 ERI tensor (DenseRKS)."""
     dtype = "fp32" if getattr(opts, "float32", False) else "fp64"
     conv = getattr(opts, "conv_tol", 0.0)
-    rks = RKS(mol, level=opts.level, dtype=dtype)
+    rks = (DenseRKS if getattr(opts, "dense_ERI", False) else RKS)(mol, level=opts.level, dtype=dtype)
     res = rks.run(its=opts.its, conv_tol=conv if conv else -1.0, diis=opts.diis)
     energies = res.energies * HARTREE_TO_EV
     grid_coords = rks.coords.cpu().numpy()

@@ -52,3 +52,38 @@ def test_eri_work_covers_all_quartets():
     nsh = b.desc[:, 1]
     npair = nsh * (nsh + 1) // 2
     assert tot == int((npair * (npair + 1) // 2).sum())
+
+
+def test_qm1b_rows_roundtrip_parquet(tmp_path):
+    """QM1B-schema rows (qm1b/README.md:7-18) from plain-form results: unit
+    conversions, column types and a Parquet round trip."""
+    from pyscf_ipu_amd import qm1b
+    from pyscf_ipu_amd.gto import METHANE, build_mol
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import HARTREE_TO_EV, SCFResult
+    mol = build_mol(METHANE)
+    N = mol.nao_nr()
+    E = np.array([-40.1, -40.2, -40.25, -40.26, -40.261, -40.2611])
+    energies = np.zeros((6, 5))
+    energies[:, 0] = E
+    mo = np.linspace(-10, 1, N)
+    r = SCFResult(E[-1], energies, np.zeros(6), mo, np.eye(N), np.eye(N), 6, True)
+    rows = qm1b.rows_from_plain([mol], [r], smiles=["C"])
+    assert len(rows) == 1
+    row = rows[0]
+    assert row["atoms"] == "CHHHH" and row["z"] == [6, 1, 1, 1, 1] and row["N"] == N
+    assert abs(row["energy"] - E[-1] * HARTREE_TO_EV) < 1e-9
+    assert abs(row["homo"] - mo[4] * HARTREE_TO_EV) < 1e-9 and abs(row["lumo"] - mo[5] * HARTREE_TO_EV) < 1e-9
+    assert abs(row["y"] - (mo[5] - mo[4]) * HARTREE_TO_EV) < 1e-9
+    assert abs(row["std"] - np.std(E[-5:] * HARTREE_TO_EV)) < 1e-12
+    assert np.allclose(np.array(row["pos"]).reshape(-1, 3), mol.atom_coords())
+    assert qm1b.formula([6, 1, 1, 1, 1, 8]) == "CH4O"
+    p = tmp_path / "qm1b.parquet"
+    qm1b.write_parquet(rows * 3, str(p))
+    df = qm1b.read_parquet(str(p))
+    assert list(df.columns) == list(qm1b.COLUMNS) and len(df) == 3
+    assert df["smile"][0] == "C" and list(df["z"][2]) == [6, 1, 1, 1, 1]
+    assert abs(df["energy"][1] - row["energy"]) == 0.0
+    # a molecule that did not converge is left out unless asked for
+    r.converged = False
+    assert qm1b.rows_from_plain([mol], [r]) == []
+    assert qm1b.rows_from_plain([mol], [r], only_converged=False)[0]["smile"] == "CH4"

@@ -73,6 +73,10 @@ def test_batch_restart_in_place(hip, dtype, tol):
     assert np.abs(again.energy[redo] - fresh.energy[redo]).max() < tol
     others = np.setdiff1d(np.arange(len(mols)), redo)
     assert np.array_equal(again.iterations[others], first.iterations[others])
+    # a molecule whose set-up failed (no valid L^-1) is refused, not revived
+    b.ctl[5, 2] = -1.0
+    with pytest.raises(ValueError):
+        b.restart(np.array([2, 5]))
 
 
 def test_batch_matches_plain_path(hip):
@@ -168,3 +172,27 @@ def test_batch_jk_row_and_column_forms(hip, dtype):
         for out in outs:
             got = out[o:o + N * N].cpu().numpy().reshape(N, N)
             assert np.abs(got - ref).max() < tol * max(1.0, np.abs(ref).max()), (m, np.abs(got - ref).max())
+
+
+def test_batch_qm1b_rows(hip, oracle_results, tmp_path):
+    """QM1B rows of a batched FP64 run (energy, HOMO, LUMO, gap in eV; std of
+    the last five iteration energies from the device ring) against the oracle
+    SCF of the same molecules, through a Parquet round trip."""
+    from pyscf_ipu_amd import qm1b
+    from pyscf_ipu_amd.nanoDFT.nanoDFT import HARTREE_TO_EV
+    b = batch.BatchRKS(batch.parse(STRS), dtype="fp64")
+    res = b.run(max_its=40, conv_tol=1e-10, graph=True)
+    rows = qm1b.rows_from_batch(b, res)
+    assert len(rows) == len(STRS)
+    path = str(tmp_path / "rows.parquet")
+    qm1b.write_parquet(rows, path)
+    df = qm1b.read_parquet(path)
+    for m, ref in enumerate(oracle_results):
+        nocc = build_mol(STRS[m]).nelectron // 2
+        assert abs(df["energy"][m] / HARTREE_TO_EV - ref.energy) < 1e-7
+        assert abs(df["homo"][m] / HARTREE_TO_EV - ref.mo_energy[nocc - 1]) < 1e-6
+        assert abs(df["lumo"][m] / HARTREE_TO_EV - ref.mo_energy[nocc]) < 1e-6
+        assert abs(df["y"][m] / HARTREE_TO_EV - ref.hlgap) < 1e-6
+        if ref.iterations >= 5 and res.iterations[m] == ref.iterations:
+            assert abs(df["std"][m] - np.std(ref.energies[-5:, 0] * HARTREE_TO_EV)) < 1e-5
+        assert df["N"][m] == build_mol(STRS[m]).nao_nr()

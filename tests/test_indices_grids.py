@@ -158,3 +158,17 @@ def test_grid_sizes_level0():
         c, v = grids.atomic_grid(z, 0)
         r = np.unique(np.round(np.linalg.norm(c, axis=1), 12))
         assert len(r) == nrad
+
+
+@pytest.mark.parametrize("mol_str", [gto.METHANE, "O 0 0 0; H 0 0.9 0.3; H 0 -0.9 0.3",
+                                     "C 0 0 0; N 1.4 0 0; O 0 1.3 0; H -1 0 0; H 0 0 1"])
+def test_einsum_do_lists_match_reference_loop(mol_str):
+    """prepare_einsum_inputs (vectorised first-occurrence flags) equals the
+    reference's dictionary loop (direct.py:1029-1110), bit for bit, and the
+    applied index images cover every ordered shell quartet exactly once."""
+    mol = gto.build_mol(mol_str)
+    ind, do, N, n = direct.prepare_einsum_inputs(mol)
+    ref_ind, ref_do = oidx.einsum_inputs_bruteforce(mol._bas)
+    assert np.array_equal(np.array(ind), ref_ind) and np.array_equal(np.array(do), ref_do)
+    assert N == mol.nao_nr() and n == len(ref_ind)
+    assert int(np.sum(do)) == mol.nbas ** 4
