@@ -155,7 +155,23 @@ __device__ inline void tri_split(int c, int& r, int& s) {
 // for it, and the group's reads would become a chain of round trips.
 constexpr int S8_GROUP = 4;
 
-template <typename T, bool PRESCALED, typename TA = T>
+// Order-independent FP64 accumulation for the batched forms: every contribution
+// is rounded to a multiple of 2^-JK_QBITS before it is added, so each FP64 sum
+// of such values (and its intermediate sums) is exact while its magnitude stays
+// below 2^(53 - JK_QBITS) = 8192 (J/K elements here are below 100).  Exact sums
+// do not depend on the order of the adds: the LDS and global atomics of
+// different waves and workgroups then give the same bits in every run.  The
+// rounding adds at most 2^-41 (4.5e-13) per contribution.
+constexpr int JK_QBITS = 40;
+__device__ __forceinline__ double jk_quant(double v) { return rint(v * 0x1p40) * 0x1p-40; }
+
+template <bool QUANT, typename TA>
+__device__ __forceinline__ TA jk_term(TA v) {
+  if constexpr (QUANT) return (TA)jk_quant((double)v);
+  return v;
+}
+
+template <typename T, bool PRESCALED, typename TA = T, bool QUANT = false>
 __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p, int q, const T* __restrict__ dm,
                                        TA* __restrict__ acc, int N, T hhalf, int lane) {
   const T dpq = dm[p * N + q];
@@ -197,9 +213,9 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
       const T wk = -hhalf * w;
       jrow += w2 * d_rs[u];
       if (valid) {
-        atomicAdd(acc + r[u] * N + s[u], (TA)(w2 * dpq));
-        atomicAdd(acc + p * N + s[u], (TA)(wk * d_qr[u]));
-        atomicAdd(acc + q * N + s[u], (TA)(wk * d_pr[u]));
+        atomicAdd(acc + r[u] * N + s[u], jk_term<QUANT>((TA)(w2 * dpq)));
+        atomicAdd(acc + p * N + s[u], jk_term<QUANT>((TA)(wk * d_qr[u])));
+        atomicAdd(acc + q * N + s[u], jk_term<QUANT>((TA)(wk * d_pr[u])));
       }
       T a = wk * d_qs[u], b = wk * d_ps[u];
       const int key = valid ? r[u] : -1;
@@ -207,14 +223,14 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
       // the run of this lane's r ends here: the next position starts a new r,
       // or the chunk or the row ends
       if (valid && (s[u] == r[u] || lane == 63 || rs == pq)) {
-        atomicAdd(acc + p * N + r[u], (TA)a);
-        atomicAdd(acc + q * N + r[u], (TA)b);
+        atomicAdd(acc + p * N + r[u], jk_term<QUANT>((TA)a));
+        atomicAdd(acc + q * N + r[u], jk_term<QUANT>((TA)b));
       }
     }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) jrow += __shfl_xor(jrow, off, 64);
-  if (lane == 0) atomicAdd(acc + p * N + q, (TA)jrow);
+  if (lane == 0) atomicAdd(acc + p * N + q, jk_term<QUANT>((TA)jrow));
 }
 
 // One wavefront per row pq of the packed lower triangle; lanes walk rs <= pq.
@@ -297,7 +313,7 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_batch_kernel(const Batc
   for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
     int p, q;
     tri_split(pq, p, q);
-    s8_row<T, true, double>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
+    s8_row<T, true, double, true>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
   const bool atomic = gridDim.y > 1;
@@ -424,19 +440,19 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
       }
       const T tot = transpose_reduce16(t, lane) * -hhalf;
       const int u = lane & 7;
-      if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, (double)tot);
+      if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, jk_quant((double)tot));
     }
     jrow = wave_sum_last(jrow) * T(2);
     if (lane < N) {
-      atomicAdd(acc + p * N + lane, (double)(kp * -hhalf));
-      atomicAdd(acc + q * N + lane, (double)(kq * -hhalf));
+      atomicAdd(acc + p * N + lane, jk_quant((double)(kp * -hhalf)));
+      atomicAdd(acc + q * N + lane, jk_quant((double)(kq * -hhalf)));
     }
-    if (lane == 63) atomicAdd(acc + p * N + q, (double)jrow);
+    if (lane == 63) atomicAdd(acc + p * N + q, jk_quant((double)jrow));
   }
   // (predicated rather than cut short, so the loop stays unrolled and jacc in registers)
 #pragma unroll
   for (int r = 0; r < JKC_MAXN; ++r)
-    if (r < N && lane <= r) atomicAdd(acc + r * N + lane, (double)jacc[r]);
+    if (r < N && lane <= r) atomicAdd(acc + r * N + lane, jk_quant((double)jacc[r]));
   __syncthreads();
   const bool atomic = gridDim.y > 1;
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {

@@ -37,14 +37,32 @@ def test_batch_fp64_matches_oracle(hip, oracle_results):
 
 
 def test_batch_fp32_within_1mha(hip, oracle_results):
-    # FP32 J/K accumulates with LDS atomics, so two FP32 runs differ by ~1e-5 Ha;
-    # each must stay within the 1 mHa gate of the FP64 oracle.
     mols = batch.parse(STRS)
     eager = batch.BatchRKS(mols, dtype="fp32").run(max_its=30, conv_tol=1e-7, graph=False)
     graphed = batch.BatchRKS(mols, dtype="fp32").run(max_its=30, conv_tol=1e-7, graph=True)
     for m, ref in enumerate(oracle_results):
         assert abs(eager.energy[m] - ref.energy) < 1e-3
         assert abs(graphed.energy[m] - ref.energy) < 1e-3
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp64"])
+def test_batch_runs_bit_identical(hip, dtype):
+    """Two runs of the same batch give the same bits: the J/K atomics add
+    contributions rounded to 2^-40, whose FP64 sums are exact in any order, and
+    every other reduction has a fixed order.  Different workgroup splits of the
+    J/K and XC launches give the same bits too."""
+    mols = batch.parse(STRS + molgen.batched_set(24, (5, 9), seed=33))
+    tol = 1e-6 if dtype == "fp32" else 1e-9
+    runs = []
+    for split, threads in ((1, 256), (1, 256), (4, 512)):
+        b = batch.BatchRKS(mols, dtype=dtype)
+        b.setup()
+        b.split, b.jk_threads = {"xc": 1, "jk": split}, threads
+        runs.append(b.run(max_its=30, conv_tol=tol, graph=True))
+    for r in runs[1:]:
+        assert np.array_equal(r.energy, runs[0].energy)
+        assert np.array_equal(r.iterations, runs[0].iterations)
+        assert np.array_equal(r.hlgap, runs[0].hlgap)
 
 
 def test_batch_graph_replay_equals_eager_fp64(hip):
