@@ -232,9 +232,14 @@ int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, const doubl
 /* E_xc (d_exc[m]) and V_xc of the active molecules d_order[0..nmol)
  * (nanoDFT.py:105-123), grid tiles of `tile` points (32, or 64 in FP32);
  * max_nao bounds the nao of these molecules (it sizes the LDS and the tile
- * shapes; a larger molecule gets NaN).  split workgroups share each molecule's
- * tiles; split > 1 needs part buffers of part_v_elems >= split * nmol *
- * max_nao^2 elements of dtype and part_e_elems >= split * nmol doubles. */
+ * shapes; a larger molecule gets NaN).  Each molecule's grid is cut into
+ * chunks of PSCF_XC_CHUNK_PTS points whose V and E_xc partials are summed in
+ * chunk order, so the result does not depend on `split` (workgroups sharing a
+ * molecule's chunks).  Chunk partials go to d_part_v (part_v_elems >=
+ * nchunk_max * nmol * max_nao^2 elements of dtype) and d_part_e (part_e_elems
+ * >= nchunk_max * nmol doubles), nchunk_max = ceil(largest grid /
+ * PSCF_XC_CHUNK_PTS); a molecule whose chunks do not fit gets NaN. */
+#define PSCF_XC_CHUNK_PTS 512
 int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                   const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
                   int split, int tile, void* d_part_v, double* d_part_e, int64_t part_v_elems, int64_t part_e_elems,

@@ -349,6 +349,7 @@ __device__ inline double lane_read(double x, int l) {
 
 constexpr int JKC_MAXN = 64;
 constexpr int JKC_GROUP = 8;  // r values whose ERI loads are issued together
+constexpr int JKC_VWAVES = 128;  // fixed row partition of a molecule (see the kernel)
 
 template <typename T, typename TO, bool ACC>
 __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const BatchMol* __restrict__ mols,
@@ -383,76 +384,85 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
   // then live in scalar registers and come from scalar loads
   const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) + blockIdx.y * nwaves);
   const int sl = lane < N ? lane : N - 1;  // in-bounds column for lanes past N (their values are masked)
+  // Rows go to JKC_VWAVES virtual waves (row pq to virtual wave pq mod
+  // JKC_VWAVES, in ascending order), and physical wave w runs virtual waves
+  // w, w + W, ... (W = waves per workgroup x workgroups per molecule).  Each
+  // virtual wave's register J sums therefore cover the same rows in the same
+  // order whatever W is, and their exact (quantized) adds make the result
+  // independent of the launch shape.
+  const int W = nwaves * gridDim.y;
   T jacc[JKC_MAXN];
+  for (int vw = wave; vw < JKC_VWAVES && vw < P; vw += W) {
 #pragma unroll
-  for (int r = 0; r < JKC_MAXN; ++r) jacc[r] = T(0);
-  for (int pq = wave; pq < P; pq += nwaves * gridDim.y) {
-    int p, q;
-    tri_split(pq, p, q);
-    p = __builtin_amdgcn_readfirstlane(p);
-    q = __builtin_amdgcn_readfirstlane(q);
-    const T* row = eri + tri(pq);
-    const T* gp = dm_g + p * N;
-    const T* gq = dm_g + q * N;
-    const T cj = T(2) * gp[q];
-    const T dp_s = lane < N ? dm[p * N + sl] : T(0);
-    const T dq_s = lane < N ? dm[q * N + sl] : T(0);
-    T kp = T(0), kq = T(0), jrow = T(0);
-    // row pq holds (r, s) for r < p, s <= r, and for r = p, s <= q.  In FP32
-    // the loads of group g + 8 are issued before group g is used (software
-    // pipeline); FP64 has no registers to spare for it.
-    constexpr bool PIPE = sizeof(T) == 4;
-    auto load_group = [&](int g0, T (&dst)[JKC_GROUP]) {
+    for (int r = 0; r < JKC_MAXN; ++r) jacc[r] = T(0);
+    for (int pq = vw; pq < P; pq += JKC_VWAVES) {
+      int p, q;
+      tri_split(pq, p, q);
+      p = __builtin_amdgcn_readfirstlane(p);
+      q = __builtin_amdgcn_readfirstlane(q);
+      const T* row = eri + tri(pq);
+      const T* gp = dm_g + p * N;
+      const T* gq = dm_g + q * N;
+      const T cj = T(2) * gp[q];
+      const T dp_s = lane < N ? dm[p * N + sl] : T(0);
+      const T dq_s = lane < N ? dm[q * N + sl] : T(0);
+      T kp = T(0), kq = T(0), jrow = T(0);
+      // row pq holds (r, s) for r < p, s <= r, and for r = p, s <= q.  In FP32
+      // the loads of group g + 8 are issued before group g is used (software
+      // pipeline); FP64 has no registers to spare for it.
+      constexpr bool PIPE = sizeof(T) == 4;
+      auto load_group = [&](int g0, T (&dst)[JKC_GROUP]) {
 #pragma unroll
-      for (int u = 0; u < JKC_GROUP; ++u) {
-        const int r = g0 + u;
-        const bool valid = r <= p && lane <= (r == p ? q : r);
-        dst[u] = valid ? row[r * (r + 1) / 2 + lane] : T(0);
+        for (int u = 0; u < JKC_GROUP; ++u) {
+          const int r = g0 + u;
+          const bool valid = r <= p && lane <= (r == p ? q : r);
+          dst[u] = valid ? row[r * (r + 1) / 2 + lane] : T(0);
+        }
+      };
+      T vn[JKC_GROUP];
+      if constexpr (PIPE) load_group(0, vn);
+#pragma unroll
+      for (int g = 0; g < JKC_MAXN; g += JKC_GROUP) {
+        if (g > p) break;  // uniform across the wavefront
+        T v[JKC_GROUP];
+        if constexpr (PIPE) {
+#pragma unroll
+          for (int u = 0; u < JKC_GROUP; ++u) v[u] = vn[u];
+          if (g + JKC_GROUP <= p && g + JKC_GROUP < JKC_MAXN) load_group(g + JKC_GROUP, vn);
+        } else {
+          load_group(g, v);
+        }
+        // t[u] = w dm[q][s] (to K(p, g+u)), t[8+u] = w dm[p][s] (to K(q, g+u))
+        T t[16];
+#pragma unroll
+        for (int u = 0; u < JKC_GROUP; ++u) {
+          const int r = g + u;
+          t[u] = t[JKC_GROUP + u] = T(0);
+          if (r > p) continue;  // uniform
+          const T w = v[u];
+          jacc[r] += w * cj;
+          jrow += w * dm[r * N + (lane <= r ? lane : r)];
+          kp += w * gq[r];
+          kq += w * gp[r];
+          t[u] = w * dq_s;
+          t[JKC_GROUP + u] = w * dp_s;
+        }
+        const T tot = transpose_reduce16(t, lane) * -hhalf;
+        const int u = lane & 7;
+        if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, jk_quant((double)tot));
       }
-    };
-    T vn[JKC_GROUP];
-    if constexpr (PIPE) load_group(0, vn);
-#pragma unroll
-    for (int g = 0; g < JKC_MAXN; g += JKC_GROUP) {
-      if (g > p) break;  // uniform across the wavefront
-      T v[JKC_GROUP];
-      if constexpr (PIPE) {
-#pragma unroll
-        for (int u = 0; u < JKC_GROUP; ++u) v[u] = vn[u];
-        if (g + JKC_GROUP <= p && g + JKC_GROUP < JKC_MAXN) load_group(g + JKC_GROUP, vn);
-      } else {
-        load_group(g, v);
+      jrow = wave_sum_last(jrow) * T(2);
+      if (lane < N) {
+        atomicAdd(acc + p * N + lane, jk_quant((double)(kp * -hhalf)));
+        atomicAdd(acc + q * N + lane, jk_quant((double)(kq * -hhalf)));
       }
-      // t[u] = w dm[q][s] (to K(p, g+u)), t[8+u] = w dm[p][s] (to K(q, g+u))
-      T t[16];
-#pragma unroll
-      for (int u = 0; u < JKC_GROUP; ++u) {
-        const int r = g + u;
-        t[u] = t[JKC_GROUP + u] = T(0);
-        if (r > p) continue;  // uniform
-        const T w = v[u];
-        jacc[r] += w * cj;
-        jrow += w * dm[r * N + (lane <= r ? lane : r)];
-        kp += w * gq[r];
-        kq += w * gp[r];
-        t[u] = w * dq_s;
-        t[JKC_GROUP + u] = w * dp_s;
-      }
-      const T tot = transpose_reduce16(t, lane) * -hhalf;
-      const int u = lane & 7;
-      if (lane < 16 && g + u <= p) atomicAdd(acc + (lane < 8 ? p : q) * N + g + u, jk_quant((double)tot));
+      if (lane == 63) atomicAdd(acc + p * N + q, jk_quant((double)jrow));
     }
-    jrow = wave_sum_last(jrow) * T(2);
-    if (lane < N) {
-      atomicAdd(acc + p * N + lane, jk_quant((double)(kp * -hhalf)));
-      atomicAdd(acc + q * N + lane, jk_quant((double)(kq * -hhalf)));
-    }
-    if (lane == 63) atomicAdd(acc + p * N + q, jk_quant((double)jrow));
+    // (predicated rather than cut short, so the loop stays unrolled and jacc in registers)
+#pragma unroll
+    for (int r = 0; r < JKC_MAXN; ++r)
+      if (r < N && lane <= r) atomicAdd(acc + r * N + lane, jk_quant((double)jacc[r]));
   }
-  // (predicated rather than cut short, so the loop stays unrolled and jacc in registers)
-#pragma unroll
-  for (int r = 0; r < JKC_MAXN; ++r)
-    if (r < N && lane <= r) atomicAdd(acc + r * N + lane, jk_quant((double)jacc[r]));
   __syncthreads();
   const bool atomic = gridDim.y > 1;
   for (int x = threadIdx.x; x < NN; x += blockDim.x) {

@@ -218,7 +218,7 @@ template <typename T, int XT, int NP>
 using XVacc = typename XMTraits<T>::v4[XTile<XT, NP>::NBW];
 
 template <typename T, int XT, int NP>
-__device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride,
+__device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride, int tile_end,
                                 XVacc<T, XT, NP>& vacc, double& e_loc) {
   typedef typename XMTraits<T>::v4 v4;
   typedef XTile<XT, NP> L;
@@ -271,8 +271,7 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
   // AO strides as 32-bit element offsets (the host checks that a molecule's
   // AO block is below 2^31 elements)
   const int sc = (int)a.sc, smu = (int)a.smu, sp = (int)a.sp;
-  const int ntiles = (a.g + XT - 1) / XT;
-  for (int tile = tile0; tile < ntiles; tile += tstride) {
+  for (int tile = tile0; tile < tile_end; tile += tstride) {
     const int p0 = tile * XT;
     __syncthreads();
     // Stage the tile: Phi[c][p][mu] = ao[c][mu][p0 + p].  Each wave moves
@@ -421,7 +420,7 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
   constexpr int XT = XMTraits<T>::TILE;
   double e_loc = 0.0;
   XVacc<T, XT, NP> vacc;
-  xc_tiles<T, XT, NP>(a, smem, blockIdx.x, gridDim.x, vacc, e_loc);
+  xc_tiles<T, XT, NP>(a, smem, blockIdx.x, gridDim.x, (a.g + XT - 1) / XT, vacc, e_loc);
   xm_store_v<T, XT, NP>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
   double* red = reinterpret_cast<double*>(smem);
   __syncthreads();
@@ -430,18 +429,32 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
 }
 
 // Batched: workgroup (blockIdx.x, blockIdx.y) takes molecule order[blockIdx.x]
-// (nao <= max_nao <= NP) and its tiles blockIdx.y, blockIdx.y + gridDim.y, ...;
-// skips converged molecules.  With one part per molecule it writes V + V^T and
-// E_xc directly; otherwise V and E_xc partials go to part_v
-// ([part][blockIdx.x][max_nao^2], row stride nao) and part_e
-// ([part][blockIdx.x]) for xc_batch_reduce_kernel.
+// (nao <= max_nao <= NP) and skips converged molecules.  A molecule's grid is
+// cut into chunks of XC_CHUNK_PTS points; part blockIdx.y runs chunks
+// blockIdx.y, blockIdx.y + gridDim.y, ..., each from zeroed accumulators, and
+// stores the chunk's V and E_xc partials in slot (chunk, blockIdx.x) of part_v
+// ([slot][max_nao^2], row stride nao) and part_e; xc_batch_reduce_kernel adds
+// the chunks in order.  Each chunk's sums cover the same points in the same
+// order whatever the number of parts, so the result does not depend on the
+// split.  A molecule with a single chunk writes V + V^T and E_xc directly.
+// Slots beyond the part buffers' capacities are not written; the reduction
+// marks such a molecule with NaN.
+constexpr int XC_CHUNK_PTS = PSCF_XC_CHUNK_PTS;
+
+__host__ __device__ inline bool xc_parts_fit(int nchunk, int nmol, int max_nao, int64_t part_v_elems,
+                                             int64_t part_e_elems) {
+  return (int64_t)nchunk * nmol * max_nao * max_nao <= part_v_elems && (int64_t)nchunk * nmol <= part_e_elems;
+}
+
 template <typename T, int XT, int NP>
 __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __restrict__ mols,
                                                               const int* __restrict__ order, int max_nao,
                                                               const double* __restrict__ ctl, const T* __restrict__ ao,
                                                               const T* __restrict__ dm, const T* __restrict__ w,
                                                               T eps, T* __restrict__ vxc, double* __restrict__ exc,
-                                                              T* __restrict__ part_v, double* __restrict__ part_e) {
+                                                              T* __restrict__ part_v, double* __restrict__ part_e,
+                                                              int64_t part_v_elems, int64_t part_e_elems) {
+  static_assert(XC_CHUNK_PTS % XT == 0, "a chunk is a whole number of tiles");
   extern __shared__ __align__(16) unsigned char smem[];
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
@@ -467,57 +480,75 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   a.dm = dm + Mo.mat0;
   a.w = w + Mo.grid0;
   a.eps = eps;
-  double e_loc = 0.0;
-  XVacc<T, XT, NP> vacc;
-  xc_tiles<T, XT, NP>(a, smem, part, nparts, vacc, e_loc);
-  __syncthreads();
+  constexpr int TPC = XC_CHUNK_PTS / XT;  // tiles per chunk
+  const int ntiles = (a.g + XT - 1) / XT;
+  const int nchunk = (a.g + XC_CHUNK_PTS - 1) / XC_CHUNK_PTS;
   double* red = reinterpret_cast<double*>(smem + sizeof(T) * N * N);
-  if (nparts > 1) {
-    const int64_t slot = (int64_t)part * gridDim.x + blockIdx.x;
+  XVacc<T, XT, NP> vacc;
+  if (nchunk <= 1) {
+    if (part != 0) return;
+    double e_loc = 0.0;
+    xc_tiles<T, XT, NP>(a, smem, 0, 1, ntiles, vacc, e_loc);
+    __syncthreads();
+    T* Vl = reinterpret_cast<T*>(smem);
+    xm_store_v<T, XT, NP>(vacc, N, Vl, N);
+    __syncthreads();
+    T* out = vxc + Mo.mat0;
+    for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      out[x] = Vl[x] + Vl[j * N + i];
+    }
+    __syncthreads();
+    const double e = xc_block_energy(e_loc, red);
+    if (threadIdx.x == 0) exc[m] = e;
+    return;
+  }
+  if (!xc_parts_fit(nchunk, gridDim.x, max_nao, part_v_elems, part_e_elems)) return;  // marked by the reduction
+  for (int c = part; c < nchunk; c += nparts) {
+    double e_loc = 0.0;
+    xc_tiles<T, XT, NP>(a, smem, c * TPC, 1, min(ntiles, (c + 1) * TPC), vacc, e_loc);
+    __syncthreads();
+    const int64_t slot = (int64_t)c * gridDim.x + blockIdx.x;
     xm_store_v<T, XT, NP>(vacc, N, part_v + slot * max_nao * max_nao, N);
     const double e = xc_block_energy(e_loc, red);
     if (threadIdx.x == 0) part_e[slot] = e;
-    return;
   }
-  T* Vl = reinterpret_cast<T*>(smem);
-  xm_store_v<T, XT, NP>(vacc, N, Vl, N);
-  __syncthreads();
-  T* out = vxc + Mo.mat0;
-  for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    out[x] = Vl[x] + Vl[j * N + i];
-  }
-  __syncthreads();
-  const double e = xc_block_energy(e_loc, red);
-  if (threadIdx.x == 0) exc[m] = e;
 }
 
-// Sums the parts of xc_batch_kernel: V_xc = sum_k (V_k + V_k^T), E_xc = sum_k e_k
-// (fixed order), one workgroup per active molecule.
+// Sums the chunks of xc_batch_kernel in order: V_xc = sum_c (V_c + V_c^T),
+// E_xc = sum_c e_c, one workgroup per active molecule with more than one chunk.
 template <typename T>
 __global__ void __launch_bounds__(256) xc_batch_reduce_kernel(const BatchMol* __restrict__ mols,
                                                               const int* __restrict__ order, int max_nao,
                                                               const double* __restrict__ ctl,
                                                               const T* __restrict__ part_v,
-                                                              const double* __restrict__ part_e, int nparts,
-                                                              T* __restrict__ vxc, double* __restrict__ exc) {
+                                                              const double* __restrict__ part_e, int64_t part_v_elems,
+                                                              int64_t part_e_elems, T* __restrict__ vxc,
+                                                              double* __restrict__ exc) {
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao;
   if (N > max_nao || (int64_t)4 * N * Mo.ngrid >= ((int64_t)1 << 31)) return;  // flagged by xc_batch_kernel
+  const int nchunk = (int)((Mo.ngrid + XC_CHUNK_PTS - 1) / XC_CHUNK_PTS);
+  if (nchunk <= 1) return;  // written by xc_batch_kernel
+  if (!xc_parts_fit(nchunk, gridDim.x, max_nao, part_v_elems, part_e_elems)) {
+    for (int x = threadIdx.x; x < N * N; x += blockDim.x) vxc[Mo.mat0 + x] = T(NAN);
+    if (threadIdx.x == 0) exc[m] = NAN;
+    return;
+  }
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
     T v = T(0);
-    for (int k = 0; k < nparts; ++k) {
-      const T* P = part_v + ((int64_t)k * gridDim.x + blockIdx.x) * max_nao * max_nao;
+    for (int c = 0; c < nchunk; ++c) {
+      const T* P = part_v + ((int64_t)c * gridDim.x + blockIdx.x) * max_nao * max_nao;
       v += P[x] + P[j * N + i];
     }
     vxc[Mo.mat0 + x] = v;
   }
   if (threadIdx.x == 0) {
     double e = 0.0;
-    for (int k = 0; k < nparts; ++k) e += part_e[(int64_t)k * gridDim.x + blockIdx.x];
+    for (int c = 0; c < nchunk; ++c) e += part_e[(int64_t)c * gridDim.x + blockIdx.x];
     exc[m] = e;
   }
 }
@@ -936,30 +967,30 @@ extern "C" int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, 
 template <typename T, int XT, int NP>
 static int batch_xc_launch(const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                            const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc,
-                           double* d_exc, int split, void* d_part_v, double* d_part_e, hipStream_t st) {
+                           double* d_exc, int split, void* d_part_v, double* d_part_e, int64_t part_v_elems,
+                           int64_t part_e_elems, hipStream_t st) {
   const size_t lds = xc_lds<T, XT, NP>(max_nao);
   PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
   PSCF_SET_LDS((xc_batch_kernel<T, XT, NP>), lds);
   xc_batch_kernel<T, XT, NP><<<dim3(nmol, split), XC_THREADS, lds, st>>>(
       (const BatchMol*)d_mols, d_order, max_nao, d_ctl, (const T*)d_ao, (const T*)d_dm, (const T*)d_w, (T)eps,
-      (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e);
+      (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e, part_v_elems, part_e_elems);
   PSCF_HIP(hipGetLastError());
-  if (split > 1) {
-    xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, max_nao, d_ctl,
-                                                   (const T*)d_part_v, d_part_e, split, (T*)d_vxc, d_exc);
-    PSCF_HIP(hipGetLastError());
-  }
+  xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, max_nao, d_ctl,
+                                                 (const T*)d_part_v, d_part_e, part_v_elems, part_e_elems,
+                                                 (T*)d_vxc, d_exc);
+  PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
 
 template <typename T, int XT>
 static int batch_xc_np(int np, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
                        const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
-                       int split, void* d_part_v, double* d_part_e, hipStream_t st) {
+                       int split, void* d_part_v, double* d_part_e, int64_t pv, int64_t pe, hipStream_t st) {
 #define PSCF_XC_NP(V)                                                                                              \
   case V:                                                                                                          \
     return batch_xc_launch<T, XT, V>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, \
-                                     split, d_part_v, d_part_e, st);
+                                     split, d_part_v, d_part_e, pv, pe, st);
   switch (np) {
     PSCF_XC_NP(16)
     PSCF_XC_NP(32)
@@ -980,20 +1011,17 @@ extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, 
   PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && d_ao && d_dm && d_w && d_vxc && d_exc,
                "pscf_batch_xc: bad arguments");
   PSCF_REQUIRE(split >= 1 && split <= 64, "pscf_batch_xc: bad split %d", split);
-  PSCF_REQUIRE(split == 1 || (d_part_v && d_part_e && part_v_elems >= (int64_t)split * nmol * max_nao * max_nao &&
-                              part_e_elems >= (int64_t)split * nmol),
-               "pscf_batch_xc: split %d needs part buffers of %lld and %lld elements", split,
-               (long long)split * nmol * max_nao * max_nao, (long long)split * nmol);
+  PSCF_REQUIRE(d_part_v && d_part_e && part_v_elems >= 0 && part_e_elems >= 0, "pscf_batch_xc: no chunk buffers");
   PSCF_REQUIRE(tile == 32 || (tile == 64 && dtype == PSCF_F32), "pscf_batch_xc: tile %d (32, or 64 in FP32)", tile);
   PSCF_REQUIRE(max_nao <= XM_MAXN, "pscf_batch_xc: nao %d too large", max_nao);
   hipStream_t st = (hipStream_t)stream;
   const int np = xm_np(max_nao);
   if (dtype == PSCF_F64)
     return batch_xc_np<double, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                   split, d_part_v, d_part_e, st);
+                                   split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
   if (tile == 64)
     return batch_xc_np<float, 64>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                  split, d_part_v, d_part_e, st);
+                                  split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
   return batch_xc_np<float, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, split,
-                                d_part_v, d_part_e, st);
+                                d_part_v, d_part_e, part_v_elems, part_e_elems, st);
 }

@@ -47,17 +47,20 @@ def test_batch_fp32_within_1mha(hip, oracle_results):
 
 @pytest.mark.parametrize("dtype", ["fp32", "fp64"])
 def test_batch_runs_bit_identical(hip, dtype):
-    """Two runs of the same batch give the same bits: the J/K atomics add
-    contributions rounded to 2^-40, whose FP64 sums are exact in any order, and
-    every other reduction has a fixed order.  Different workgroup splits of the
-    J/K and XC launches give the same bits too."""
+    """Two runs of the same batch give the same bits, and so do other launch
+    shapes (XC and J/K workgroups per molecule, J/K workgroup size): the J/K
+    atomics add contributions rounded to 2^-40, whose FP64 sums are exact in
+    any order; the J/K column form's register sums and the XC partials cover
+    fixed row sets and fixed 512-point chunks whatever the launch shape; every
+    other reduction has a fixed order."""
     mols = batch.parse(STRS + molgen.batched_set(24, (5, 9), seed=33))
     tol = 1e-6 if dtype == "fp32" else 1e-9
     runs = []
-    for split, threads in ((1, 256), (1, 256), (4, 512)):
+    for xc_split, jk_split, threads in ((1, 1, 256), (1, 1, 256), (4, 4, 512), (2, 8, 256)):
         b = batch.BatchRKS(mols, dtype=dtype)
         b.setup()
-        b.split, b.jk_threads = {"xc": 1, "jk": split}, threads
+        b.split, b.jk_threads = {"xc": xc_split, "jk": jk_split}, threads
+        print(f"bit-identical {dtype} xc split {xc_split} jk split {jk_split} threads {threads}", flush=True)
         runs.append(b.run(max_its=30, conv_tol=tol, graph=True))
     for r in runs[1:]:
         assert np.array_equal(r.energy, runs[0].energy)
