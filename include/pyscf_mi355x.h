@@ -84,6 +84,18 @@ size_t pscf_jk_s8_workspace_bytes(int dtype, int N);
 int pscf_jk_s8(int dtype, const void* d_eri_s8, int N, const void* d_dm, double hyb, void* d_out, void* d_ws,
                size_t ws_bytes, void* stream);
 
+/* pscf_jk_s8_noalloc: pscf_jk_s8 without a workspace argument and without any
+ * allocation, for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The partials live in a library-owned device
+ * array of pscf_jk_s8_noalloc_max_bytes() bytes (32 MiB) per GPU, held until
+ * the process ends; a call whose pscf_jk_s8_workspace_bytes(dtype, N) exceeds
+ * it returns PSCF_EINVAL.  Calls that overlap on one GPU share that array and
+ * race: serialise them (one stream, or events).  Results equal pscf_jk_s8's. */
+size_t pscf_jk_s8_noalloc_max_bytes(void);
+int pscf_jk_s8_noalloc(int dtype, const void* d_eri_s8, int N, const void* d_dm, double hyb, void* d_out,
+                       void* stream);
+
 /* pscf_jk_blocks: vj and vk of ipu_einsum(__out3, dm, mol)
  * (pyscf_ipu/electron_repulsion/direct.py:489-504, vertices
  * poplar_direct_s1*, int2e_sph.cpp:7598-8176): vj[k][l] = sum (ij|kl) dm[j][i],

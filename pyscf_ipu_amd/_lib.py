@@ -31,6 +31,8 @@ _SIGS = {
     "pscf_jk_blocks": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pscf_jk_s8": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pscf_jk_s8_noalloc_max_bytes": (c_size_t, []),
+    "pscf_jk_s8_noalloc": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p]),
     "pscf_ao_eval": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
                              c_void_p]),
     "pscf_becke_weights": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,

@@ -11,6 +11,8 @@
 //   * jk_s8: the packed s8 array of pscf_eri_s8, indices regenerated from the
 //     element position (4 B of HBM traffic per ERI), dm symmetric, so J and K
 //     fold into one accumulator M with dJK = M + M^T (6 scatters per ERI).
+#include <mutex>
+
 #include "batch.h"
 #include "common.h"
 #include "reduce.h"
@@ -706,6 +708,36 @@ static int jk_s8_impl(const T* eri, int N, const T* dm, double hyb, T* out, void
   return reduce_parts<T, true>((const T*)ws, nb, N, lvl, out, nullptr, nullptr, st);
 }
 
+// ------------------------------------------------------ allocation-free form ---
+// The partial matrices and chunk sums of jk_s8_impl in a file-scope device
+// array instead of a caller workspace, for callers that cannot allocate.  Each
+// GPU holds its own copy (JK_SCRATCH_BYTES, until the process ends); its
+// address on the current device comes from hipGetSymbolAddress once per device
+// and is kept, since the array's name in host code is not a device address.
+// Every partial and chunk sum is written before it is read, so nothing needs
+// clearing between calls.  Calls that overlap on one GPU share the array and
+// race.
+constexpr size_t JK_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr int JK_SCRATCH_MAX_DEV = 64;
+static __device__ __align__(256) unsigned char scratch[JK_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[JK_SCRATCH_MAX_DEV];
+
+static int scratch_ptr(void** out) {
+  int dev = 0;
+  PSCF_HIP(hipGetDevice(&dev));
+  PSCF_REQUIRE(dev >= 0 && dev < JK_SCRATCH_MAX_DEV, "scratch: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    PSCF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PSCF_REQUIRE(p != nullptr, "scratch: no device address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PSCF_OK;
+}
+
 }  // namespace pscf
 
 using namespace pscf;
@@ -838,4 +870,23 @@ extern "C" int pscf_batch_eri_scale(int dtype, const void* d_mols, int nmol, int
     eri_scale_kernel<float><<<grid, 256, 0, st>>>((const BatchMol*)d_mols, (float*)d_eri);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
+}
+
+extern "C" size_t pscf_jk_s8_noalloc_max_bytes(void) { return JK_SCRATCH_BYTES; }
+
+extern "C" int pscf_jk_s8_noalloc(int dtype, const void* d_eri_s8, int N, const void* d_dm, double hyb, void* d_out,
+                                  void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_jk_s8_noalloc: bad dtype %d", dtype);
+  PSCF_REQUIRE(N > 0 && N < 65536, "pscf_jk_s8_noalloc: N=%d out of range", N);
+  PSCF_REQUIRE(d_eri_s8 && d_dm && d_out, "pscf_jk_s8_noalloc: null pointer");
+  const size_t need = jk_s8_ws_bytes(dtype == PSCF_F64 ? 8 : 4, N);
+  PSCF_REQUIRE(need <= JK_SCRATCH_BYTES, "pscf_jk_s8_noalloc: N=%d needs %zu B of scratch, above the %zu B it has", N,
+               need, JK_SCRATCH_BYTES);
+  void* ws = nullptr;
+  int rc = scratch_ptr(&ws);
+  if (rc != PSCF_OK) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSCF_F64)
+    return jk_s8_impl<double>((const double*)d_eri_s8, N, (const double*)d_dm, hyb, (double*)d_out, ws, need, st);
+  return jk_s8_impl<float>((const float*)d_eri_s8, N, (const float*)d_dm, hyb, (float*)d_out, ws, need, st);
 }
