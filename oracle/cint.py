@@ -8,6 +8,8 @@ Functions and what pins them (tests/test_oracle_integrals.py):
   eri_s8      Szabo-Ostlund H2 ERIs and PyQuante (ss|ss) = 1.128379, (ss|pp) = 0.940316
               (test_integrals.py:111-133); THO/MD second algorithm (md_eri.py)
   sparse_jk   dense einsum definition of J and K (nanoDFT.py:130-132)
+  eri_ip1, int1e_ip   central finite differences of eri_s8 / int1e under a
+              displaced atom (tests/test_grad_oracle.py)
 """
 from __future__ import annotations
 
@@ -116,3 +118,31 @@ def s8_to_s1(eri_s8: np.ndarray, N: int) -> np.ndarray:
     pair[q, p] = np.arange(P)
     a, b = np.maximum(pair[:, :, None, None], pair[None, None, :, :]), np.minimum(pair[:, :, None, None], pair[None, None, :, :])
     return eri_s8[a * (a + 1) // 2 + b]
+
+
+def eri_ip1(mol) -> np.ndarray:
+    """(3, N, N, N, N) (nabla i j | k l), PySCF's int2e_ip1 layout (derivative
+    on the electron coordinate of the first function).  Small molecules only."""
+    atm, bas, env = _mol_arrays(mol)
+    N = mol.nao_nr()
+    out = np.zeros((3, N, N, N, N))
+    rc = lib().oracle_eri_ip1(_p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)), _p(env),
+                              ctypes.c_int(N), _p(out))
+    if rc != 0:
+        raise ValueError("oracle_eri_ip1: unsupported shell")
+    return out
+
+
+def int1e_ip(mol, nuc: int = -1):
+    """(ipovlp, ipkin, ipnuc, iprinv) each (3, N, N): <nabla i | O | j> for O =
+    1, -nabla^2/2, sum_C -Z_C/|r - C|, and (nuc >= 0) 1/|r - R_nuc| (else None)."""
+    atm, bas, env = _mol_arrays(mol)
+    N = mol.nao_nr()
+    o, k, v = np.zeros((3, N, N)), np.zeros((3, N, N)), np.zeros((3, N, N))
+    r = np.zeros((3, N, N)) if nuc >= 0 else None
+    rc = lib().oracle_int1e_ip(_p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)), _p(env),
+                               ctypes.c_int(N), _p(o), _p(k), _p(v), ctypes.c_int(nuc),
+                               _p(r) if r is not None else None)
+    if rc != 0:
+        raise ValueError("oracle_int1e_ip: unsupported shell")
+    return o, k, v, r

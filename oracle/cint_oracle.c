@@ -368,6 +368,85 @@ static void os_overlap_1d(double pa, double pb, double p, int imax, int jmax, do
             S[i][j + 1] = pb * S[i][j] + h * ((i > 0 ? i * S[i - 1][j] : 0.0) + (j > 0 ? j * S[i][j - 1] : 0.0));
 }
 
+/* Contracted cartesian one-electron blocks <a|O|b> (row-major [fa][fb], l <= 2
+ * on either side): overlap S, kinetic T = <a|-nabla^2/2|b>, and V: with nuc < 0
+ * the nuclear attraction sum_C -Z_C <a|1/|r-C||b>, with nuc = C only
+ * <a|1/|r-C||b> (charge 1, the rinv operator of PySCF's int1e_*rinv). */
+static void int1e_block(const Shell *A, const Shell *B, const int *atm, int natm, const double *env, int nuc,
+                        double *S, double *T, double *V)
+{
+    int la = A->l, lb = B->l, na = ncart(la), nb = ncart(lb);
+    int ca[6][3], cb[6][3];
+    cart_list(la, ca); cart_list(lb, cb);
+    memset(S, 0, sizeof(double) * na * nb);
+    memset(T, 0, sizeof(double) * na * nb);
+    memset(V, 0, sizeof(double) * na * nb);
+    double AB[3];
+    for (int d = 0; d < 3; d++) AB[d] = A->r[d] - B->r[d];
+    double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    for (int pa = 0; pa < A->nprim; pa++)
+        for (int pb = 0; pb < B->nprim; pb++) {
+            double a = A->e[pa], b = B->e[pb], p = a + b, mu = a * b / p;
+            double P[3];
+            for (int d = 0; d < 3; d++) P[d] = (a * A->r[d] + b * B->r[d]) / p;
+            double cc = A->c[pa] * B->c[pb];
+            double S1[3][5][5];
+            for (int d = 0; d < 3; d++) os_overlap_1d(P[d] - A->r[d], P[d] - B->r[d], p, la + 2, lb + 2, S1[d]);
+            double s00 = pow(PI / p, 1.5) * exp(-mu * rab2);
+            /* nuclear attraction by Rys quadrature over each nucleus */
+            int nr = (la + lb) / 2 + 1;
+            double vbuf[6][6];
+            memset(vbuf, 0, sizeof(vbuf));
+            for (int iat = 0; iat < natm; iat++) {
+                if (nuc >= 0 && iat != nuc) continue;
+                const double *Cn = env + atm[iat * ATM_SLOTS + 1];
+                double Z = nuc >= 0 ? -1.0 : atm[iat * ATM_SLOTS + 0];
+                double PC[3];
+                for (int d = 0; d < 3; d++) PC[d] = P[d] - Cn[d];
+                double Tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                double u[MAXROOTS], w[MAXROOTS];
+                oracle_rys(nr, Tn, u, w);
+                for (int r = 0; r < nr; r++) {
+                    double J1[3][5][5];
+                    for (int d = 0; d < 3; d++) {
+                        double pa_ = P[d] - A->r[d] - u[r] * PC[d];
+                        double pb_ = P[d] - B->r[d] - u[r] * PC[d];
+                        double h = 0.5 * (1.0 - u[r]) / p;
+                        double (*J)[5] = J1[d];
+                        J[0][0] = 1.0;
+                        for (int i = 0; i < la; i++) J[i + 1][0] = pa_ * J[i][0] + (i > 0 ? i * h * J[i - 1][0] : 0.0);
+                        for (int j = 0; j < lb; j++)
+                            for (int i = 0; i <= la; i++)
+                                J[i][j + 1] = pb_ * J[i][j] + h * ((i > 0 ? i * J[i - 1][j] : 0.0) + (j > 0 ? j * J[i][j - 1] : 0.0));
+                    }
+                    for (int fa = 0; fa < na; fa++)
+                        for (int fb = 0; fb < nb; fb++)
+                            vbuf[fa][fb] += -Z * w[r] * J1[0][ca[fa][0]][cb[fb][0]] * J1[1][ca[fa][1]][cb[fb][1]]
+                                            * J1[2][ca[fa][2]][cb[fb][2]];
+                }
+            }
+            double vpre = 2.0 * PI / p * exp(-mu * rab2);
+            for (int fa = 0; fa < na; fa++)
+                for (int fb = 0; fb < nb; fb++) {
+                    const int *x = ca[fa], *y = cb[fb];
+                    double sx = S1[0][x[0]][y[0]], sy = S1[1][x[1]][y[1]], sz = S1[2][x[2]][y[2]];
+                    double tt = 0.0;
+                    for (int d = 0; d < 3; d++) {
+                        int i = x[d], j = y[d];
+                        double t1 = b * (2 * j + 1) * S1[d][i][j] - 2.0 * b * b * S1[d][i][j + 2]
+                                    - (j >= 2 ? 0.5 * j * (j - 1) * S1[d][i][j - 2] : 0.0);
+                        double others = 1.0;
+                        for (int e = 0; e < 3; e++)
+                            if (e != d) others *= S1[e][x[e]][y[e]];
+                        tt += t1 * others;
+                    }
+                    S[fa * nb + fb] += cc * s00 * sx * sy * sz;
+                    T[fa * nb + fb] += cc * s00 * tt;
+                    V[fa * nb + fb] += cc * vpre * vbuf[fa][fb];
+                }
+        }
+}
+
 /* S, T, V (all nuclei) in the spherical AO basis, each nao x nao row-major. */
 int oracle_int1e(const int *atm, int natm, const int *bas, int nbas, const double *env, int nao,
                  double *S, double *T, double *V)
@@ -379,77 +458,160 @@ int oracle_int1e(const int *atm, int natm, const int *bas, int nbas, const doubl
     memset(S, 0, sizeof(double) * nao * nao);
     memset(T, 0, sizeof(double) * nao * nao);
     memset(V, 0, sizeof(double) * nao * nao);
+    double bs[36], bt[36], bv[36];
     for (int ia = 0; ia < nbas; ia++)
         for (int ib = 0; ib < nbas; ib++) {
             const Shell *A = sh + ia, *B = sh + ib;
-            int la = A->l, lb = B->l, na = ncart(la), nb = ncart(lb);
-            int ca[6][3], cb[6][3];
-            cart_list(la, ca); cart_list(lb, cb);
-            double AB[3];
-            for (int d = 0; d < 3; d++) AB[d] = A->r[d] - B->r[d];
-            double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
-            for (int pa = 0; pa < A->nprim; pa++)
-                for (int pb = 0; pb < B->nprim; pb++) {
-                    double a = A->e[pa], b = B->e[pb], p = a + b, mu = a * b / p;
-                    double P[3];
-                    for (int d = 0; d < 3; d++) P[d] = (a * A->r[d] + b * B->r[d]) / p;
-                    double cc = A->c[pa] * B->c[pb];
-                    double S1[3][5][5];
-                    for (int d = 0; d < 3; d++) os_overlap_1d(P[d] - A->r[d], P[d] - B->r[d], p, la + 2, lb + 2, S1[d]);
-                    double s00 = pow(PI / p, 1.5) * exp(-mu * rab2);
-                    /* nuclear attraction by Rys quadrature over each nucleus */
-                    int nr = (la + lb) / 2 + 1;
-                    double vbuf[6][6];
-                    memset(vbuf, 0, sizeof(vbuf));
-                    for (int iat = 0; iat < natm; iat++) {
-                        const double *Cn = env + atm[iat * ATM_SLOTS + 1];
-                        double Z = atm[iat * ATM_SLOTS + 0];
-                        double PC[3];
-                        for (int d = 0; d < 3; d++) PC[d] = P[d] - Cn[d];
-                        double Tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                        double u[MAXROOTS], w[MAXROOTS];
-                        oracle_rys(nr, Tn, u, w);
-                        for (int r = 0; r < nr; r++) {
-                            double J1[3][5][5];
-                            for (int d = 0; d < 3; d++) {
-                                double pa_ = P[d] - A->r[d] - u[r] * PC[d];
-                                double pb_ = P[d] - B->r[d] - u[r] * PC[d];
-                                double h = 0.5 * (1.0 - u[r]) / p;
-                                double (*J)[5] = J1[d];
-                                J[0][0] = 1.0;
-                                for (int i = 0; i < la; i++) J[i + 1][0] = pa_ * J[i][0] + (i > 0 ? i * h * J[i - 1][0] : 0.0);
-                                for (int j = 0; j < lb; j++)
-                                    for (int i = 0; i <= la; i++)
-                                        J[i][j + 1] = pb_ * J[i][j] + h * ((i > 0 ? i * J[i - 1][j] : 0.0) + (j > 0 ? j * J[i][j - 1] : 0.0));
-                            }
-                            for (int fa = 0; fa < na; fa++)
-                                for (int fb = 0; fb < nb; fb++)
-                                    vbuf[fa][fb] += -Z * w[r] * J1[0][ca[fa][0]][cb[fb][0]] * J1[1][ca[fa][1]][cb[fb][1]]
-                                                    * J1[2][ca[fa][2]][cb[fb][2]];
-                        }
-                    }
-                    double vpre = 2.0 * PI / p * exp(-mu * rab2);
-                    for (int fa = 0; fa < na; fa++)
-                        for (int fb = 0; fb < nb; fb++) {
-                            const int *x = ca[fa], *y = cb[fb];
-                            double sx = S1[0][x[0]][y[0]], sy = S1[1][x[1]][y[1]], sz = S1[2][x[2]][y[2]];
-                            double tt = 0.0;
-                            for (int d = 0; d < 3; d++) {
-                                int i = x[d], j = y[d];
-                                double t1 = b * (2 * j + 1) * S1[d][i][j] - 2.0 * b * b * S1[d][i][j + 2]
-                                            - (j >= 2 ? 0.5 * j * (j - 1) * S1[d][i][j - 2] : 0.0);
-                                double others = 1.0;
-                                for (int e = 0; e < 3; e++)
-                                    if (e != d) others *= S1[e][x[e]][y[e]];
-                                tt += t1 * others;
-                            }
-                            int row = A->ao0 + fa, col = B->ao0 + fb;
-                            S[row * nao + col] += cc * s00 * sx * sy * sz;
-                            T[row * nao + col] += cc * s00 * tt;
-                            V[row * nao + col] += cc * vpre * vbuf[fa][fb];
-                        }
+            int na = ncart(A->l), nb = ncart(B->l);
+            int1e_block(A, B, atm, natm, env, -1, bs, bt, bv);
+            for (int fa = 0; fa < na; fa++)
+                for (int fb = 0; fb < nb; fb++) {
+                    int row = A->ao0 + fa, col = B->ao0 + fb;
+                    S[row * nao + col] = bs[fa * nb + fb];
+                    T[row * nao + col] = bt[fa * nb + fb];
+                    V[row * nao + col] = bv[fa * nb + fb];
                 }
         }
+    free(sh);
+    return 0;
+}
+
+/* ------------------------------------------------------ derivative integrals ---- */
+/* The derivative of a cartesian Gaussian x^ix y^iy z^iz exp(-a r^2) centred at
+ * A along the electron coordinate x: ix (i - 1_x) - 2a (i + 1_x).  The two
+ * terms are integrals over the same shell with l - 1 (coefficients c) and
+ * l + 1 (coefficients 2 a c), the coefficients keeping the s/p angular factor
+ * of the original shell. */
+static void shell_raised(const Shell *s, Shell *up)
+{
+    *up = *s;
+    up->l = s->l + 1;
+    for (int k = 0; k < s->nprim; k++) up->c[k] = 2.0 * s->e[k] * s->c[k];
+}
+
+static void shell_lowered(const Shell *s, Shell *dn)
+{
+    *dn = *s;
+    dn->l = s->l - 1;
+}
+
+/* index of the cartesian component with exponents e in cart_list(l) order */
+static int cart_index(int l, const int *e)
+{
+    int c[6][3];
+    cart_list(l, c);
+    for (int k = 0; k < ncart(l); k++)
+        if (c[k][0] == e[0] && c[k][1] == e[1] && c[k][2] == e[2]) return k;
+    return -1;
+}
+
+/* (nabla_x i, j | k l) of the s/p basis, dense [3][nao][nao][nao][nao] in the
+ * layout of PySCF's mol.intor("int2e_ip1") (nanoDFT.py:428): the derivative
+ * acts on the electron coordinate of the first function, i.e. it is minus the
+ * derivative with respect to that function's centre. */
+int oracle_eri_ip1(const int *atm, int natm, const int *bas, int nbas, const double *env, int nao, double *out)
+{
+    (void)natm;
+    Shell *sh = (Shell *)malloc(sizeof(Shell) * nbas);
+    load_shells(atm, bas, nbas, env, sh);
+    for (int i = 0; i < nbas; i++)
+        if (sh[i].l > 1) { free(sh); return -1; }
+    const int64_t n1 = nao, n2 = n1 * nao, n3 = n2 * nao, n4 = n3 * nao;
+    memset(out, 0, sizeof(double) * 3 * n4);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < nbas; i++) {
+        double up[6 * 27], dn[3 * 27];
+        Shell A1, A0;
+        shell_raised(sh + i, &A1);
+        if (sh[i].l > 0) shell_lowered(sh + i, &A0);
+        int ci[3][3];
+        cart_list(sh[i].l, ci);
+        for (int j = 0; j < nbas; j++)
+            for (int k = 0; k < nbas; k++)
+                for (int l = 0; l < nbas; l++) {
+                    const Shell *B = sh + j, *C = sh + k, *D = sh + l;
+                    int nj = ncart(B->l), nk = ncart(C->l), nl = ncart(D->l), rest = nj * nk * nl;
+                    eri_block(&A1, B, C, D, up);
+                    if (sh[i].l > 0) eri_block(&A0, B, C, D, dn);
+                    for (int fi = 0; fi < ncart(sh[i].l); fi++)
+                        for (int x = 0; x < 3; x++) {
+                            int e[3] = {ci[fi][0], ci[fi][1], ci[fi][2]};
+                            e[x] += 1;
+                            int iu = cart_index(sh[i].l + 1, e);
+                            e[x] -= 2;
+                            int id = ci[fi][x] > 0 ? cart_index(sh[i].l - 1, e) : -1;
+                            for (int r = 0; r < rest; r++) {
+                                int fj = r / (nk * nl), fk = (r / nl) % nk, fl = r % nl;
+                                double v = -up[iu * rest + r];
+                                if (id >= 0) v += ci[fi][x] * dn[id * rest + r];
+                                int64_t p = sh[i].ao0 + fi, q = B->ao0 + fj, rr = C->ao0 + fk, ss = D->ao0 + fl;
+                                out[x * n4 + p * n3 + q * n2 + rr * n1 + ss] = v;
+                            }
+                        }
+                }
+    }
+    free(sh);
+    return 0;
+}
+
+/* <nabla_x i | O | j> for O = 1 (ipovlp), -nabla^2/2 (ipkin), sum_C -Z_C/|r-C|
+ * (ipnuc), each [3][nao][nao] (PySCF int1e_ipovlp / ipkin / ipnuc layout,
+ * nanoDFT.py:429-431); with nuc >= 0, `nuc_out` gets <nabla_x i | 1/|r-C| | j>
+ * for C = nuc (int1e_iprinv under with_rinv_at_nucleus, nanoDFT.py:441). */
+int oracle_int1e_ip(const int *atm, int natm, const int *bas, int nbas, const double *env, int nao, double *ipovlp,
+                    double *ipkin, double *ipnuc, int nuc, double *nuc_out)
+{
+    Shell *sh = (Shell *)malloc(sizeof(Shell) * nbas);
+    load_shells(atm, bas, nbas, env, sh);
+    for (int i = 0; i < nbas; i++)
+        if (sh[i].l > 1) { free(sh); return -1; }
+    const int64_t nn = (int64_t)nao * nao;
+    if (ipovlp) memset(ipovlp, 0, sizeof(double) * 3 * nn);
+    if (ipkin) memset(ipkin, 0, sizeof(double) * 3 * nn);
+    if (ipnuc) memset(ipnuc, 0, sizeof(double) * 3 * nn);
+    if (nuc_out) memset(nuc_out, 0, sizeof(double) * 3 * nn);
+    double us[18], ut[18], uv[18], ds[9], dt[9], dv[9], ur[18], dr[9], tmp_s[18], tmp_t[18];
+    for (int ia = 0; ia < nbas; ia++) {
+        Shell A1, A0;
+        shell_raised(sh + ia, &A1);
+        if (sh[ia].l > 0) shell_lowered(sh + ia, &A0);
+        int ci[3][3];
+        cart_list(sh[ia].l, ci);
+        for (int ib = 0; ib < nbas; ib++) {
+            const Shell *B = sh + ib;
+            int nb = ncart(B->l);
+            int1e_block(&A1, B, atm, natm, env, -1, us, ut, uv);
+            if (sh[ia].l > 0) int1e_block(&A0, B, atm, natm, env, -1, ds, dt, dv);
+            if (nuc >= 0) {
+                int1e_block(&A1, B, atm, natm, env, nuc, tmp_s, tmp_t, ur);
+                if (sh[ia].l > 0) int1e_block(&A0, B, atm, natm, env, nuc, tmp_s, tmp_t, dr);
+            }
+            for (int fa = 0; fa < ncart(sh[ia].l); fa++)
+                for (int x = 0; x < 3; x++) {
+                    int e[3] = {ci[fa][0], ci[fa][1], ci[fa][2]};
+                    e[x] += 1;
+                    int iu = cart_index(sh[ia].l + 1, e);
+                    e[x] -= 2;
+                    int id = ci[fa][x] > 0 ? cart_index(sh[ia].l - 1, e) : -1;
+                    double lx = ci[fa][x];
+                    for (int fb = 0; fb < nb; fb++) {
+                        int64_t o = x * nn + (int64_t)(sh[ia].ao0 + fa) * nao + B->ao0 + fb;
+                        double vs = -us[iu * nb + fb], vt = -ut[iu * nb + fb], vv = -uv[iu * nb + fb];
+                        double vr = nuc >= 0 ? -ur[iu * nb + fb] : 0.0;
+                        if (id >= 0) {
+                            vs += lx * ds[id * nb + fb];
+                            vt += lx * dt[id * nb + fb];
+                            vv += lx * dv[id * nb + fb];
+                            if (nuc >= 0) vr += lx * dr[id * nb + fb];
+                        }
+                        if (ipovlp) ipovlp[o] = vs;
+                        if (ipkin) ipkin[o] = vt;
+                        if (ipnuc) ipnuc[o] = vv;
+                        if (nuc_out) nuc_out[o] = vr;
+                    }
+                }
+        }
+    }
     free(sh);
     return 0;
 }
