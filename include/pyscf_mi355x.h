@@ -193,6 +193,31 @@ int pscf_rks_run(int dtype, int N, int nocc, int g, int max_its, double conv_tol
                  void* d_ws_jk, size_t ws_jk_bytes, double e_nuc, double* h_pinned, double* h_log, int* h_iterations,
                  int* h_converged, void* stream);
 
+/* ------------------------------------------------------ nuclear gradient ----
+ * grad(mol, coords, weight, mo_coeff, mo_energy) (pyscf_ipu/nanoDFT/nanoDFT.py:
+ * 451-477, grad_elec :403-433) without forming its integral tensors: each
+ * entry contracts one class of derivative integrals with the densities as it
+ * generates them and ADDS into d_grad (natm x 3 doubles, Ha/Bohr, zeroed by
+ * the caller).  d_dm = dm0 = 2 C_occ C_occ^T, d_dme = dme0 = 2 C_occ e_occ
+ * C_occ^T (N x N, FP64).  Atomic adds of contributions rounded to 2^-40, so
+ * the sums are exact and repeat bit for bit.  The nuclear repulsion term
+ * (grad_nuc :435-449) is the caller's.  FP64 only. */
+/* h1aos . dm0 and -2 s1 . dme0 (int1e_ipkin, int1e_ipnuc, int1e_iprinv,
+ * int1e_ipovlp; nanoDFT.py:456-470): over ordered shell pairs. */
+int pscf_grad_int1e(const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env, int nao,
+                    const double* d_dm, const double* d_dme, double* d_grad, void* stream);
+/* vj - hyb/2 vk of int2e_ip1 contracted with dm0 (nanoDFT.py:416-418, :428):
+ * the unique shell quartets of pscf_eri_s8 (same class-sorted pair list). */
+int pscf_grad_eri(const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env, const int* d_pairs,
+                  const int* h_class_counts, int nao, const double* d_dm, double hyb, double* d_grad, void* stream);
+/* The B3LYP part (vmat and the deriv-2 AO term, nanoDFT.py:407-415) on the
+ * SCF's grid (coords g x 3 Bohr, weights g), AO values and derivatives
+ * evaluated in the kernel; workspace pscf_grad_xc_workspace_bytes. */
+size_t pscf_grad_xc_workspace_bytes(int nao, int nbas, int g);
+int pscf_grad_xc(const double* d_coords, const double* d_weights, int g, const int* d_atm, int natm, const int* d_bas,
+                 int nbas, const double* d_env, int nao, const double* d_dm, double eps, void* d_ws, size_t ws_bytes,
+                 double* d_grad, void* stream);
+
 /* --------------------------------------------------------- batched form ----
  * Many molecules of uneven size per launch.  d_mols is an int64 [nmol][16]
  * descriptor array (natm, nbas, nao, nocc, atm0, bas0, ns, np, shl0, ngrid,

@@ -47,6 +47,14 @@ _SIGS = {
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "pscf_energy": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "pscf_cast": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p]),
+    # nuclear gradient
+    "pscf_grad_int1e": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                c_void_p]),
+    "pscf_grad_eri": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                              c_double, c_void_p, c_void_p]),
+    "pscf_grad_xc_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "pscf_grad_xc": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                             c_double, c_void_p, c_size_t, c_void_p, c_void_p]),
     # batched form
     "pscf_batch_eri_s8": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),

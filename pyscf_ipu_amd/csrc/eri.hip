@@ -770,6 +770,251 @@ static int64_t s8_class_quartets(int X, int Y, const int* cnt) {
   return X == Y ? tri((int64_t)cnt[X]) : (int64_t)cnt[X] * cnt[Y];
 }
 
+// ------------------------------------------------------------ gradients ---
+// The two-electron part of the nuclear gradient (grad_elec,
+// pyscf_ipu/nanoDFT/nanoDFT.py:403-433, where int2e_ip1 is contracted with dm0
+// into vj and vk): E_2e = 1/2 sum (ij|kl) G_ijkl with
+//   G_ijkl = D_ij D_kl - (hyb/4) (D_ik D_jl + D_il D_jk),
+// so dE_2e/dR = 1/2 sum over unique shell quartets of their orbit size
+// 8 / 2^([i=j] + [k=l] + [ij=kl]) times sum_components G d(ij|kl)/dR.  The
+// derivative of a cartesian Gaussian with respect to its centre along d is
+// 2a (e + 1_d) - e_d (e - 1_d), so the Rys 2D integrals are built one unit
+// higher in n and m than the energy needs; the centres of shells i, j and k
+// are differentiated, the centre of l takes minus their sum (translational
+// invariance).  FP64 only.
+
+// Horizontal transfer for one axis, j <= 2 and l <= 1: (i j | k l) from the
+// moments g(n, m) = (n 0 | m 0), with (x - B)^j = sum_a C(j, a) (x - A)^a AB^(j-a)
+// and (x - D)^l likewise with CD.  The indices are compile-time constants
+// after unrolling, so the branches fold away.
+template <int NG, int MG>
+__device__ inline double hrr_axis(const double (&g)[NG][MG], int i, int j, int k, int l, double ab, double cd) {
+  auto col = [&](int n) { return l == 0 ? g[n][k] : g[n][k + 1] + cd * g[n][k]; };
+  if (j == 0) return col(i);
+  if (j == 1) return col(i + 1) + ab * col(i);
+  return col(i + 2) + 2.0 * ab * col(i + 1) + ab * ab * col(i);
+}
+
+template <int LI, int LJ, int LK, int LL>
+__device__ inline void shell_quartet_grad(const ShellD& A, const ShellD& B, const ShellD& C, const ShellD& D,
+                                          const double* __restrict__ tab, const double* __restrict__ gam,
+                                          double (&gx)[3][3]) {
+  constexpr int LT = LI + LJ + LK + LL + 1;
+  constexpr int NR = LT / 2 + 1;
+  constexpr int NMAX = LI + LJ + 1, MMAX = LK + LL + 1;
+  constexpr int NI = NComp<LI>::v, NJ = NComp<LJ>::v, NK = NComp<LK>::v, NL = NComp<LL>::v;
+  const double angc = (LI ? ANG_P : ANG_S) * (LJ ? ANG_P : ANG_S) * (LK ? ANG_P : ANG_S) * (LL ? ANG_P : ANG_S);
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gx[x][d] = 0.0;
+  double AB[3], CD[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    AB[d] = A.r[d] - B.r[d];
+    CD[d] = C.r[d] - D.r[d];
+  }
+  const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+  const double rcd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
+#pragma unroll 1
+  for (int pa = 0; pa < A.np; ++pa) {
+    const double ai = A.e[pa];
+#pragma unroll 1
+    for (int pb = 0; pb < B.np; ++pb) {
+      const double aj = B.e[pb];
+      const double aij = ai + aj, r_aij = 1.0 / aij;
+      const double kab = exp(-ai * aj * r_aij * rab2) * A.c[pa] * B.c[pb];
+      double PA[3], P[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        PA[d] = -aj * r_aij * AB[d];
+        P[d] = A.r[d] + PA[d];
+      }
+#pragma unroll 1
+      for (int pc = 0; pc < C.np; ++pc) {
+        const double ak = C.e[pc];
+#pragma unroll 1
+        for (int pd = 0; pd < D.np; ++pd) {
+          const double al = D.e[pd];
+          const double akl = ak + al, r_akl = 1.0 / akl, alr = al * r_akl;
+          const double kcd = exp(-ak * alr * rcd2) * C.c[pc] * D.c[pd];
+          double QC[3], PQ[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            QC[d] = -alr * CD[d];
+            PQ[d] = P[d] - (C.r[d] + QC[d]);
+          }
+          const double r_s = 1.0 / (aij + akl);
+          const double tt = aij * akl * r_s * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+          const double pref = 34.986836655249725 * r_aij * r_akl * sqrt(r_s) * kab * kcd * angc;
+          double f[2 * NR];
+          boys<double, 2 * NR - 1, BOYS_TAYLOR>(tt, tab, f);
+          double u[NR], w[NR];
+          if constexpr (NR == 1) {
+            u[0] = f[1] / f[0];
+            w[0] = f[0];
+          } else {
+            rys_from_moments<NR>(f, u, w);
+          }
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const double t2 = u[r];
+            const double b00 = 0.5 * t2 * r_s;
+            const double b10 = 0.5 * r_aij * (1.0 - akl * t2 * r_s);
+            const double b01 = 0.5 * r_akl * (1.0 - aij * t2 * r_s);
+            // I[d][i][j][k][l]: i <= LI + 1, j <= LJ + 1, k <= LK + 1, l <= LL,
+            // at most one of i, j, k above its shell's l (the others unused)
+            double I[3][LI + 2][LJ + 2][LK + 2][LL + 1];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const double c00 = PA[d] - akl * PQ[d] * t2 * r_s;
+              const double c0p = QC[d] + aij * PQ[d] * t2 * r_s;
+              double g[NMAX + 1][MMAX + 1];
+              g[0][0] = 1.0;
+#pragma unroll
+              for (int n = 0; n < NMAX; ++n) g[n + 1][0] = c00 * g[n][0] + (n > 0 ? n * b10 * g[n - 1][0] : 0.0);
+#pragma unroll
+              for (int m = 0; m < MMAX; ++m)
+#pragma unroll
+                for (int n = 0; n <= NMAX; ++n)
+                  g[n][m + 1] = c0p * g[n][m] + (m > 0 ? m * b01 * g[n][m - 1] : 0.0) +
+                                (n > 0 ? n * b00 * g[n - 1][m] : 0.0);
+#pragma unroll
+              for (int i = 0; i <= LI + 1; ++i)
+#pragma unroll
+                for (int j = 0; j <= LJ + 1; ++j)
+#pragma unroll
+                  for (int k = 0; k <= LK + 1; ++k)
+#pragma unroll
+                    for (int l = 0; l <= LL; ++l) {
+                      if ((i > LI) + (j > LJ) + (k > LK) > 1) continue;
+                      I[d][i][j][k][l] = hrr_axis(g, i, j, k, l, AB[d], CD[d]);
+                    }
+            }
+            const double wr = w[r] * pref;
+            double sA[3] = {0.0, 0.0, 0.0}, sB[3] = {0.0, 0.0, 0.0}, sC[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int fi = 0; fi < NI; ++fi)
+#pragma unroll
+              for (int fj = 0; fj < NJ; ++fj)
+#pragma unroll
+                for (int fk = 0; fk < NK; ++fk)
+#pragma unroll
+                  for (int fl = 0; fl < NL; ++fl) {
+                    const double G = gam[((fi * NJ + fj) * NK + fk) * NL + fl];
+                    double base[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                      base[d] = I[d][cexp<LI>(fi, d)][cexp<LJ>(fj, d)][cexp<LK>(fk, d)][cexp<LL>(fl, d)];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                      const int ei = cexp<LI>(fi, d), ej = cexp<LJ>(fj, d), ek = cexp<LK>(fk, d), el = cexp<LL>(fl, d);
+                      const double rest = G * base[(d + 1) % 3] * base[(d + 2) % 3];
+                      double dA = 2.0 * ai * I[d][ei + 1][ej][ek][el];
+                      if (ei > 0) dA -= ei * I[d][ei - 1][ej][ek][el];
+                      double dB = 2.0 * aj * I[d][ei][ej + 1][ek][el];
+                      if (ej > 0) dB -= ej * I[d][ei][ej - 1][ek][el];
+                      double dC = 2.0 * ak * I[d][ei][ej][ek + 1][el];
+                      if (ek > 0) dC -= ek * I[d][ei][ej][ek - 1][el];
+                      sA[d] += rest * dA;
+                      sB[d] += rest * dB;
+                      sC[d] += rest * dC;
+                    }
+                  }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              gx[0][d] += wr * sA[d];
+              gx[1][d] += wr * sB[d];
+              gx[2][d] += wr * sC[d];
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Order-independent accumulation of the gradient: each contribution is rounded
+// to a multiple of 2^-40 first, so the FP64 sums are exact (below 2^13 in
+// magnitude) whatever the order of the atomic adds.
+__device__ __forceinline__ double grad_quant(double v) { return rint(v * 0x1p40) * 0x1p-40; }
+
+template <int X, int Y>
+__device__ inline void s8_quartet_grad(const int* __restrict__ bas, const double* __restrict__ env,
+                                       const int* __restrict__ atm, const int* __restrict__ ao_loc,
+                                       const int* __restrict__ pairs, int x0, int nx, int y0, int ny, int64_t q,
+                                       const double* tab, const double* __restrict__ dm, int N, double hq,
+                                       double* __restrict__ g_lds) {
+  constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
+  constexpr int NI = NComp<LI>::v, NJ = NComp<LJ>::v, NK = NComp<LK>::v, NL = NComp<LL>::v;
+  const bool tri_launch = (X == Y);
+  const int64_t total = tri_launch ? tri((int64_t)nx) : (int64_t)nx * ny;
+  if (q >= total) return;
+  int64_t a, b;
+  if (tri_launch) {
+    a = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+    while (tri(a + 1) <= q) ++a;
+    while (tri(a) > q) --a;
+    b = q - tri(a);
+  } else {
+    a = q / ny;
+    b = q - a * ny;
+  }
+  const int i = pairs[2 * (x0 + a)], j = pairs[2 * (x0 + a) + 1];
+  const int k = pairs[2 * (y0 + b)], l = pairs[2 * (y0 + b) + 1];
+  const ShellD A = load_shell(atm, bas, env, ao_loc, i), B = load_shell(atm, bas, env, ao_loc, j);
+  const ShellD C = load_shell(atm, bas, env, ao_loc, k), D = load_shell(atm, bas, env, ao_loc, l);
+  double gam[NI * NJ * NK * NL];
+#pragma unroll
+  for (int fi = 0; fi < NI; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < NJ; ++fj)
+#pragma unroll
+      for (int fk = 0; fk < NK; ++fk)
+#pragma unroll
+        for (int fl = 0; fl < NL; ++fl) {
+          const int p = A.ao0 + fi, qq = B.ao0 + fj, r = C.ao0 + fk, s = D.ao0 + fl;
+          gam[((fi * NJ + fj) * NK + fk) * NL + fl] =
+              dm[p * N + qq] * dm[r * N + s] - hq * (dm[p * N + r] * dm[qq * N + s] + dm[p * N + s] * dm[qq * N + r]);
+        }
+  double gx[3][3];
+  shell_quartet_grad<LI, LJ, LK, LL>(A, B, C, D, tab, gam, gx);
+  const int orbit = (i == j ? 1 : 2) * (k == l ? 1 : 2) * (tri_launch && a == b ? 1 : 2);
+  const double f = 0.5 * orbit;
+  const int at[4] = {bas[i * BAS_SLOTS], bas[j * BAS_SLOTS], bas[k * BAS_SLOTS], bas[l * BAS_SLOTS]};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double gA = f * gx[0][d], gB = f * gx[1][d], gC = f * gx[2][d];
+    atomicAdd(g_lds + at[0] * 3 + d, grad_quant(gA));
+    atomicAdd(g_lds + at[1] * 3 + d, grad_quant(gB));
+    atomicAdd(g_lds + at[2] * 3 + d, grad_quant(gC));
+    atomicAdd(g_lds + at[3] * 3 + d, grad_quant(-(gA + gB + gC)));
+  }
+}
+
+// One launch per shell-pair class (X, Y), so each class has its own register
+// budget: workgroups take 256 consecutive quartets of the class.
+template <int X, int Y>
+__global__ void __launch_bounds__(256) eri_grad_kernel(const int* __restrict__ atm, int natm,
+                                                       const int* __restrict__ bas, const double* __restrict__ env,
+                                                       int nbas, const int* __restrict__ pairs, int x0, int nx, int y0,
+                                                       int ny, const double* __restrict__ dm, int N, double hq,
+                                                       double* __restrict__ grad) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* tab = reinterpret_cast<double*>(smem);
+  double* g_lds = tab + BOYS_NT * BOYS_NM;
+  int* ao_loc = reinterpret_cast<int*>(g_lds + 3 * natm);
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) tab[i] = g_boys[i];
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x) g_lds[i] = 0.0;
+  stage_ao_loc(ao_loc, bas, nbas);
+  __syncthreads();
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  s8_quartet_grad<X, Y>(bas, env, atm, ao_loc, pairs, x0, nx, y0, ny, q, tab, dm, N, hq, g_lds);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x)
+    if (g_lds[i] != 0.0) atomicAdd(grad + i, g_lds[i]);
+}
+
 template <typename T>
 static int eri_s8_all(const int* atm, const int* bas, const double* env, int nbas, const int* pairs, const int* cnt,
                       T* out, hipStream_t st) {
@@ -794,6 +1039,209 @@ static int eri_s8_all(const int* atm, const int* bas, const double* env, int nba
   return PSCF_OK;
 }
 
+// One-electron part of the nuclear gradient (the h1aos and s1 terms of
+// grad_elec, pyscf_ipu/nanoDFT/nanoDFT.py:422-433, with hcore_deriv
+// :463-470).  Over ordered shell pairs (i, j), with d_A the derivative with
+// respect to the centre A of shell i (2a (e + 1_d) - e_d (e - 1_d) on its
+// cartesian exponents):
+//   grad[atom(i)] += 2 sum D_ij d_A (T + V)_ij - 2 sum W_ij d_A S_ij,
+//   grad[C]       -= 2 sum D_ij d_A V^C_ij       for every nucleus C,
+// where V = sum_C V^C, V^C = -Z_C <i|1/|r - C||j>: its derivative with respect
+// to C is -(d_A + d_B) V^C, and summing over ordered pairs folds d_B into d_A
+// (D and the integrals are symmetric).  W is the energy-weighted density.
+template <int LA, int LB>
+__device__ inline void int1e_grad_pair(const ShellD& A, const ShellD& B, const int* __restrict__ atm, int natm,
+                                       const double* __restrict__ env, const double* __restrict__ tab,
+                                       const double* __restrict__ dm, const double* __restrict__ dme, int N,
+                                       double (&gA)[3], double* __restrict__ g_lds) {
+  constexpr int NA = NComp<LA>::v, NB = NComp<LB>::v;
+  constexpr int NR = (LA + 1 + LB) / 2 + 1;
+  const double angc = (LA ? ANG_P : ANG_S) * (LB ? ANG_P : ANG_S);
+  double Dab[NA][NB], Wab[NA][NB];
+#pragma unroll
+  for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+    for (int fb = 0; fb < NB; ++fb) {
+      Dab[fa][fb] = dm[(A.ao0 + fa) * N + B.ao0 + fb];
+      Wab[fa][fb] = dme[(A.ao0 + fa) * N + B.ao0 + fb];
+    }
+  double AB[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) AB[d] = A.r[d] - B.r[d];
+  const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gA[d] = 0.0;
+#pragma unroll 1
+  for (int pa = 0; pa < A.np; ++pa) {
+#pragma unroll 1
+    for (int pb = 0; pb < B.np; ++pb) {
+      const double a = A.e[pa], b = B.e[pb], p = a + b, rp = 1.0 / p, mu = a * b * rp;
+      const double cc = A.c[pa] * B.c[pb] * angc;
+      const double eab = exp(-mu * rab2);
+      double P[3], PA[3], PB[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        P[d] = (a * A.r[d] + b * B.r[d]) * rp;
+        PA[d] = P[d] - A.r[d];
+        PB[d] = P[d] - B.r[d];
+      }
+      // overlap moments S[d][i][j], i <= LA + 1, j <= LB + 2 (Obara-Saika)
+      double S[3][LA + 2][LB + 3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        S[d][0][0] = 1.0;
+#pragma unroll
+        for (int i = 0; i <= LA; ++i) S[d][i + 1][0] = PA[d] * S[d][i][0] + (i > 0 ? i * 0.5 * rp * S[d][i - 1][0] : 0.0);
+#pragma unroll
+        for (int j = 0; j <= LB + 1; ++j)
+#pragma unroll
+          for (int i = 0; i <= LA + 1; ++i)
+            S[d][i][j + 1] = PB[d] * S[d][i][j] +
+                             0.5 * rp * ((i > 0 ? i * S[d][i - 1][j] : 0.0) + (j > 0 ? j * S[d][i][j - 1] : 0.0));
+      }
+      // kinetic factor of one axis: b (2j + 1) S(i, j) - 2 b^2 S(i, j + 2) - j (j - 1) / 2 S(i, j - 2)
+      auto kin = [&](int d, int i, int j) {
+        return b * (2 * j + 1) * S[d][i][j] - 2.0 * b * b * S[d][i][j + 2] - (j >= 2 ? 0.5 * j * (j - 1) * S[d][i][j - 2] : 0.0);
+      };
+      const double s00 = cc * PI * rp * sqrt(PI * rp) * eab;
+      double dS[3] = {0.0, 0.0, 0.0}, dT[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+          int ea[3], eb[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            ea[d] = cexp<LA>(fa, d);
+            eb[d] = cexp<LB>(fb, d);
+          }
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+            const double sd = 2.0 * a * S[d][ea[d] + 1][eb[d]] - (ea[d] > 0 ? ea[d] * S[d][ea[d] - 1][eb[d]] : 0.0);
+            const double td = 2.0 * a * kin(d, ea[d] + 1, eb[d]) - (ea[d] > 0 ? ea[d] * kin(d, ea[d] - 1, eb[d]) : 0.0);
+            const double s1 = S[d1][ea[d1]][eb[d1]], s2 = S[d2][ea[d2]][eb[d2]];
+            dS[d] += Wab[fa][fb] * sd * s1 * s2;
+            dT[d] += Dab[fa][fb] * (td * s1 * s2 + sd * (kin(d1, ea[d1], eb[d1]) * s2 + s1 * kin(d2, ea[d2], eb[d2])));
+          }
+        }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gA[d] += 2.0 * s00 * (dT[d] - dS[d]);
+      // nuclear attraction, Rys quadrature over each nucleus
+      const double vpre = cc * 2.0 * PI * rp * eab;
+      for (int c = 0; c < natm; ++c) {
+        const double* Cn = env + atm[c * ATM_SLOTS + 1];
+        const double Z = (double)atm[c * ATM_SLOTS];
+        double PC[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) PC[d] = P[d] - Cn[d];
+        const double tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+        double f[2 * NR], u[NR], w[NR];
+        boys<double, 2 * NR - 1, BOYS_TAYLOR>(tn, tab, f);
+        rys_from_moments<NR>(f, u, w);
+        double dV[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const double h = 0.5 * (1.0 - u[r]) * rp;
+          double J[3][LA + 2][LB + 1];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double pa_ = PA[d] - u[r] * PC[d], pb_ = PB[d] - u[r] * PC[d];
+            J[d][0][0] = 1.0;
+#pragma unroll
+            for (int i = 0; i <= LA; ++i) J[d][i + 1][0] = pa_ * J[d][i][0] + (i > 0 ? i * h * J[d][i - 1][0] : 0.0);
+#pragma unroll
+            for (int j = 0; j < LB; ++j)
+#pragma unroll
+              for (int i = 0; i <= LA + 1; ++i)
+                J[d][i][j + 1] = pb_ * J[d][i][j] + h * ((i > 0 ? i * J[d][i - 1][j] : 0.0) + (j > 0 ? j * J[d][i][j - 1] : 0.0));
+          }
+#pragma unroll
+          for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+            for (int fb = 0; fb < NB; ++fb)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+                const int ed = cexp<LA>(fa, d), fd = cexp<LB>(fb, d);
+                const double jd = 2.0 * a * J[d][ed + 1][fd] - (ed > 0 ? ed * J[d][ed - 1][fd] : 0.0);
+                dV[d] += w[r] * Dab[fa][fb] * jd * J[d1][cexp<LA>(fa, d1)][cexp<LB>(fb, d1)] *
+                         J[d2][cexp<LA>(fa, d2)][cexp<LB>(fb, d2)];
+              }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double t = 2.0 * (-Z) * vpre * dV[d];  // 2 D d_A V^C
+          gA[d] += t;
+          atomicAdd(g_lds + c * 3 + d, grad_quant(-t));
+        }
+      }
+    }
+  }
+}
+
+// Lane per ordered shell pair (i, j), i = pair / nbas.
+__global__ void __launch_bounds__(256) int1e_grad_kernel(const int* __restrict__ atm, int natm,
+                                                         const int* __restrict__ bas, const double* __restrict__ env,
+                                                         int nbas, const double* __restrict__ dm,
+                                                         const double* __restrict__ dme, int N,
+                                                         double* __restrict__ grad) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* tab = reinterpret_cast<double*>(smem);
+  double* g_lds = tab + BOYS_NT * BOYS_NM;
+  int* ao_loc = reinterpret_cast<int*>(g_lds + 3 * natm);
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_NM; i += blockDim.x) tab[i] = g_boys[i];
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x) g_lds[i] = 0.0;
+  stage_ao_loc(ao_loc, bas, nbas);
+  __syncthreads();
+  const int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pr < (int64_t)nbas * nbas) {
+    const int i = (int)(pr / nbas), j = (int)(pr - (int64_t)i * nbas);
+    const ShellD A = load_shell(atm, bas, env, ao_loc, i), B = load_shell(atm, bas, env, ao_loc, j);
+    const int la = bas[i * BAS_SLOTS + 1], lb = bas[j * BAS_SLOTS + 1];
+    double gA[3];
+    if (la == 0 && lb == 0) int1e_grad_pair<0, 0>(A, B, atm, natm, env, tab, dm, dme, N, gA, g_lds);
+    else if (la == 0) int1e_grad_pair<0, 1>(A, B, atm, natm, env, tab, dm, dme, N, gA, g_lds);
+    else if (lb == 0) int1e_grad_pair<1, 0>(A, B, atm, natm, env, tab, dm, dme, N, gA, g_lds);
+    else int1e_grad_pair<1, 1>(A, B, atm, natm, env, tab, dm, dme, N, gA, g_lds);
+    const int at = bas[i * BAS_SLOTS];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) atomicAdd(g_lds + at * 3 + d, grad_quant(gA[d]));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x)
+    if (g_lds[i] != 0.0) atomicAdd(grad + i, g_lds[i]);
+}
+
+// The six shell-pair classes of the class-sorted pair list of pscf_eri_s8,
+// one launch each.
+template <int X, int Y>
+static int eri_grad_class(const int* atm, int natm, const int* bas, const double* env, int nbas, const int* pairs,
+                          const int* cnt, const double* dm, int N, double hyb, double* grad, hipStream_t st) {
+  const int off[3] = {0, cnt[0], cnt[0] + cnt[1]};
+  const int64_t nq = s8_class_quartets(X, Y, cnt);
+  if (nq == 0) return PSCF_OK;
+  const int64_t blocks = (nq + 255) / 256;
+  PSCF_REQUIRE(blocks < (1ll << 31), "pscf_grad_eri: too many quartets");
+  const size_t shm = sizeof(double) * (BOYS_NT * BOYS_NM + 3 * (size_t)natm) + sizeof(int) * (nbas + 1);
+  PSCF_SET_LDS((eri_grad_kernel<X, Y>), shm);
+  eri_grad_kernel<X, Y><<<(unsigned)blocks, 256, shm, st>>>(atm, natm, bas, env, nbas, pairs, off[X], cnt[X], off[Y],
+                                                            cnt[Y], dm, N, hyb / 4.0, grad);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+static int eri_grad_all(const int* atm, int natm, const int* bas, const double* env, int nbas, const int* pairs,
+                        const int* cnt, const double* dm, int N, double hyb, double* grad, hipStream_t st) {
+  int rc = eri_grad_class<2, 2>(atm, natm, bas, env, nbas, pairs, cnt, dm, N, hyb, grad, st);
+  if (rc == PSCF_OK) rc = eri_grad_class<2, 1>(atm, natm, bas, env, nbas, pairs, cnt, dm, N, hyb, grad, st);
+  if (rc == PSCF_OK) rc = eri_grad_class<2, 0>(atm, natm, bas, env, nbas, pairs, cnt, dm, N, hyb, grad, st);
+  if (rc == PSCF_OK) rc = eri_grad_class<1, 1>(atm, natm, bas, env, nbas, pairs, cnt, dm, N, hyb, grad, st);
+  if (rc == PSCF_OK) rc = eri_grad_class<1, 0>(atm, natm, bas, env, nbas, pairs, cnt, dm, N, hyb, grad, st);
+  if (rc == PSCF_OK) rc = eri_grad_class<0, 0>(atm, natm, bas, env, nbas, pairs, cnt, dm, N, hyb, grad, st);
+  return rc;
+}
+
 }  // namespace pscf
 
 using namespace pscf;
@@ -813,6 +1261,37 @@ extern "C" int pscf_eri_s8(int dtype, const int* d_atm, int natm, const int* d_b
   hipStream_t st = (hipStream_t)stream;
   if (dtype == PSCF_F64) return eri_s8_all<double>(d_atm, d_bas, d_env, nbas, d_pairs, h_class_counts, (double*)d_out, st);
   return eri_s8_all<float>(d_atm, d_bas, d_env, nbas, d_pairs, h_class_counts, (float*)d_out, st);
+}
+
+extern "C" int pscf_grad_eri(const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env,
+                             const int* d_pairs, const int* h_class_counts, int nao, const double* d_dm, double hyb,
+                             double* d_grad, void* stream) {
+  PSCF_REQUIRE(natm > 0 && natm <= 4096 && nbas > 0 && nbas <= 4096 && nao > 0, "pscf_grad_eri: bad molecule");
+  PSCF_REQUIRE(d_atm && d_bas && d_env && d_pairs && h_class_counts && d_dm && d_grad, "pscf_grad_eri: null pointer");
+  int64_t npair = (int64_t)h_class_counts[0] + h_class_counts[1] + h_class_counts[2];
+  PSCF_REQUIRE(h_class_counts[0] >= 0 && h_class_counts[1] >= 0 && h_class_counts[2] >= 0 &&
+                   npair == tri((int64_t)nbas),
+               "pscf_grad_eri: class counts must cover all nbas(nbas+1)/2 shell pairs");
+  int rc = ensure_boys_table();
+  if (rc) return rc;
+  return eri_grad_all(d_atm, natm, d_bas, d_env, nbas, d_pairs, h_class_counts, d_dm, nao, hyb, d_grad,
+                      (hipStream_t)stream);
+}
+
+extern "C" int pscf_grad_int1e(const int* d_atm, int natm, const int* d_bas, int nbas, const double* d_env, int nao,
+                               const double* d_dm, const double* d_dme, double* d_grad, void* stream) {
+  PSCF_REQUIRE(natm > 0 && natm <= 4096 && nbas > 0 && nbas <= 4096 && nao > 0, "pscf_grad_int1e: bad molecule");
+  PSCF_REQUIRE(d_atm && d_bas && d_env && d_dm && d_dme && d_grad, "pscf_grad_int1e: null pointer");
+  int rc = ensure_boys_table();
+  if (rc) return rc;
+  const int64_t npairs = (int64_t)nbas * nbas;
+  const size_t shm = sizeof(double) * (BOYS_NT * BOYS_NM + 3 * (size_t)natm) + sizeof(int) * (nbas + 1);
+  PSCF_SET_LDS(int1e_grad_kernel, shm);
+  int1e_grad_kernel<<<(unsigned)((npairs + 255) / 256), 256, shm, (hipStream_t)stream>>>(d_atm, natm, d_bas, d_env,
+                                                                                        nbas, d_dm, d_dme, nao,
+                                                                                        d_grad);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
 }
 
 extern "C" int pscf_eri_quartets(int dtype, const int* d_atm, int natm, const int* d_bas, int nbas,

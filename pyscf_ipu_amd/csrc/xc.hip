@@ -838,6 +838,221 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
   return reduce_parts<T, true>(a.part_v, nb, N, (T*)((char*)ws + lay.lvl_off), vxc, a.part_e, exc, st);
 }
 
+// ------------------------------------------------------- nuclear gradient ----
+// XC part of the nuclear gradient (grad_elec, pyscf_ipu/nanoDFT/nanoDFT.py:
+// 403-421: vmat and the second-derivative term, without grid response).  With
+// X_i = sum_j D_ij phi_j, the potential weights c = w [v_rho / 2, 4 v_sigma
+// grad rho / 2] of the energy path, Y_j = c_0 phi_j + sum_e c_e d_e phi_j and
+// YD_i = sum_j D_ij Y_j, each AO i collects
+//   t_s[i] = sum_p d_s phi_i YD_i + (c_0 d_s phi_i + sum_e c_e d_s d_e phi_i) X_i
+// and the atom A of AO i gets -2 t_s[i].  One workgroup per tile of TP
+// points: the AO values, gradients and second derivatives are evaluated in
+// the kernel (LDS [10][TP][N]), X, Y and YD live in LDS, D in LDS when it
+// fits; per-tile partials t[tile][N][3] are summed in tile order by
+// xc_grad_reduce_kernel.
+constexpr int XCG_THREADS = 256;
+
+// second-derivative component of (s, e) after value and gradient: xx xy xz yy yz zz
+__device__ inline int d2_comp(int s, int e) {
+  const int a = s < e ? s : e, b = s < e ? e : s;
+  return 4 + (a == 0 ? b : (a == 1 ? 2 + b : 5));
+}
+
+__global__ void __launch_bounds__(XCG_THREADS) xc_grad_kernel(const double* __restrict__ coords,
+                                                              const double* __restrict__ weights, int g,
+                                                              const int* __restrict__ atm,
+                                                              const int* __restrict__ bas, int nbas,
+                                                              const double* __restrict__ env, int N,
+                                                              const double* __restrict__ dm_g, int dm_lds, int TP,
+                                                              double eps, double* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int TN = TP * N;
+  double* Phi = reinterpret_cast<double*>(smem);  // [10][TP][N]
+  double* X = Phi + 10 * TN;                       // [TP][N]
+  double* Y = X + TN;                              // [TP][N]
+  double* YD = Y + TN;                             // [TP][N]
+  double* coef = YD + TN;                          // [4][TP]
+  double* Dl = coef + 4 * TP;                      // [N][N] when dm_lds
+  int* ao_loc = reinterpret_cast<int*>(Dl + (dm_lds ? N * N : 0));
+  const double* D = dm_lds ? Dl : dm_g;
+  const int tid = threadIdx.x;
+  if (dm_lds)
+    for (int x = tid; x < N * N; x += blockDim.x) Dl[x] = dm_g[x];
+  if (tid == 0) {
+    int acc = 0;
+    for (int i = 0; i < nbas; ++i) {
+      ao_loc[i] = acc;
+      acc += 2 * bas[i * BAS_SLOTS + 1] + 1;
+    }
+    ao_loc[nbas] = acc;
+  }
+  __syncthreads();
+  const int p0 = blockIdx.x * TP;
+  // AO values, gradients and second derivatives: item (point, shell)
+  for (int it = tid; it < TP * nbas; it += blockDim.x) {
+    const int p = it / nbas, ib = it - p * nbas;
+    const int pt = p0 + p;
+    const int* b = bas + ib * BAS_SLOTS;
+    const int mu = ao_loc[ib];
+    const int l = b[1];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, dd[3] = {0.0, 0.0, 0.0};
+    if (pt < g) {
+      const double* R = env + atm[b[0] * ATM_SLOTS + 1];
+      const double* e = env + b[5];
+      const double* c = env + b[6];
+      for (int d = 0; d < 3; ++d) dd[d] = coords[3 * pt + d] - R[d];
+      const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+      for (int k = 0; k < b[2]; ++k) {
+        const double ek = c[k] * exp(-e[k] * r2);
+        f0 += ek;
+        f1 += e[k] * ek;
+        f2 += e[k] * e[k] * ek;
+      }
+      const double ang = l ? ANG_P : ANG_S;
+      f0 *= ang;
+      f1 *= -2.0 * ang;
+      f2 *= 4.0 * ang;
+    }
+    if (l == 0) {
+      Phi[p * N + mu] = f0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) Phi[(1 + x) * TN + p * N + mu] = f1 * dd[x];
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = x; y < 3; ++y) Phi[d2_comp(x, y) * TN + p * N + mu] = f2 * dd[x] * dd[y] + (x == y ? f1 : 0.0);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int o = p * N + mu + a;
+        Phi[o] = dd[a] * f0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) Phi[(1 + x) * TN + o] = (a == x ? f0 : 0.0) + dd[a] * f1 * dd[x];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+          for (int y = x; y < 3; ++y) {
+            double v = dd[a] * (f2 * dd[x] * dd[y] + (x == y ? f1 : 0.0));
+            if (a == x) v += f1 * dd[y];
+            if (a == y) v += f1 * dd[x];
+            Phi[d2_comp(x, y) * TN + o] = v;
+          }
+      }
+    }
+  }
+  __syncthreads();
+  // X = phi0 D
+  for (int it = tid; it < TN; it += blockDim.x) {
+    const int p = it / N, j = it - p * N;
+    double v = 0.0;
+    for (int k = 0; k < N; ++k) v += Phi[p * N + k] * D[k * N + j];
+    X[it] = v;
+  }
+  __syncthreads();
+  // rho_c = sum_j X phi_c, B3LYP, potential weights
+  if (tid < TP) {
+    const int pt = p0 + tid;
+    double rc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double v = 0.0;
+      for (int j = 0; j < N; ++j) v += X[tid * N + j] * Phi[c * TN + tid * N + j];
+      rc[c] = v;
+    }
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    if (pt < g) {
+      double e, vr, vn;
+      b3lyp_point<double>(rc[0], rc[1], rc[2], rc[3], eps, e, vr, vn);
+      const double wt = weights[pt];
+      c0 = 0.5 * vr * wt;
+      const double f = 4.0 * vn * wt;
+      c1 = f * rc[1];
+      c2 = f * rc[2];
+      c3 = f * rc[3];
+    }
+    coef[tid] = c0;
+    coef[TP + tid] = c1;
+    coef[2 * TP + tid] = c2;
+    coef[3 * TP + tid] = c3;
+  }
+  __syncthreads();
+  for (int it = tid; it < TN; it += blockDim.x) {
+    const int p = it / N;
+    Y[it] = coef[p] * Phi[it] + coef[TP + p] * Phi[TN + it] + coef[2 * TP + p] * Phi[2 * TN + it] +
+            coef[3 * TP + p] * Phi[3 * TN + it];
+  }
+  __syncthreads();
+  for (int it = tid; it < TN; it += blockDim.x) {
+    const int p = it / N, j = it - p * N;
+    double v = 0.0;
+    for (int k = 0; k < N; ++k) v += Y[p * N + k] * D[k * N + j];
+    YD[it] = v;
+  }
+  __syncthreads();
+  // per AO, summed over the tile's points in order
+  for (int i = tid; i < N; i += blockDim.x) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int p = 0; p < TP; ++p) {
+      const int o = p * N + i;
+      const double x = X[o], yd = YD[o], c0 = coef[p];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        double z = c0 * Phi[(1 + s) * TN + o];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) z += coef[(1 + e) * TP + p] * Phi[d2_comp(s, e) * TN + o];
+        t[s] += Phi[(1 + s) * TN + o] * yd + z * x;
+      }
+    }
+    double* dst = part + ((int64_t)blockIdx.x * N + i) * 3;
+    dst[0] = t[0];
+    dst[1] = t[1];
+    dst[2] = t[2];
+  }
+}
+
+// grad[atom][s] += -2 sum_{AO i of atom} sum_tiles part[tile][i][s] (fixed order)
+__global__ void __launch_bounds__(256) xc_grad_reduce_kernel(const double* __restrict__ part, int ntiles, int N,
+                                                             const int* __restrict__ bas, int nbas, int natm,
+                                                             double* __restrict__ grad) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* tot = reinterpret_cast<double*>(smem);  // [N][3]
+  for (int x = threadIdx.x; x < 3 * N; x += blockDim.x) {
+    double v = 0.0;
+    for (int t = 0; t < ntiles; ++t) v += part[(int64_t)t * 3 * N + x];
+    tot[x] = v;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < 3 * natm; x += blockDim.x) {
+    const int a = x / 3, s = x - 3 * a;
+    double v = 0.0;
+    int mu = 0;
+    for (int ib = 0; ib < nbas; ++ib) {
+      const int w = 2 * bas[ib * BAS_SLOTS + 1] + 1;
+      if (bas[ib * BAS_SLOTS] == a)
+        for (int k = 0; k < w; ++k) v += tot[(mu + k) * 3 + s];
+      mu += w;
+    }
+    grad[x] += -2.0 * v;
+  }
+}
+
+static size_t xc_grad_lds(int N, int nbas, int TP, bool dm_lds) {
+  return sizeof(double) * (13 * (size_t)TP * N + 4 * (size_t)TP + (dm_lds ? (size_t)N * N : 0)) +
+         sizeof(int) * (nbas + 1);
+}
+
+// Largest tile (points) whose LDS fits, D in LDS if that still allows 8 points.
+static void xc_grad_shape(int N, int nbas, int& TP, bool& dm_lds) {
+  const size_t cap = (size_t)xc_max_lds();
+  dm_lds = xc_grad_lds(N, nbas, 8, true) <= cap;
+  TP = 0;
+  for (int t = 32; t >= 1; t >>= 1)
+    if (xc_grad_lds(N, nbas, t, dm_lds) <= cap) {
+      TP = t;
+      break;
+    }
+}
+
 }  // namespace pscf
 
 using namespace pscf;
@@ -1024,4 +1239,37 @@ extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, 
                                   split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
   return batch_xc_np<float, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, split,
                                 d_part_v, d_part_e, part_v_elems, part_e_elems, st);
+}
+
+extern "C" size_t pscf_grad_xc_workspace_bytes(int nao, int nbas, int g) {
+  int TP;
+  bool dm_lds;
+  xc_grad_shape(nao, nbas, TP, dm_lds);
+  if (TP == 0) return 0;
+  return sizeof(double) * 3 * (size_t)nao * ((g + TP - 1) / TP);
+}
+
+extern "C" int pscf_grad_xc(const double* d_coords, const double* d_weights, int g, const int* d_atm, int natm,
+                            const int* d_bas, int nbas, const double* d_env, int nao, const double* d_dm, double eps,
+                            void* d_ws, size_t ws_bytes, double* d_grad, void* stream) {
+  PSCF_REQUIRE(g > 0 && natm > 0 && nbas > 0 && nao > 0 && nao <= 4096, "pscf_grad_xc: bad sizes");
+  PSCF_REQUIRE(d_coords && d_weights && d_atm && d_bas && d_env && d_dm && d_ws && d_grad, "pscf_grad_xc: null pointer");
+  int TP;
+  bool dm_lds;
+  xc_grad_shape(nao, nbas, TP, dm_lds);
+  PSCF_REQUIRE(TP > 0, "pscf_grad_xc: nao %d does not fit the LDS", nao);
+  const int ntiles = (g + TP - 1) / TP;
+  PSCF_REQUIRE(ws_bytes >= sizeof(double) * 3 * (size_t)nao * ntiles, "pscf_grad_xc: workspace too small");
+  PSCF_REQUIRE((size_t)sizeof(double) * 3 * nao <= (size_t)xc_max_lds(), "pscf_grad_xc: nao %d too large", nao);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = xc_grad_lds(nao, nbas, TP, dm_lds);
+  PSCF_SET_LDS(xc_grad_kernel, lds);
+  xc_grad_kernel<<<ntiles, XCG_THREADS, lds, st>>>(d_coords, d_weights, g, d_atm, d_bas, nbas, d_env, nao, d_dm,
+                                                   dm_lds ? 1 : 0, TP, eps, (double*)d_ws);
+  PSCF_HIP(hipGetLastError());
+  const size_t lds2 = sizeof(double) * 3 * (size_t)nao;
+  PSCF_SET_LDS(xc_grad_reduce_kernel, lds2);
+  xc_grad_reduce_kernel<<<1, 256, lds2, st>>>((const double*)d_ws, ntiles, nao, d_bas, nbas, natm, d_grad);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
 }

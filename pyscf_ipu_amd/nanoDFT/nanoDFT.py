@@ -7,6 +7,9 @@ Kept names and meanings (pyscf_ipu/nanoDFT/nanoDFT.py):
   nanoDFT(mol, opts) -> (energies (its, 5) eV, (logged_energies, hlgaps,
                           mo_energy, mo_coeff, grid_coords, grid_weights))  :259-336
   hlgap(L_inv, H, n_electrons_half, _np)                        :366-368
+  grad_nuc(charges, coords), grad(mol, coords, weight, mo_coeff,
+           mo_energy) -> (natm, 3) Ha/Bohr                       :435-477
+  structure_optimization (the loop main runs with the option)   :645-691
   nanoDFT_options(...)                                          :548-618
   main()                                                        :620-643
 
@@ -442,6 +445,93 @@ def nanoDFT(mol, opts):
                       grid_weights)
 
 
+# ------------------------------------------------------------------ gradients ---
+def grad_nuc(charges, coords):
+    """d E_nuc / d R, (natm, 3) Ha/Bohr (nanoDFT.py:435-449; host, natm^2 terms)."""
+    charges = np.asarray(charges, dtype=np.float64)
+    coords = np.asarray(coords, dtype=np.float64)
+    natm = len(charges)
+    diff = coords[None, :, :] - coords[:, None, :]
+    r3 = np.linalg.norm(diff, axis=2) ** 3
+    r3[np.arange(natm), np.arange(natm)] = np.inf
+    return np.sum(-(charges[:, None] * charges[None, :])[:, :, None] * diff / r3[:, :, None], axis=0)
+
+
+def grad(mol, coords, weight, mo_coeff, mo_energy):
+    """Nuclear gradient of the RKS B3LYP energy, (natm, 3) Ha/Bohr, as
+    nanoDFT.py:451-477 (no grid-response term: the grid `coords`, `weight` of
+    the SCF stay fixed).  The reference builds int2e_ip1, int1e_ip* and the
+    deriv-2 AO values with PySCF and contracts them with dm0 = 2 C_occ C_occ^T
+    and dme0 = 2 C_occ e_occ C_occ^T (grad_elec :403-433).  Here dm0 and dme0
+    are two FP64 GEMMs on the device and each integral class is contracted as
+    it is generated: pscf_grad_int1e (overlap, kinetic, nuclear attraction and
+    its operator derivative), pscf_grad_eri (derivative ERIs over the unique
+    shell quartets with the J and exchange densities), pscf_grad_xc (AO second
+    derivatives in the kernel, B3LYP potential); the nuclear repulsion term
+    (grad_nuc) is added on the host."""
+    device.require_gpu()
+    N, natm = mol.nao_nr(), mol.natm
+    nocc = mol.nelectron // 2
+    mo_coeff = np.asarray(mo_coeff, dtype=np.float64)
+    e_occ = np.asarray(mo_energy, dtype=np.float64)[:nocc]
+    c_occ = device.to_dev(np.ascontiguousarray(mo_coeff[:, :nocc]))
+    c2 = device.to_dev(np.ascontiguousarray(2.0 * mo_coeff[:, :nocc]))           # 2 C_occ
+    c2e = device.to_dev(np.ascontiguousarray(2.0 * mo_coeff[:, :nocc] * e_occ))  # 2 C_occ e_occ
+    f64 = torch.float64
+    dm0, dme0 = device.empty((N, N), f64), device.empty((N, N), f64)
+    P, st = device.ptr, device.stream()
+    G_F64 = 3
+    _lib.call("pscf_gemm", G_F64, 1, 0, 1, 0, N, N, nocc, P(c2), nocc, P(c_occ), nocc, P(dm0), N, 1, st)
+    _lib.call("pscf_gemm", G_F64, 1, 0, 1, 0, N, N, nocc, P(c2e), nocc, P(c_occ), nocc, P(dme0), N, 1, st)
+    atm, bas, env = direct._mol_device(mol)
+    g_dev = device.zeros((natm, 3), f64)
+    _lib.call("pscf_grad_int1e", P(atm), natm, P(bas), mol.nbas, P(env), N, P(dm0), P(dme0), P(g_dev), st)
+    pairs, counts = direct.shell_pairs_by_class(mol)
+    dp = device.to_dev(pairs)
+    _lib.call("pscf_grad_eri", P(atm), natm, P(bas), mol.nbas, P(env), P(dp), (ctypes.c_int * 3)(*counts), N,
+              P(dm0), HYB_B3LYP, P(g_dev), st)
+    coords_d = device.to_dev(np.ascontiguousarray(coords, dtype=np.float64))
+    w_d = device.to_dev(np.ascontiguousarray(weight, dtype=np.float64))
+    g = len(w_d)
+    wsb = _lib.lib().pscf_grad_xc_workspace_bytes(N, mol.nbas, g)
+    ws = device.workspace(wsb)
+    _lib.call("pscf_grad_xc", P(coords_d), P(w_d), g, P(atm), natm, P(bas), mol.nbas, P(env), N, P(dm0),
+              EPSILON_B3LYP, P(ws), wsb, P(g_dev), st)
+    out = g_dev.cpu().numpy()
+    return out + grad_nuc(mol.atom_charges(), mol.atom_coords())
+
+
+def structure_optimization(opts, steps: int = 20, mol_str: str = None):
+    """The reference's structure-optimisation loop (nanoDFT.py:645-691, run
+    when opts.structure_optimization is set): 6-31G, a slightly randomised
+    water dimer (positions [[0,1,1], ..., [0,6,6]] Angstrom plus normal(0, 0.3)
+    noise from numpy seed 42), `steps` times: nanoDFT, grad, then positions
+    -= forces, with the reference's units (positions in Angstrom, gradient in
+    Ha/Bohr).  The reference renders every step with mogli/imageio; here the
+    energies (eV, last SCF iteration) and the positions of each step are
+    returned.  `mol_str` replaces the starting geometry."""
+    opts = opts._replace(basis="6-31G")
+    if mol_str is None:
+        p = np.array([[0, 1, 1], [0, 2, 2], [0, 3, 3], [0, 4, 4], [0, 5, 5], [0, 6, 6]], dtype=np.float64)
+        np.random.seed(42)
+        p = p + np.random.normal(0, 0.3, p.shape)
+        A = ["H", "O", "H", "H", "O", "H"]
+    else:
+        parts = [q.split() for q in mol_str.split(";") if q.strip()]
+        A = [q[0] for q in parts]
+        p = np.array([[float(x) for x in q[1:4]] for q in parts])
+    energies, positions = [], []
+    for _ in range(steps):
+        s = "; ".join(f"{a} {float(x[0])!r} {float(x[1])!r} {float(x[2])!r}" for a, x in zip(A, p))
+        mol = build_mol(s, opts.basis)
+        E, (_logged, _gaps, mo_energy, mo_coeff, grid_coords, grid_weights) = nanoDFT(mol, opts)
+        energies.append(float(E[-1, 0]))
+        positions.append(p.copy())
+        forces = grad(mol, grid_coords, grid_weights, mo_coeff, mo_energy)
+        p = p - forces
+    return np.array(energies), np.array(positions)
+
+
 _OPTION_FIELDS = ["its", "mol_str", "float32", "basis", "xc", "backend", "level", "multv", "intv", "threads",
                   "threads_int", "diis", "structure_optimization", "eri_threshold", "ao_threshold", "batches",
                   "ndevices", "dense_ERI", "v", "profile", "vis_num_error", "molecule_name", "conv_tol"]
@@ -484,9 +574,15 @@ def main(argv=None):
     p.add_argument("--level", type=int, default=0)
     p.add_argument("--conv_tol", type=float, default=0.0)
     p.add_argument("--no-diis", dest="diis", action="store_false")
+    p.add_argument("--structure_optimization", action="store_true")
     a = p.parse_args(argv)
     opts, mol_str = nanoDFT_options(its=a.its, mol_str=a.mol_str, float32=a.float32, basis=a.basis, level=a.level,
-                                    diis=a.diis, conv_tol=a.conv_tol)
+                                    diis=a.diis, conv_tol=a.conv_tol, structure_optimization=a.structure_optimization)
+    if opts.structure_optimization:
+        E, _pos = structure_optimization(opts)
+        for i, e in enumerate(E):
+            print(f"step {i:3d}  E = {e:.8f} eV")
+        return E
     mol = build_mol(mol_str, opts.basis)
     print(f"Number of Atomic Orbitals\t{mol.nao_nr():15d}")
     print(f"Number of electrons\t{mol.nelectron:15d}")

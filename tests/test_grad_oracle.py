@@ -102,10 +102,11 @@ def test_grad_nuc_closed_form():
             assert abs(g[a, x] - (e_nuc(h) - e_nuc(-h)) / (2 * h)) < 1e-8
 
 
-def test_gradient_against_energy_finite_differences():
+@pytest.mark.parametrize("basis", ["sto-3g", "6-31g"])
+def test_gradient_against_energy_finite_differences(basis):
     """dE/dR of the converged B3LYP energy on a frozen grid, for every atom
-    and axis of a distorted water molecule (STO-3G)."""
-    mol = build_mol(WATER)
+    and axis of a distorted water molecule."""
+    mol = build_mol(WATER, basis)
     grid = grids.build_grid(mol, 0)
     ref = rks.scf(mol, grid=grid, its=60, conv_tol=1e-12)
     assert ref.converged
@@ -116,7 +117,7 @@ def test_gradient_against_energy_finite_differences():
         for axis in range(3):
             e = []
             for s in (h, -h):
-                m = build_mol(_displaced(WATER, atom, axis, s))
+                m = build_mol(_displaced(WATER, atom, axis, s), basis)
                 e.append(rks.scf(m, grid=grid, its=60, conv_tol=1e-12, dm0=dm0).energy)
             fd = (e[0] - e[1]) / (2 * h)
             # measured agreement 1.6e-8 Ha/Bohr (FD truncation and SCF convergence)
