@@ -70,8 +70,8 @@ def per_launch_work(kernel: str, mols, dtype: str) -> dict:
     es = 8 if dtype == "fp64" else 4
     if kernel == "eri_s8":
         f = float(np.mean([eri_s8_flops(m) for m in mols]))
-        return {"bound": "mfma", "flops": f, "model": "Rys ERI op-count model (pyscf_ipu_amd/roofline.py), "
-                                                     "FP VALU; peak = MI355X dense FP64/FP32 rate"}
+        return {"bound": "valu", "flops": f, "model": "Rys ERI op-count model (pyscf_ipu_amd/roofline.py), "
+                                                     "FP VALU; peak = MI355X FP64/FP32 vector rate"}
     if kernel == "jk_s8":
         b = float(np.mean([(m.nao_nr() * (m.nao_nr() + 1) // 2) * (m.nao_nr() * (m.nao_nr() + 1) // 2 + 1) // 2
                            for m in mols])) * es
@@ -82,5 +82,8 @@ def per_launch_work(kernel: str, mols, dtype: str) -> dict:
         return {"bound": "hbm", "bytes": b, "model": "4 g N + g elements (AO values/gradients, weights)"}
     if kernel == "scf_step":
         f = float(np.mean([38.0 * m.nao_nr() ** 3 for m in mols]))
-        return {"bound": "mfma", "flops": f, "model": "38 N^3 FP64"}
+        # FP64 arithmetic (MFMA products plus VALU rotations) whose peak is the
+        # MI355X FP64 rate; what limits it is latency: one workgroup per molecule
+        return {"bound": "mfma", "flops": f, "model": "38 N^3 FP64",
+                "limiter": "latency: one workgroup (one CU) per molecule, dependent Jacobi rounds"}
     raise KeyError(kernel)

@@ -20,6 +20,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PLAIN = json.load(open(os.path.join(HERE, "golden", "plain_sto3g_l0.json")))["molecules"]
 PICK = [0, 3, 7, 11, 15, 19, 23, 27]
 LARGE_PATH = os.path.join(HERE, "golden", "large_631g_l1.json")
+# The FP32 forms stop at |dE| < 1e-6 Ha (plain: 1e-5), above their rounding
+# noise.  The HOMO-LUMO gap converges more slowly than the energy: molecule 3 of
+# the set stops with a gap 1.6 mHa from the converged one, and the FP64 oracle
+# stopped at the same |dE| < 1e-6 gives that same gap (0.04188 against 0.04348
+# Ha fully converged).  The gap error is stated, the energy is the gate.
+FP32_GAP_TOL = 3e-3
 
 
 def _gold(key):
@@ -44,10 +50,7 @@ def test_plain_at_benchmark_size(hip, dtype):
         assert np.abs(gap - _gold("hlgap")).max() < 1e-5
     else:
         assert np.abs(E - _gold("energy")).max() < 1e-3, np.abs(E - _gold("energy"))
-        # the gap converges linearly where the energy converges quadratically:
-        # at the 1e-5 Ha energy stop of this form it is within a few mHa
-        # (measured: at most 1.6 mHa, on a molecule with a 0.043 Ha gap)
-        assert np.abs(gap - _gold("hlgap")).max() < 3e-3
+        assert np.abs(gap - _gold("hlgap")).max() < FP32_GAP_TOL
 
 
 @pytest.mark.parametrize("dtype", ["fp64", "fp32"])
@@ -65,7 +68,7 @@ def test_batch_at_benchmark_size(hip, dtype):
         assert dg.max() < 1e-5
     else:
         assert dE.max() < 1e-3, dE
-        assert dg.max() < 1e-4
+        assert dg.max() < FP32_GAP_TOL
 
 
 @pytest.mark.skipif(not os.path.exists(LARGE_PATH), reason="large-molecule golden file not generated")
