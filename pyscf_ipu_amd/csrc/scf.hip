@@ -12,6 +12,7 @@
 // iterations, so a few sweeps suffice.
 #include "batch.h"
 #include "common.h"
+#include "wave.h"
 
 namespace pscf {
 
@@ -63,16 +64,78 @@ __device__ inline void lds_matmul_mfma(const double* __restrict__ X, const doubl
   }
 }
 
+// The same product for 32 < N <= 64 by 4 waves (one per SIMD): wave w
+// computes the 32 x 32 output block (w / 2, w % 2) as 2 x 2 tiles of 16, four
+// independent MFMA chains sharing each operand fragment, so that the SIMD's
+// matrix unit (64 cycles per FP64 16x16x4 operation) bounds the time rather
+// than LDS traffic (half that of one tile per wave).  Tiles wholly outside N
+// x N are skipped.
+template <bool TX, bool TY>
+__device__ inline void lds_matmul_mfma4(const double* __restrict__ X, const double* __restrict__ Y,
+                                        double* __restrict__ C, int N, int ld, int ldc) {
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w >= 4) return;  // wave-uniform
+  const int li = lane & 15, lg = lane >> 4;
+  const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+  const bool t1r = r0 + 16 < N, t1c = c0 + 16 < N;  // second row / column tile inside N
+  int rowc[2], colc[2];
+  bool rv[2], cv[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int row = r0 + 16 * t + li, col = c0 + 16 * t + li;
+    rv[t] = row < N;
+    cv[t] = col < N;
+    rowc[t] = rv[t] ? row : N - 1;
+    colc[t] = cv[t] ? col : N - 1;
+  }
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+  for (int k0 = 0; k0 < N; k0 += 4) {
+    const int k = k0 + lg;
+    const bool kv = k < N;
+    const int kc = kv ? k : N - 1;
+    double a[2], b[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const double av = TX ? X[kc * ld + rowc[t]] : X[rowc[t] * ld + kc];
+      const double bv = TY ? Y[colc[t] * ld + kc] : Y[kc * ld + colc[t]];
+      a[t] = (rv[t] && kv) ? av : 0.0;
+      b[t] = (cv[t] && kv) ? bv : 0.0;
+    }
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], acc[0][0], 0, 0, 0);
+    if (t1c) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[1], acc[0][1], 0, 0, 0);
+    if (t1r) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[0], acc[1][0], 0, 0, 0);
+    if (t1r && t1c) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], acc[1][1], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + 16 * i + lg + 4 * q, c = c0 + 16 * j + li;
+        if (r < N && c < N) C[r * ldc + c] = acc[i][j][q];
+      }
+}
+
 // C = op(X) op(Y), N x N, X and Y in LDS (leading dimension ld), C in LDS or
 // global (leading dimension ldc), never aliasing X or Y.  N <= 64 with 1024
-// threads takes the MFMA form above.  Otherwise each thread owns rows
+// threads takes the MFMA forms above.  Otherwise each thread owns rows
 // {i, i + h} x columns {j, j + h} (h = ceil(N / 2)), so the lanes of a wave walk
 // consecutive columns and share rows.
 template <bool TX, bool TY>
 __device__ inline void lds_matmul(const double* __restrict__ X, const double* __restrict__ Y, double* __restrict__ C,
                                   int N, int ld, int ldc) {
   if (N <= 64 && blockDim.x == 1024) {
-    lds_matmul_mfma<TX, TY>(X, Y, C, N, ld, ldc);
+    if (N > 32)
+      lds_matmul_mfma4<TX, TY>(X, Y, C, N, ld, ldc);
+    else
+      lds_matmul_mfma<TX, TY>(X, Y, C, N, ld, ldc);
     return;
   }
   const int h = (N + 1) >> 1;
@@ -109,13 +172,68 @@ __device__ inline void lds_load(double* __restrict__ dst, const double* __restri
   }
 }
 
+// Sum over the 16 lanes of each DPP row (lanes 16 r .. 16 r + 15), in every
+// lane of the row: quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror (VALU moves, no LDS traffic).  The whole wavefront must be active.
+__device__ inline double row16_sum(double v) {
+  v = dpp_add<0xb1, 0xf>(v);
+  v = dpp_add<0x4e, 0xf>(v);
+  v = dpp_add<0x141, 0xf>(v);
+  return dpp_add<0x140, 0xf>(v);
+}
+
+// Lane `src`'s value of v (src uniform across the wavefront).
+__device__ inline double readlane_f64(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes, in every lane: row sums, then the rows added by the
+// permlane16 / permlane32 swaps.  The whole wavefront must be active.
+__device__ inline double wave_sum(double v) { return xor32_sum(xor16_sum(row16_sum(v))); }
+
+template <int CTRL>
+__device__ inline double dpp_max(double v) {
+  return fmax(v, dpp_t<CTRL, 0xf>(v));
+}
+
+// Maximum over the 64 lanes, in every lane (the whole wavefront active).
+__device__ inline double wave_max(double v) {
+  v = dpp_max<0xb1>(v);
+  v = dpp_max<0x4e>(v);
+  v = dpp_max<0x141>(v);
+  v = dpp_max<0x140>(v);
+  const auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  v = fmax(__hiloint2double(h[0], l[0]), __hiloint2double(h[1], l[1]));
+  const auto h2 = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l2 = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  return fmax(__hiloint2double(h2[0], l2[0]), __hiloint2double(h2[1], l2[1]));
+}
+
+// Sum (max) of the per-wavefront values part[0..n), n <= 16, in every lane of
+// the calling (whole) wavefront: lanes 0..15 load one value each.
+__device__ inline double combine_sum(const double* part, int n) {
+  const int lane = threadIdx.x & 63;
+  return readlane_f64(row16_sum(lane < n ? part[lane] : 0.0), 0);
+}
+__device__ inline double combine_max(const double* part, int n) {
+  const int lane = threadIdx.x & 63;
+  double v = lane < n ? part[lane] : -1.0 / 0.0;
+  v = dpp_max<0xb1>(v);
+  v = dpp_max<0x4e>(v);
+  v = dpp_max<0x141>(v);
+  return readlane_f64(dpp_max<0x140>(v), 0);
+}
+
 __device__ inline double block_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  v = wave_sum(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  double s = 0.0;
-  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+  const double s = combine_sum(red, (int)(blockDim.x >> 6));
   __syncthreads();
   return s;
 }
@@ -127,10 +245,6 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 // 1 / sqrt(x) for x >= 1 in FP64: the hardware estimate refined by two
 // Newton steps y <- y (3 - x y^2) / 2 (each doubles the correct bits).
@@ -302,7 +416,8 @@ __device__ inline void v_item_store(double* V, const VItem& it) {
 // Pair indices come from the round-robin formula.  Every thread issues its
 // loads before its stores (A blocks and V items are disjoint), so the
 // arithmetic and the result are those of the two-phase form below.
-__device__ inline int jacobi_pipelined(double* A, double* V, int N, int ld, double* rc, double* rs, double* red) {
+__device__ __forceinline__ int jacobi_pipelined(double* A, double* V, int N, int ld, double* rc, double* rs, double* red,
+                                       double stop) {
   const int m = N + (N & 1), half = m / 2;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int k = tid >> 5, l = tid & 31;
@@ -317,7 +432,7 @@ __device__ inline int jacobi_pipelined(double* A, double* V, int N, int ld, doub
     }
     off = block_sum(off, red);
     tot = block_sum(tot, red);
-    if (off <= 1e-18 * tot) break;
+    if (off <= stop * tot) break;
     if (tid < half) jacobi_pair_rotation(A, ld, N, m, 0, tid, rc, rs);
     __syncthreads();
     for (int r = 0; r < m - 1; ++r) {
@@ -366,10 +481,11 @@ __device__ inline int jacobi_pipelined(double* A, double* V, int N, int ld, doub
 // disjoint rotations per round; the two-sided update A <- J^T A J is one pass
 // over the 2x2 blocks (pair k rows, pair l columns): Y = R_k^T X R_l with
 // R = [[c, s], [-s, c]].  N <= 64 with 1024 threads takes jacobi_pipelined,
-// which needs rc and rs of 64 entries.  Returns the number of sweeps.
-__device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* rp, int* rq, double* rc, double* rs,
-                                      double* red) {
-  if (N <= 64 && blockDim.x == 1024) return jacobi_pipelined(A, V, N, ld, rc, rs, red);
+// which needs rc and rs of 64 entries.  Sweeps until the off-diagonal share of
+// |A|^2 is at most `stop`; returns the number of sweeps.
+__device__ __forceinline__ int parallel_jacobi(double* A, double* V, int N, int ld, int* rp, int* rq, double* rc, double* rs,
+                                      double* red, double stop = 1e-18) {
+  if (N <= 64 && blockDim.x == 1024) return jacobi_pipelined(A, V, N, ld, rc, rs, red, stop);
   const int m = N + (N & 1);  // even number of players; index N is a dummy
   const int half = m / 2;
   int sweep = 0;
@@ -383,9 +499,9 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
     }
     off = block_sum(off, red);
     tot = block_sum(tot, red);
-    // relative off-diagonal norm 1e-9: eigenvectors to ~1e-9, so D to ~1e-9 and
-    // the energy (second order in D) to ~1e-18 of its scale
-    if (off <= 1e-18 * tot) break;
+    // stop = 1e-18, a relative off-diagonal norm of 1e-9: eigenvectors to ~1e-9,
+    // so D to ~1e-9 and the energy (second order in D) to ~1e-18 of its scale
+    if (off <= stop * tot) break;
     for (int r = 0; r < m - 1; ++r) {
       if ((int)threadIdx.x < half) {
         const int k = threadIdx.x;
@@ -465,7 +581,19 @@ __device__ inline int parallel_jacobi(double* A, double* V, int N, int ld, int* 
 struct ScfLds {
   double *A, *U, *B, *Bst, *Bm, *Bv, *cvec, *red, *rc, *rs, *eig;
   int *rp, *rq, *rank;
+  double* tri;  // scratch of tri_eigh (N <= 64 with 1024 threads), else null
+  double* M;    // fourth N x ld matrix (N <= 64 with 1024 threads), else null
 };
+
+// tri_eigh scratch: d, e, lo, hi, w, lam, dot (64 doubles each), kpart (16);
+// then ints: cnt[16][64], block lo / hi (64 each), fail flag
+constexpr int TRI_DOUBLES = 7 * 64 + 16;
+constexpr int TRI_INTS = 16 * 64 + 2 * 64 + 4;
+constexpr size_t TRI_BYTES = sizeof(double) * TRI_DOUBLES + sizeof(int) * TRI_INTS;
+// multisection: sections per round (one wavefront each) and rounds, (8 + 1)^17
+// > 2^53: the intervals shrink to the rounding of their end points
+constexpr int TRI_SECT = 8;
+constexpr int TRI_ROUNDS = 17;
 
 constexpr int SCF_RED = (SCF_THREADS / 64) * DIIS_N;
 
@@ -475,7 +603,8 @@ __host__ __device__ inline int scf_pairs(int N) { return N <= 64 ? 64 : N / 2 + 
 
 __host__ __device__ inline size_t scf_lds_bytes(int N) {
   const int h = scf_pairs(N), ld = lds_ld(N);
-  return sizeof(double) * (3 * (size_t)N * ld + 300 + 16 + SCF_RED + 2 * h + N) + sizeof(int) * (2 * h + N);
+  return sizeof(double) * (3 * (size_t)N * ld + 300 + 16 + SCF_RED + 2 * h + N) + sizeof(int) * (2 * h + N) +
+         (N <= 64 ? 8 + TRI_BYTES + sizeof(double) * (size_t)N * ld : 0);
 }
 
 __device__ inline ScfLds scf_lds_carve(unsigned char* smem, int N) {
@@ -495,7 +624,443 @@ __device__ inline ScfLds scf_lds_carve(unsigned char* smem, int N) {
   L.rp = reinterpret_cast<int*>(L.eig + N);
   L.rq = L.rp + h;
   L.rank = L.rq + h;
+  L.tri = nullptr;
+  L.M = nullptr;
+  if (N <= 64) {  // 8-byte aligned after the int tables
+    const size_t off = (reinterpret_cast<unsigned char*>(L.rank + N) - smem + 7) & ~(size_t)7;
+    L.tri = reinterpret_cast<double*>(smem + off);
+    L.M = reinterpret_cast<double*>(smem + off + TRI_BYTES);
+  }
   return L;
+}
+
+// ------------------------------------------------- tridiagonal eigensolver ---
+// Symmetric eigen-decomposition for N <= 64 by 1024 threads, in place of the
+// Jacobi sweeps: a fixed number of barrier phases instead of a number of
+// sweeps that depends on how far A is from diagonal.
+//   1. Householder tridiagonalisation T = Q^T A Q: step k reflects column k
+//      below the diagonal onto e_(k+1); its unit vector v (H = I - 2 v v^T) is
+//      kept in that column of A.  p = 2 A22 v by 16 lanes per row, K = v.p,
+//      w = p - K v, A22 -= v w^T + w v^T.
+//   2. Off-diagonals below 1e-12 (|d_i| + |d_i+1|) are set to zero: T falls
+//      into unreduced blocks, so eigenvalues that are degenerate up to
+//      rounding come from different blocks.
+//   3. Eigenvalue slot k belongs to the block holding position k (a block of
+//      n rows owns n slots): bisection on the block's Sturm count, each of the
+//      16 waves evaluating one of 16 interior points of every slot's interval
+//      per round (width / 17 per round, 14 rounds from the Gershgorin
+//      interval).  The count uses the division-free recurrence
+//      p_i = (d_i - x) p_(i-1) - e_(i-1)^2 p_(i-2) with power-of-two rescaling
+//      and counts sign changes (a zero takes the opposite sign of its
+//      predecessor).
+//   4. Eigenvector of slot k from the twisted factorisation of T - lambda I on
+//      its block (Dp top-down, Dm bottom-up, gamma_i = Dp_i + Dm_i - (d_i -
+//      lambda), r = argmin |gamma|, z_r = 1, z_i = -e_i / Dp_i z_(i+1) above,
+//      z_i = -e_(i-1) / Dm_i z_(i-1) below), one lane per slot, up to 32
+//      slots at a time (1 / Dp and 1 / Dm of a slot in B).
+//   5. Columns in ascending eigenvalue order; eigenvalues closer than 1e-6 of
+//      the scale are orthogonalised in order (modified Gram-Schmidt).
+//   6. U = Q Z by applying the reflectors in reverse order.
+// Returns false (the caller then runs Jacobi) if a Gram-Schmidt residual
+// falls below 1/2 or a result is not finite.  On success U holds the
+// eigenvectors as columns, eig the eigenvalues, both ascending.
+__device__ inline double recip_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = y * (2.0 - x * y);
+  return y * (2.0 - x * y);
+}
+
+// Number of eigenvalues below x of the tridiagonal block [lo, hi).
+// Sturm count of the block [b0, b1) of the tridiagonal matrix (lane i of the
+// wavefront holds d_i in dv and e_(i-1)^2 in e2v, zero where the matrix
+// splits): the number of its eigenvalues below x.  The product recurrence p_i
+// = (d_i - x) p_(i-1) - e_(i-1)^2 p_(i-2) runs over all N positions with a
+// wavefront-uniform index (d_i, e_(i-1)^2 by readlane, no LDS traffic),
+// counting sign changes inside [b0, b1) only: a
+// zero coupling at the block start makes sign(p_b0 / p_(b0-1)) = sign(d_b0 -
+// x), the block's own first term.  A zero p takes the sign opposite to its
+// predecessor.  Rescaled by a power of two every 8 positions.
+__device__ inline int sturm_count_all(double dv, double e2v, int N, int b0, int b1, double x) {
+  double p = 1.0, pm = 0.0;
+  bool neg_prev = false;
+  int c = 0;
+  for (int i0 = 0; i0 < N; i0 += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u;
+      if (i < N) {
+        const double di = readlane_f64(dv, i), e2 = readlane_f64(e2v, i);
+        const double pn = (di - x) * p - e2 * pm;
+        pm = p;
+        p = pn;
+        const bool neg = pn < 0.0 || (pn == 0.0 && !neg_prev);
+        c += (neg != neg_prev && i >= b0 && i < b1) ? 1 : 0;
+        neg_prev = neg;
+      }
+    }
+    const double m = fmax(fabs(p), fabs(pm));
+    if (m > 0x1p200 || (m < 0x1p-200 && m > 0.0)) {
+      const int ex = __builtin_amdgcn_frexp_exp(m);
+      p = ldexp(p, -ex);
+      pm = ldexp(pm, -ex);
+    }
+  }
+  return c;
+}
+
+// Buffers of one tri_eigh call: A (input, overwritten) and Z (eigenvectors)
+// with leading dimension ld, W scratch of wcap doubles (at least N ld), eig /
+// rank (N), the solver's own scratch region `tri` (TRI_BYTES), optionally W2,
+// a second N x ld scratch matrix (with it the eigenvectors of all slots are
+// formed in one pass, from register-held d and e), and optionally Q0, an
+// orthogonal N x N matrix in global memory (leading dimension N): Z then
+// receives Q0 times the eigenvectors of A (those of Q0 A Q0^T).
+struct TriBufs {
+  double *A, *Z, *W, *eig;
+  int* rank;
+  double* tri;
+  int ld, wcap;
+  double* W2;
+  const double* Q0;
+};
+
+__device__ inline bool tri_eigh(const TriBufs& T, int N) {
+  const int ld = T.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double *A = T.A, *Z = T.Z, *W = T.W;
+  double* d = T.tri;
+  double* e = d + 64;
+  double* lo = e + 64;
+  double* hi = lo + 64;
+  double* wv = hi + 64;
+  double* lam = wv + 64;
+  double* dot = lam + 64;
+  double* kpart = dot + 64;
+  int* cnt = reinterpret_cast<int*>(kpart + 16);
+  int* blo = cnt + 16 * 64;
+  int* bhi = blo + 64;
+  int* fail = bhi + 64;
+  // Q accumulates Q0 H_1 H_2 ... (Q0 or the identity to start) in Z during
+  // the reduction, so that no back-transformation pass is needed
+  for (int x = tid; x < N * ld; x += blockDim.x) {
+    const int i = x / ld, j = x - i * ld;
+    Z[x] = j < N ? (T.Q0 ? T.Q0[i * N + j] : (i == j ? 1.0 : 0.0)) : 0.0;
+  }
+  // 1. tridiagonalisation: H_k = I - 2 v v^T (|v| = 1) zeroes column k below
+  // the subdiagonal; with p = 2 A v and w = p - (v^T p) v the two-sided update
+  // is A <- A - v w^T - w v^T, and Q <- Q - (2 Q v) v^T
+  for (int k = 0; k + 2 < N; ++k) {
+    const int m = N - k - 1;
+    if (wave == 0) {
+      const double x = lane < m ? A[(k + 1 + lane) * ld + k] : 0.0;
+      const double ss = wave_sum(x * x);
+      const double x0 = __shfl(x, 0, 64);
+      double alpha = 0.0, v = 0.0;
+      if (ss > 0.0) {
+        const double nx = sqrt(ss);
+        alpha = x0 >= 0.0 ? -nx : nx;
+        const double v0 = x0 - alpha;
+        v = (lane == 0 ? v0 : x) / sqrt(ss - x0 * x0 + v0 * v0);
+      }
+      if (lane < m) A[(k + 1 + lane) * ld + k] = v;
+      if (lane == 0) {
+        e[k] = alpha;
+        d[k] = A[k * ld + k];
+        *fail = 0;
+      }
+    }
+    __syncthreads();
+    {
+      const int i = tid >> 4, g = tid & 15;  // row i, 16 lanes per row
+      double s = 0.0, y = 0.0;
+      if (i < m)
+        for (int j = g; j < m; j += 16) s += A[(k + 1 + i) * ld + k + 1 + j] * A[(k + 1 + j) * ld + k];
+      if (i < N)
+        for (int j = g; j < m; j += 16) y += Z[i * ld + k + 1 + j] * A[(k + 1 + j) * ld + k];
+      s = row16_sum(s);
+      y = row16_sum(y);
+      const double p = 2.0 * s;
+      const double vi = i < m ? A[(k + 1 + i) * ld + k] : 0.0;
+      const double kp = wave_sum((g == 0 && i < m) ? vi * p : 0.0);
+      if (lane == 0) kpart[wave] = kp;
+      if (g == 0 && i < N) dot[i] = 2.0 * y;
+      __syncthreads();
+      double K = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) K += kpart[q];
+      if (g == 0 && i < m) wv[i] = p - K * vi;
+    }
+    __syncthreads();
+    for (int x = tid; x < m * m; x += blockDim.x) {
+      const int i = x / m, j = x - i * m;
+      A[(k + 1 + i) * ld + k + 1 + j] -= A[(k + 1 + i) * ld + k] * wv[j] + wv[i] * A[(k + 1 + j) * ld + k];
+    }
+    for (int x = tid; x < N * m; x += blockDim.x) {
+      const int i = x / m, j = x - i * m;
+      Z[i * ld + k + 1 + j] -= dot[i] * A[(k + 1 + j) * ld + k];
+    }
+    __syncthreads();
+  }
+  // 2. last entries, splits and the block of every position
+  if (wave == 0) {
+    if (lane == 0) {
+      if (N >= 2) {
+        d[N - 2] = A[(N - 2) * ld + N - 2];
+        e[N - 2] = A[(N - 1) * ld + N - 2];
+      }
+      d[N - 1] = A[(N - 1) * ld + N - 1];
+      *fail = 0;
+    }
+    wave_sync();
+    const bool sp = lane < N - 1 && fabs(e[lane]) <= 1e-12 * (fabs(d[lane]) + fabs(d[lane + 1]));
+    if (sp) e[lane] = 0.0;
+    const unsigned long long mask = __ballot(sp);  // bit i: split after position i
+    if (lane < N) {
+      const unsigned long long below = lane > 0 ? (mask & ((1ull << lane) - 1)) : 0ull;
+      blo[lane] = below ? 64 - __clzll(below) : 0;   // highest split below lane, + 1
+      const unsigned long long at = mask >> lane;
+      bhi[lane] = at ? lane + __ffsll(at) : N;       // lowest split at or above lane, + 1
+    }
+    wave_sync();
+    // Gershgorin interval of slot lane's block
+    if (lane < N) {
+      double a = 1e300, b = -1e300;
+      for (int i = blo[lane]; i < bhi[lane]; ++i) {
+        const double r = (i > blo[lane] ? fabs(e[i - 1]) : 0.0) + (i + 1 < bhi[lane] ? fabs(e[i]) : 0.0);
+        a = fmin(a, d[i] - r);
+        b = fmax(b, d[i] + r);
+      }
+      const double pad = 4.0 * 2.220446049250313e-16 * fmax(fabs(a), fabs(b)) + 1e-300;
+      lo[lane] = a - pad;
+      hi[lane] = b + pad;
+    } else {
+      lo[lane] = 0.0;
+      hi[lane] = 0.0;
+      blo[lane] = 0;
+      bhi[lane] = 0;
+    }
+    wv[lane] = (lane > 0 && lane < N) ? e[lane - 1] * e[lane - 1] : 0.0;  // e_(i-1)^2, 0 at splits
+  }
+  __syncthreads();
+  // the accumulated Q moves to A (its reflectors are no longer needed), Z is
+  // free for the eigenvectors of the tridiagonal matrix
+  for (int x = tid; x < N * ld; x += blockDim.x) A[x] = Z[x];
+  // 3. multisection: wave q < TRI_SECT evaluates section point q of every
+  // slot (lane), TRI_ROUNDS rounds narrow each interval (TRI_SECT + 1)-fold
+  {
+    const int b0 = blo[lane], b1 = bhi[lane];
+    const double dv = lane < N ? d[lane] : 0.0, e2v = wv[lane];
+    for (int round = 0; round < TRI_ROUNDS; ++round) {
+      if (wave < TRI_SECT) {
+        const double x = lo[lane] + (wave + 1) * ((hi[lane] - lo[lane]) * (1.0 / (TRI_SECT + 1)));
+        cnt[wave * 64 + lane] = sturm_count_all(dv, e2v, N, b0, b1, x);
+      }
+      __syncthreads();
+      if (wave == 0 && lane < N) {
+        const int j = lane - b0;
+        const double a = lo[lane], w = (hi[lane] - a) * (1.0 / (TRI_SECT + 1));
+        double na = a, nb = hi[lane];
+        for (int q = 0; q < TRI_SECT; ++q) {
+          const double x = a + (q + 1) * w;
+          if (cnt[q * 64 + lane] <= j) {
+            na = x;
+          } else {
+            nb = x;
+            break;
+          }
+        }
+        lo[lane] = na;
+        hi[lane] = nb;
+      }
+      __syncthreads();
+    }
+  }
+  // 4. ascending order of the slots, then the eigenvectors
+  if (wave == 0 && lane < N) {
+    const double lk = 0.5 * (lo[lane] + hi[lane]);
+    int r = 0;
+    for (int q = 0; q < N; ++q) {
+      const double lq = 0.5 * (lo[q] + hi[q]);
+      r += (lq < lk) || (lq == lk && q < lane);
+    }
+    T.rank[lane] = r;
+    lam[lane] = lk;
+  }
+  __syncthreads();
+  if (wave == 0 && lane < N) T.eig[T.rank[lane]] = lam[lane];
+  for (int x = tid; x < N * ld; x += blockDim.x) Z[x] = 0.0;
+  __syncthreads();
+  if (T.W2 != nullptr) {
+    // all slots in one pass (lane = slot): 1 / Dp in W, 1 / Dm in W2 at the
+    // odd stride ld, loops over all positions with a uniform index so that
+    // d_i and e_i come from registers by readlane
+    if (wave == 0) {
+      const bool act = lane < N;
+      const int a0 = act ? blo[lane] : 0, a1 = act ? bhi[lane] : 0;
+      const double lk = act ? lam[lane] : 0.0;
+      const double dv = act ? d[lane] : 0.0, ev = lane < N - 1 ? e[lane] : 0.0;
+      double* rdp = W + (act ? lane : 0) * ld;
+      double* rdm = T.W2 + (act ? lane : 0) * ld;
+      double sc = 0.0;
+      for (int i = 0; i < N; ++i) {
+        const double di = readlane_f64(dv, i), ei = readlane_f64(ev, i);
+        if (i >= a0 && i < a1) sc = fmax(sc, fabs(di) + (i < a1 - 1 ? fabs(ei) : 0.0));
+      }
+      const double tiny = 2.220446049250313e-16 * sc + 1e-300;
+      auto guard = [&](double v) { return fabs(v) >= tiny ? v : (v < 0.0 ? -tiny : tiny); };
+      double r_prev = 0.0;  // top-down pivots Dp
+      for (int i = 0; i < N; ++i) {
+        const double di = readlane_f64(dv, i), em = i > 0 ? readlane_f64(ev, i - 1) : 0.0;
+        if (i >= a0 && i < a1) {
+          r_prev = recip_nr(guard((di - lk) - (i > a0 ? em * em * r_prev : 0.0)));
+          rdp[i] = r_prev;
+        }
+      }
+      // bottom-up pivots Dm and gamma_i = Dm_i - e_(i-1)^2 / Dp_(i-1): the
+      // twist r is where |gamma| is smallest
+      double rnext = 0.0, gbest = 1.0 / 0.0;
+      int r = a1 - 1;
+      for (int i = N - 1; i >= 0; --i) {
+        const double di = readlane_f64(dv, i), ei = readlane_f64(ev, i);
+        const double em = i > 0 ? readlane_f64(ev, i - 1) : 0.0;
+        if (i >= a0 && i < a1) {
+          const double dm = guard((di - lk) - (i < a1 - 1 ? ei * ei * rnext : 0.0));
+          rnext = recip_nr(dm);
+          rdm[i] = rnext;
+          const double g = fabs(i > a0 ? dm - em * em * rdp[i - 1] : dm);
+          if (g < gbest) {
+            gbest = g;
+            r = i;
+          }
+        }
+      }
+      const int col = act ? T.rank[lane] : 0;
+      double z = 1.0, nrm = 1.0;
+      if (act) Z[r * ld + col] = 1.0;
+      for (int i = N - 2; i >= 0; --i) {
+        const double ei = readlane_f64(ev, i);
+        if (act && i >= a0 && i < r) {
+          z = -ei * rdp[i] * z;
+          Z[i * ld + col] = z;
+          nrm += z * z;
+        }
+      }
+      z = 1.0;
+      for (int i = 1; i < N; ++i) {
+        const double em = readlane_f64(ev, i - 1);
+        if (act && i > r && i < a1) {
+          z = -em * rdm[i] * z;
+          Z[i * ld + col] = z;
+          nrm += z * z;
+        }
+      }
+      const double inv = 1.0 / sqrt(nrm);
+      if (act)
+        for (int i = a0; i < a1; ++i) Z[i * ld + col] *= inv;
+    }
+    __syncthreads();
+  }
+  // slots per pass, each with N doubles of 1 / Dp and of 1 / Dm at an odd
+  // stride (distinct LDS banks across the lanes)
+  const int stride = 2 * N + 1;
+  const int bs = T.W2 != nullptr ? N : (T.wcap / stride < 32 ? T.wcap / stride : 32);
+  for (int b0 = T.W2 != nullptr ? N : 0; b0 < N; b0 += bs) {
+    const int k = b0 + lane;
+    if (wave == 0 && lane < bs && k < N) {
+      const int a0 = blo[k], a1 = bhi[k];
+      const double lk = lam[k];
+      double* rdp = W + lane * stride;  // 1 / Dp (rows a0..a1 at offsets 0..)
+      double* rdm = rdp + N;            // 1 / Dm
+      double sc = 0.0;
+      for (int i = a0; i < a1; ++i) sc = fmax(sc, fabs(d[i]) + (i < a1 - 1 ? fabs(e[i]) : 0.0));
+      const double tiny = 2.220446049250313e-16 * sc + 1e-300;
+      auto guard = [&](double v) { return fabs(v) >= tiny ? v : (v < 0.0 ? -tiny : tiny); };
+      double dp = guard(d[a0] - lk);
+      double r_prev = recip_nr(dp);
+      rdp[0] = r_prev;
+      for (int i = a0 + 1; i < a1; ++i) {
+        const double ei = e[i - 1];
+        dp = guard((d[i] - lk) - ei * ei * r_prev);
+        r_prev = recip_nr(dp);
+        rdp[i - a0] = r_prev;
+      }
+      // bottom-up, with gamma_i = Dm_i - e_(i-1)^2 / Dp_(i-1)
+      double dm = guard(d[a1 - 1] - lk);
+      double rnext = recip_nr(dm);
+      rdm[a1 - 1 - a0] = rnext;
+      int r = a1 - 1;
+      double gbest = fabs(a1 - 1 > a0 ? dm - e[a1 - 2] * e[a1 - 2] * rdp[a1 - 2 - a0] : dm);
+      for (int i = a1 - 2; i >= a0; --i) {
+        const double ei = e[i];
+        dm = guard((d[i] - lk) - ei * ei * rnext);
+        rnext = recip_nr(dm);
+        rdm[i - a0] = rnext;
+        const double g = fabs(i > a0 ? dm - e[i - 1] * e[i - 1] * rdp[i - 1 - a0] : dm);
+        if (g < gbest) {
+          gbest = g;
+          r = i;
+        }
+      }
+      const int col = T.rank[k];
+      double z = 1.0, nrm = 1.0;
+      Z[r * ld + col] = 1.0;
+      for (int i = r - 1; i >= a0; --i) {
+        z = -e[i] * rdp[i - a0] * z;
+        Z[i * ld + col] = z;
+        nrm += z * z;
+      }
+      z = 1.0;
+      for (int i = r + 1; i < a1; ++i) {
+        z = -e[i - 1] * rdm[i - a0] * z;
+        Z[i * ld + col] = z;
+        nrm += z * z;
+      }
+      const double inv = 1.0 / sqrt(nrm);
+      for (int i = a0; i < a1; ++i) Z[i * ld + col] *= inv;
+    }
+    __syncthreads();
+  }
+  // 5. clusters: modified Gram-Schmidt in ascending order (one wave, lane = row)
+  if (wave == 0) {
+    double sc = 0.0;
+    for (int i = 0; i < N; ++i) sc = fmax(sc, fabs(d[i]) + (i < N - 1 ? fabs(e[i]) : 0.0));
+    const double tau = 1e-6 * sc;
+    int c0 = 0;
+    for (int c = 1; c < N; ++c) {
+      if (T.eig[c] - T.eig[c - 1] >= tau) {
+        c0 = c;
+        continue;
+      }
+      double zc = lane < N ? Z[lane * ld + c] : 0.0;
+      for (int q = c0; q < c; ++q) {
+        const double zq = lane < N ? Z[lane * ld + q] : 0.0;
+        zc -= wave_sum(zq * zc) * zq;
+      }
+      const double nrm = sqrt(wave_sum(zc * zc));
+      if (!(nrm >= 0.5)) {
+        if (lane == 0) *fail = 1;
+        break;
+      }
+      if (lane < N) Z[lane * ld + c] = zc / nrm;
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  if (*fail) return false;
+  // 6. eigenvectors Q Z (Q in A), by way of W
+  lds_matmul<false, false>(A, Z, W, N, ld, ld);
+  __syncthreads();
+  for (int x = tid; x < N * ld; x += blockDim.x) Z[x] = W[x];
+  __syncthreads();
+  // finite results only
+  bool bad = false;
+  for (int x = tid; x < N * N; x += blockDim.x) {
+    const int i = x / N;
+    bad |= !isfinite(Z[i * ld + x - i * N]);
+  }
+  if (tid < N) bad |= !isfinite(T.eig[tid]);
+  if (bad) *fail = 1;
+  __syncthreads();
+  return *fail == 0;
 }
 
 // From the symmetric matrix in the orthonormal basis (L.A) and the initial
@@ -503,17 +1068,18 @@ __device__ inline ScfLds scf_lds_carve(unsigned char* smem, int N) {
 // stored to Uout), ascending ranks of the eigenvalues, C = L^-T U (L^-1 in
 // global Linv) left in L.A with columns unsorted, and C_occ C_occ^T (= D / 2)
 // left in L.U (leading dimension ld).
-__device__ inline int eig_density(const ScfLds& L, int N, int nocc, const double* __restrict__ Linv,
-                                  double* __restrict__ Uout) {
+__device__ __forceinline__ int eig_density(const ScfLds& L, int N, int nocc, const double* __restrict__ Linv,
+                                  double* __restrict__ Uout, bool solved = false) {
   const int ld = lds_ld(N), tid = threadIdx.x;
   double *A = L.A, *U = L.U, *B = L.B;
-  const int sweeps = parallel_jacobi(A, U, N, ld, L.rp, L.rq, L.rc, L.rs, L.red);
+  const int sweeps = solved ? 0 : parallel_jacobi(A, U, N, ld, L.rp, L.rq, L.rc, L.rs, L.red);
   if (Uout)  // eigenvectors in the orthonormal basis (next step's warm start)
     for (int x = tid; x < N * N; x += blockDim.x) {
       const int i = x / N;
       Uout[x] = U[i * ld + x - i * N];
     }
-  for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * ld + j];
+  if (!solved)
+    for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * ld + j];
   lds_load(B, Linv, N, ld);
   __syncthreads();
   for (int j = tid; j < N; j += blockDim.x) {
@@ -538,11 +1104,227 @@ __device__ inline int eig_density(const ScfLds& L, int N, int nocc, const double
   return sweeps;
 }
 
+// First row of B^-1 for the leading nv x nv block (nv <= 10) of the row-major
+// 10 x 10 matrix Bm, by Gauss-Jordan elimination with partial pivoting in one
+// wavefront: lane c < 2 nv holds column c of [B | I] in registers, pivots and
+// multipliers travel by readlane.  Writes c[j] = (B^-1)[0, j] for j < nv and
+// 0 for nv <= j < 10, and returns true, when kappa = |B|_F |B^-1|_F is finite
+// and below 4.5e11: then every eigenvalue w of B has |w| >= 1 / |B^-1|_F >
+// 1e2 (100 eps |B|_F) >= 1e2 x the pseudo-inverse cutoff 100 eps max |w|, so
+// B^-1 is the pseudo-inverse.  Otherwise returns false with c untouched.
+// Called by a whole wavefront (its return value is wavefront-uniform).
+__device__ __forceinline__ bool diis_inverse_row(const double* __restrict__ Bm, int nv, double* __restrict__ c) {
+  const int lane = threadIdx.x & 63;
+  double a[10];
+  double nb = 0.0;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    double v = 0.0;
+    if (i < nv) {
+      if (lane < nv)
+        v = Bm[i * 10 + lane];
+      else if (lane < 2 * nv)
+        v = (lane - nv == i) ? 1.0 : 0.0;
+    }
+    a[i] = v;
+    if (lane < nv) nb += v * v;
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    if (k < nv) {  // wavefront-uniform
+      int p = k;
+      double best = -1.0;
+#pragma unroll
+      for (int i = k; i < 10; ++i) {
+        const double m = fabs(a[i]);
+        if (i < nv && m > best) {
+          best = m;
+          p = i;
+        }
+      }
+      p = __builtin_amdgcn_readlane(p, k);
+#pragma unroll
+      for (int i = k + 1; i < 10; ++i)
+        if (i == p) {
+          const double t = a[i];
+          a[i] = a[k];
+          a[k] = t;
+        }
+      a[k] *= 1.0 / readlane_f64(a[k], k);
+#pragma unroll
+      for (int i = 0; i < 10; ++i)
+        if (i != k && i < nv) a[i] = fma(-readlane_f64(a[i], k), a[k], a[i]);
+    }
+  }
+  double ni = 0.0;
+  if (lane >= nv && lane < 2 * nv)
+#pragma unroll
+    for (int i = 0; i < 10; ++i) ni += a[i] * a[i];
+  const double k2 = wave_sum(nb) * wave_sum(ni);
+  if (!(k2 < 4.5e11 * 4.5e11)) return false;
+  if (lane >= nv && lane < 2 * nv) c[lane - nv] = a[0];
+  if (lane >= 20 && lane < 30 && lane - 20 >= nv) c[lane - 20] = 0.0;
+  return true;
+}
+
+// Sums of a, b and e and maxima of c and d over the workgroup (at most 16
+// wavefronts; red holds 80 doubles); every thread receives all five.
+__device__ inline void block_reduce5(double& a, double& b, double& c, double& d, double& e, double* red) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_max(c);
+  d = wave_max(d);
+  e = wave_sum(e);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[16 + w] = b;
+    red[32 + w] = c;
+    red[48 + w] = d;
+    red[64 + w] = e;
+  }
+  __syncthreads();
+  a = combine_sum(red, nw);
+  b = combine_sum(red + 16, nw);
+  c = combine_max(red + 32, nw);
+  d = combine_max(red + 48, nw);
+  e = combine_sum(red + 64, nw);
+  __syncthreads();
+}
+
+constexpr int OA_MAX = 10;           // refinement updates before giving up
+constexpr double OA_ENTRY = 3e-6;    // off-diagonal share of |A'|^2 up to which the refinement runs
+constexpr double OA_TAU = 0.2;       // |first-order correction| above which a pair is near-degenerate
+
+// Refinement of an approximate eigenbasis X (L.U, orthonormal to rounding) of
+// the symmetric A (L.M), after Ogita and Aishima, "Iterative refinement for
+// symmetric eigenvalue decomposition" (Japan J. Indust. Appl. Math. 35, 2018).
+// Each iteration, on FP64 MFMA products:
+//   G = X^T X (R = I - G),  S = X^T A X,  lam_i = S_ii / G_ii,
+//   E_ij = (S_ij + lam_j R_ij) / (lam_j - lam_i),   E_ii = R_ii / 2,
+//   X <- X + X E.
+// Convergence is quadratic while every |E_ij| is small.  A pair whose
+// first-order correction exceeds OA_TAU (or whose eigenvalues agree to 1e-13)
+// is near-degenerate: when it is the only such pair of both its rows, it
+// takes the exact 2 x 2 Jacobi rotation instead, E_ij = -E_ji = tan(theta)
+// with tan(2 theta) = 2 S_ij / (lam_j - lam_i) (bounded by 1); inside a larger
+// group it is only re-orthogonalised, E_ij = R_ij / 2 (its invariant subspace
+// still converges, and for an exactly degenerate group any basis of it is an
+// eigenbasis).  Converged when the off-diagonal share of |S|^2 is at most
+// 1e-18 (the Jacobi criterion) and |R| <= 1e-13: returns the number of
+// updates, with X in L.U and the eigenvalues in L.eig.  Returns -1 (X then
+// undefined) after OA_MAX updates or on a non-finite update.  L.A, L.B and
+// L.rp are scratch.  N <= 64, 1024 threads.
+__device__ __forceinline__ int oa_refine(const ScfLds& L, int N) {
+  const int ld = lds_ld(N), tid = threadIdx.x;
+  double *M = L.M, *X = L.U, *S = L.A, *G = L.B;
+  int* nbig = L.rp;  // near-degenerate pairs per row
+  for (int k = 0;; ++k) {
+    lds_matmul<false, false>(M, X, G, N, ld, ld);  // G = A X
+    __syncthreads();
+    lds_matmul<true, false>(X, G, S, N, ld, ld);  // S = X^T A X
+    __syncthreads();
+    lds_matmul<true, false>(X, X, G, N, ld, ld);  // G = X^T X
+    __syncthreads();
+    double off = 0.0, tot = 0.0, rmax = 0.0, lmax = 0.0, z = 0.0;
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      const double sv = S[i * ld + j], v = sv * sv;
+      tot += v;
+      rmax = fmax(rmax, fabs((i == j ? 1.0 : 0.0) - G[i * ld + j]));
+      if (i != j) {
+        off += v;
+      } else {
+        const double lam = sv / G[i * ld + i];
+        L.eig[i] = lam;
+        lmax = fmax(lmax, fabs(lam));
+      }
+    }
+    if (tid < N) nbig[tid] = 0;
+    block_reduce5(off, tot, rmax, lmax, z, L.red);  // also publishes L.eig, nbig
+    if (off <= 1e-18 * tot && rmax <= 1e-13) return k;
+    if (k == OA_MAX || !(off == off)) return -1;
+    const double degen = 1e-13 * lmax;
+    // near-degenerate pairs per row (integer counts: order-independent)
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      if (i >= j) continue;
+      const double sym = 0.5 * (S[i * ld + j] + S[j * ld + i]);
+      const double dl = L.eig[j] - L.eig[i];
+      if (fabs(sym - L.eig[j] * G[i * ld + j]) > OA_TAU * fabs(dl) || fabs(dl) <= degen) {
+        atomicAdd(&nbig[i], 1);
+        atomicAdd(&nbig[j], 1);
+      }
+    }
+    __syncthreads();
+    // E in place of S, the pair (i, j), (j, i) by one thread
+    double bad = 0.0;
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      if (i > j) continue;
+      if (i == j) {
+        S[i * ld + i] = 0.5 * (1.0 - G[i * ld + i]);
+        continue;
+      }
+      const double rij = -G[i * ld + j], rji = -G[j * ld + i];
+      const double sym = 0.5 * (S[i * ld + j] + S[j * ld + i]);
+      const double li = L.eig[i], lj = L.eig[j], dl = lj - li;
+      double eij = (sym + lj * rij) / dl, eji = -(sym + li * rji) / dl;  // |.| <= OA_TAU unless replaced
+      if (fabs(sym + lj * rij) > OA_TAU * fabs(dl) || fabs(dl) <= degen) {
+        if (nbig[i] == 1 && nbig[j] == 1) {  // isolated pair: exact rotation
+          const double h = 0.5 * dl;
+          const double t = h != 0.0 ? copysign(1.0, h) * sym / (fabs(h) + sqrt(h * h + sym * sym))
+                                    : (sym >= 0.0 ? 1.0 : -1.0);
+          eij = t;
+          eji = -t;
+        } else {
+          eij = 0.5 * rij;
+          eji = 0.5 * rji;
+        }
+      }
+      S[i * ld + j] = eij;
+      S[j * ld + i] = eji;
+      bad += (fabs(eij) <= 1.0 + 1e-12 && fabs(eji) <= 1.0 + 1e-12) ? 0.0 : 1.0;  // only non-finite entries
+    }
+    bad = block_sum(bad, L.red);
+    if (bad != 0.0) return -1;
+    lds_matmul<false, false>(X, S, G, N, ld, ld);  // G = X E
+    __syncthreads();
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      X[i * ld + j] += G[i * ld + j];
+    }
+    __syncthreads();
+  }
+}
+
+// A = L^-1 H L^-T, exactly symmetric, from H in A and L^-1 in B (LDS); U is
+// scratch.  (Rounding of the FP32 Fock terms and of the transform leaves
+// O(eps) asymmetry that Jacobi rotations cannot remove.)
+__device__ inline void orthonormal_basis(const ScfLds& L, int N) {
+  const int ld = lds_ld(N), tid = threadIdx.x;
+  double *A = L.A, *U = L.U, *B = L.B;
+  lds_matmul<false, true>(A, B, U, N, ld, ld);  // U = H L^T
+  __syncthreads();
+  lds_matmul<false, false>(B, U, A, N, ld, ld);  // A = L U
+  __syncthreads();
+  for (int x = tid; x < N * N; x += blockDim.x) {
+    const int i = x / N, j = x - i * N;
+    if (i < j) {
+      const double v = 0.5 * (A[i * ld + j] + A[j * ld + i]);
+      A[i * ld + j] = v;
+      A[j * ld + i] = v;
+    }
+  }
+  __syncthreads();
+}
+
 // One SCF step for one molecule by the whole workgroup (see pscf_scf_step).
 // Optional Dt receives D converted to T (the precision of the next J/K / XC),
 // optional dDt the change of D in T (for an incremental J/K build).
 template <typename T, typename TJ>
-__device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, const double* __restrict__ Hcore,
+__device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, int nocc, const double* __restrict__ Hcore,
                                      const double* __restrict__ S, const double* __restrict__ Linv,
                                      const TJ* __restrict__ dJK, const T* __restrict__ Vxc, double* __restrict__ D,
                                      T* __restrict__ Dt, T* __restrict__ dDt, double* __restrict__ Hout,
@@ -628,22 +1410,42 @@ __device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, c
       L.Bst[(tid + 1) * 10 + (head + 1)] = dot;
     }
     __syncthreads();
-    if (wave == 0) {
+    auto masked_B = [&]() {
       for (int x = lane; x < 100; x += 64) {
         const int i = x / 10, j = x % 10;
         const bool mi = i == 0 || i - 1 < nvalid, mj = j == 0 || j - 1 < nvalid;
         L.Bm[x] = (mi && mj) ? L.Bst[x] : 0.0;
       }
-      wave_sync();
-      wave_jacobi(L.Bm, L.Bv, 10, L.rp, L.rq, L.rc, L.rs);
+    };
+    if (wave == 0) masked_B();
+    __syncthreads();
+    // c = pinv(B)[0]: the inverse of the valid block while B is far from
+    // singular (every step of a converging SCF run), else from the
+    // eigen-decomposition of the 10 x 10 system
+    if (wave == 0) {
+      const bool inv = diis_inverse_row(L.Bm, nvalid + 1, L.cvec);
+      if (lane == 0) L.cvec[15] = inv ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const bool inv_ok = L.cvec[15] != 0.0;
+    // the tridiagonal solver (all threads; eigenvalues into red, ranks into
+    // rp, U as its scratch), else one-wavefront Jacobi
+    const bool tri_ok = !inv_ok && L.tri != nullptr && blockDim.x == 1024 && N * ld >= 20 &&
+                        tri_eigh(TriBufs{L.Bm, L.Bv, L.U, L.red, L.rp, L.tri, 10, N * ld, nullptr, nullptr}, 10);
+    if (wave == 0 && !inv_ok) {
+      if (!tri_ok) {
+        masked_B();
+        wave_sync();
+        wave_jacobi(L.Bm, L.Bv, 10, L.rp, L.rq, L.rc, L.rs);
+      }
       double wmax = 0.0;
-      for (int i = 0; i < 10; ++i) wmax = fmax(wmax, fabs(L.Bm[i * 10 + i]));
+      for (int i = 0; i < 10; ++i) wmax = fmax(wmax, fabs(tri_ok ? L.red[i] : L.Bm[i * 10 + i]));
       const double cut = 10.0 * 10.0 * 2.220446049250313e-16 * wmax;
       // pinv(B)[0, j] = sum_k V[0,k] V[j,k] / w_k over |w_k| > cut
       if (lane < 10) {
         double acc = 0.0;
         for (int k = 0; k < 10; ++k) {
-          const double wk = L.Bm[k * 10 + k];
+          const double wk = tri_ok ? L.red[k] : L.Bm[k * 10 + k];
           if (fabs(wk) > cut) acc += L.Bv[k] * L.Bv[lane * 10 + k] / wk;
         }
         L.cvec[lane] = acc;
@@ -664,23 +1466,18 @@ __device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, c
     const int i = x / N;
     Hout[x] = A[i * ld + x - i * N];
   }
-  lds_matmul<false, true>(A, B, U, N, ld, ld);  // U = H L^T
-  __syncthreads();
-  lds_matmul<false, false>(B, U, A, N, ld, ld);  // A = L U
-  __syncthreads();
-  // exact symmetry (rounding of the FP32 Fock terms and of the transform
-  // leaves O(eps) asymmetry that Jacobi rotations cannot remove)
-  for (int x = tid; x < NN; x += blockDim.x) {
-    const int i = x / N, j = x - i * N;
-    if (i < j) {
-      const double v = 0.5 * (A[i * ld + j] + A[j * ld + i]);
-      A[i * ld + j] = v;
-      A[j * ld + i] = v;
-    }
-  }
-  __syncthreads();
+  orthonormal_basis(L, N);
+  // Eigen-decomposition of A = L^-1 H L^-T (N <= 64 with 1024 threads):
+  //  * first step (no previous eigenvectors): the tridiagonal solver;
+  //  * later steps: A' = Uprev^T A Uprev with the previous eigenvectors.  Where
+  //    its off-diagonal share of |A'|^2 is at most OA_ENTRY (the SCF's later
+  //    steps), the refinement from X = Uprev (a few MFMA products per
+  //    iteration); otherwise the tridiagonal solver on A' and U = Uprev Z.
+  // Cyclic Jacobi (from the identity or from Uprev) takes over when either
+  // fails, and serves N > 64.
   const bool warm = flags[0] != 0.0;
-  if (warm) {
+  const bool small = L.M != nullptr && N <= 64 && blockDim.x == 1024;
+  auto start = [&]() {  // A = Uprev^T A Uprev, U = Uprev
     lds_load(B, Uprev, N, ld);
     __syncthreads();
     lds_matmul<false, false>(A, B, U, N, ld, ld);  // U = A Uprev
@@ -688,14 +1485,69 @@ __device__ inline void scf_step_body(const ScfLds& L, int it, int N, int nocc, c
     lds_matmul<true, false>(B, U, A, N, ld, ld);  // A = Uprev^T U
     __syncthreads();
     for (int x = tid; x < N * ld; x += blockDim.x) U[x] = B[x];
+    __syncthreads();
+  };
+  auto rebuild = [&]() {  // A from H after the tridiagonal solver overwrote it
+    lds_load(A, Hout, N, ld);
+    lds_load(B, Linv, N, ld);
+    __syncthreads();
+    orthonormal_basis(L, N);
+  };
+  int path = 0;  // 1 tridiagonal, 2 refinement
+  int refined = -1;
+  if (!warm) {
+    if (small) {
+      if (tri_eigh(TriBufs{L.A, L.U, L.B, L.eig, L.rank, L.tri, ld, N * ld, L.M, nullptr}, N))
+        path = 1;
+      else
+        rebuild();
+    }
+    if (path == 0) {
+      for (int x = tid; x < N * ld; x += blockDim.x) U[x] = (x / ld == x % ld) ? 1.0 : 0.0;
+      __syncthreads();
+    }
+  } else if (small) {
+    for (int x = tid; x < N * ld; x += blockDim.x) L.M[x] = A[x];  // A in the orthonormal basis, kept
+    __syncthreads();
+    start();
+    double off = 0.0, tot = 0.0, z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    for (int x = tid; x < NN; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      const double v = A[i * ld + j] * A[i * ld + j];
+      tot += v;
+      if (i != j) off += v;
+    }
+    block_reduce5(off, tot, z0, z1, z2, L.red);
+    if (off <= OA_ENTRY * tot) {
+      refined = oa_refine(L, N);
+      if (refined >= 0) {
+        path = 2;
+      } else {
+        for (int x = tid; x < N * ld; x += blockDim.x) A[x] = L.M[x];
+        __syncthreads();
+        start();
+      }
+    } else if (tri_eigh(TriBufs{L.A, L.U, L.B, L.eig, L.rank, L.tri, ld, N * ld, L.M, Uprev}, N)) {
+      path = 1;  // U = Uprev (eigenvectors of A')
+    } else {
+      rebuild();
+      start();
+    }
   } else {
-    for (int x = tid; x < N * ld; x += blockDim.x) U[x] = (x / ld == x % ld) ? 1.0 : 0.0;
+    start();
+  }
+  int sweeps = 0;
+  if (path == 0) {  // one call site: the sweep loop is compiled inline once
+    sweeps = parallel_jacobi(A, U, N, ld, L.rp, L.rq, L.rc, L.rs, L.red);
+    for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * ld + j];
   }
   __syncthreads();
-  const int sweeps = eig_density(L, N, nocc, Linv, Uprev);
+  eig_density(L, N, nocc, Linv, Uprev, true);
   if (tid == 0) {
     flags[0] = 1.0;
-    flags[1] = (double)sweeps;  // diagnostic: Jacobi sweeps of this step
+    // diagnostic: -1 tridiagonal solver, 100 (k + 1) refinement of k updates,
+    // s >= 0 Jacobi sweeps
+    flags[1] = path == 1 ? -1.0 : (path == 2 ? 100.0 * (refined + 1) : (double)sweeps);
   }
   for (int j = tid; j < N; j += blockDim.x) mo_e[L.rank[j]] = L.eig[j];
   for (int x = tid; x < NN; x += blockDim.x) {
@@ -757,6 +1609,8 @@ __global__ void __launch_bounds__(SCF_THREADS) scf_step_ws_kernel(int it, int N,
   L.rp = sm_i;
   L.rq = L.rp + H;
   L.rank = L.rq + H;
+  L.tri = nullptr;
+  L.M = nullptr;
   scf_step_body<T, T>(L, it, N, nocc, Hcore, S, Linv, dJK, Vxc, D, (T*)nullptr, (T*)nullptr, Hout, mo_e, Cout, state,
                       use_diis);
 }

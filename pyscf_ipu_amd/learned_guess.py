@@ -241,16 +241,28 @@ def unflatten(flat: np.ndarray):
 
 
 # ------------------------------------------------------------------ guess ----
-def learned_guess(model: PairMLP):
-    """A guess function for BatchRKS.setup: predict, assemble, purify."""
+def learned_guess(model: PairMLP, onsite: str = "network"):
+    """A guess function for BatchRKS.setup: predict, assemble, purify.
+    onsite="atomic" keeps the atomic-density guess (guess.minao's rule,
+    pscf_batch_guess) on the atom-diagonal blocks and takes only the blocks of
+    pairs of different atoms from the network."""
+    if onsite not in ("network", "atomic"):
+        raise ValueError(f"onsite must be 'network' or 'atomic', not {onsite!r}")
 
     def fill(b: BatchRKS):
         pairs, info = pair_lists(b)
+        st = device.stream()
+        if onsite == "atomic":
+            keep = pairs[:, 0] != pairs[:, 1]
+            pairs, info = np.ascontiguousarray(pairs[keep]), np.ascontiguousarray(info[keep])
+            P_ = device.ptr
+            _lib.call("pscf_batch_guess", b.code, P_(b.d_desc), b.M, P_(b.d_atm), P_(b.d_bas), P_(b.d_shell_ao),
+                      P_(b.S), P_(b.D), P_(b.Dt), st)  # block-diagonal by atom
+        else:
+            b.D.zero_()
         P = len(pairs)
         X16, _, valid = features(b, pairs)
         d_info = device.to_dev(info)
-        b.D.zero_()
-        st = device.stream()
         chunk = 1 << 18
         for r0 in range(0, P, chunk):
             n = min(chunk, P - r0)
