@@ -64,63 +64,60 @@ __device__ inline void lds_matmul_mfma(const double* __restrict__ X, const doubl
   }
 }
 
-// The same product for 32 < N <= 64 by 4 waves (one per SIMD): wave w
-// computes the 32 x 32 output block (w / 2, w % 2) as 2 x 2 tiles of 16, four
-// independent MFMA chains sharing each operand fragment, so that the SIMD's
-// matrix unit (64 cycles per FP64 16x16x4 operation) bounds the time rather
-// than LDS traffic (half that of one tile per wave).  Tiles wholly outside N
-// x N are skipped.
-template <bool TX, bool TY>
-__device__ inline void lds_matmul_mfma4(const double* __restrict__ X, const double* __restrict__ Y,
-                                        double* __restrict__ C, int N, int ld, int ldc) {
+// The same product with every operand fragment of a tile loaded before its
+// MFMA chain (eight k-steps at a time), so that LDS latency overlaps the
+// matrix unit (64 cycles per FP64 16x16x4 operation per SIMD): about 6k
+// cycles for N = 64 against 8.5k loading per step.  kSym: C is symmetric,
+// only tiles I <= J are computed and each is also stored transposed.
+template <bool TX, bool TY, bool kSym>
+__device__ inline void lds_matmul_mfma_pre(const double* __restrict__ X, const double* __restrict__ Y,
+                                           double* __restrict__ C, int N, int ld, int ldc) {
   typedef double v4d __attribute__((ext_vector_type(4)));
+  const int nt = (N + 15) >> 4;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (w >= 4) return;  // wave-uniform
-  const int li = lane & 15, lg = lane >> 4;
-  const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-  const bool t1r = r0 + 16 < N, t1c = c0 + 16 < N;  // second row / column tile inside N
-  int rowc[2], colc[2];
-  bool rv[2], cv[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int row = r0 + 16 * t + li, col = c0 + 16 * t + li;
-    rv[t] = row < N;
-    cv[t] = col < N;
-    rowc[t] = rv[t] ? row : N - 1;
-    colc[t] = cv[t] ? col : N - 1;
-  }
-  v4d acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-  for (int k0 = 0; k0 < N; k0 += 4) {
-    const int k = k0 + lg;
-    const bool kv = k < N;
-    const int kc = kv ? k : N - 1;
-    double a[2], b[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const double av = TX ? X[kc * ld + rowc[t]] : X[rowc[t] * ld + kc];
-      const double bv = TY ? Y[colc[t] * ld + kc] : Y[kc * ld + colc[t]];
-      a[t] = (rv[t] && kv) ? av : 0.0;
-      b[t] = (cv[t] && kv) ? bv : 0.0;
+  int I, J;
+  if (kSym) {  // wave w -> the w-th upper tile (I <= J), row by row
+    int t = w, r = 0;
+    while (r < nt && t >= nt - r) {
+      t -= nt - r;
+      ++r;
     }
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], acc[0][0], 0, 0, 0);
-    if (t1c) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[1], acc[0][1], 0, 0, 0);
-    if (t1r) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[0], acc[1][0], 0, 0, 0);
-    if (t1r && t1c) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], acc[1][1], 0, 0, 0);
+    if (r >= nt) return;  // wave-uniform
+    I = r;
+    J = r + t;
+  } else {
+    if (w >= nt * nt) return;  // wave-uniform
+    I = w / nt;
+    J = w - I * nt;
+  }
+  const int li = lane & 15, lg = lane >> 4;
+  const int row = I * 16 + li, col = J * 16 + li;
+  const bool rv = row < N, cv = col < N;
+  const int rowc = rv ? row : N - 1, colc = cv ? col : N - 1;
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < N; k0 += 32) {
+    double a[8], b[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 + 4 * t + lg;
+      const bool kv = k < N;
+      const int kc = kv ? k : N - 1;
+      const double av = TX ? X[kc * ld + rowc] : X[rowc * ld + kc];
+      const double bv = TY ? Y[colc * ld + kc] : Y[kc * ld + colc];
+      a[t] = (rv && kv) ? av : 0.0;
+      b[t] = (cv && kv) ? bv : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], acc, 0, 0, 0);
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = r0 + 16 * i + lg + 4 * q, c = c0 + 16 * j + li;
-        if (r < N && c < N) C[r * ldc + c] = acc[i][j][q];
-      }
+  for (int j = 0; j < 4; ++j) {
+    const int r = I * 16 + lg + 4 * j;
+    if (r < N && cv && (!kSym || I < J || r <= col)) {  // a diagonal tile stores its upper half
+      C[r * ldc + col] = acc[j];
+      if (kSym) C[col * ldc + r] = acc[j];
+    }
+  }
 }
 
 // C = op(X) op(Y), N x N, X and Y in LDS (leading dimension ld), C in LDS or
@@ -133,7 +130,7 @@ __device__ inline void lds_matmul(const double* __restrict__ X, const double* __
                                   int N, int ld, int ldc) {
   if (N <= 64 && blockDim.x == 1024) {
     if (N > 32)
-      lds_matmul_mfma4<TX, TY>(X, Y, C, N, ld, ldc);
+      lds_matmul_mfma_pre<TX, TY, false>(X, Y, C, N, ld, ldc);
     else
       lds_matmul_mfma<TX, TY>(X, Y, C, N, ld, ldc);
     return;
@@ -162,6 +159,18 @@ __device__ inline void lds_matmul(const double* __restrict__ X, const double* __
       if (vj) C[i1 * ldc + j1] = c11;
     }
   }
+}
+
+// lds_matmul for a product known to be symmetric (C = C^T up to rounding):
+// for 32 < N <= 64 only the tiles on and above the diagonal are computed and
+// mirrored, so C comes out exactly symmetric.
+template <bool TX, bool TY>
+__device__ inline void lds_matmul_sym(const double* __restrict__ X, const double* __restrict__ Y,
+                                      double* __restrict__ C, int N, int ld, int ldc) {
+  if (N > 32 && N <= 64 && blockDim.x == 1024)
+    lds_matmul_mfma_pre<TX, TY, true>(X, Y, C, N, ld, ldc);
+  else
+    lds_matmul<TX, TY>(X, Y, C, N, ld, ldc);
 }
 
 // N x N global (leading dimension N) <-> LDS (leading dimension ld)
@@ -590,10 +599,11 @@ struct ScfLds {
 constexpr int TRI_DOUBLES = 7 * 64 + 16;
 constexpr int TRI_INTS = 16 * 64 + 2 * 64 + 4;
 constexpr size_t TRI_BYTES = sizeof(double) * TRI_DOUBLES + sizeof(int) * TRI_INTS;
-// multisection: sections per round (one wavefront each) and rounds, (8 + 1)^17
-// > 2^53: the intervals shrink to the rounding of their end points
-constexpr int TRI_SECT = 8;
-constexpr int TRI_ROUNDS = 17;
+// multisection: sections per round (one wavefront each, one per SIMD: the
+// Sturm recurrence is bound by instruction issue) and rounds, 5^23 > 2^53: the
+// intervals shrink to the rounding of their end points
+constexpr int TRI_SECT = 4;
+constexpr int TRI_ROUNDS = 23;
 
 constexpr int SCF_RED = (SCF_THREADS / 64) * DIIS_N;
 
@@ -670,42 +680,98 @@ __device__ inline double recip_nr(double x) {
   return y * (2.0 - x * y);
 }
 
-// Number of eigenvalues below x of the tridiagonal block [lo, hi).
-// Sturm count of the block [b0, b1) of the tridiagonal matrix (lane i of the
-// wavefront holds d_i in dv and e_(i-1)^2 in e2v, zero where the matrix
-// splits): the number of its eigenvalues below x.  The product recurrence p_i
-// = (d_i - x) p_(i-1) - e_(i-1)^2 p_(i-2) runs over all N positions with a
-// wavefront-uniform index (d_i, e_(i-1)^2 by readlane, no LDS traffic),
-// counting sign changes inside [b0, b1) only: a
-// zero coupling at the block start makes sign(p_b0 / p_(b0-1)) = sign(d_b0 -
-// x), the block's own first term.  A zero p takes the sign opposite to its
-// predecessor.  Rescaled by a power of two every 8 positions.
-__device__ inline int sturm_count_all(double dv, double e2v, int N, int b0, int b1, double x) {
+// Sturm count of the block [b0, b1) of the tridiagonal matrix (diagonal d,
+// e2m1[i] = e_(i-1)^2, zero where the matrix splits): the number of its
+// eigenvalues below x.  The product recurrence p_i = (d_i - x) p_(i-1) -
+// e_(i-1)^2 p_(i-2) runs over all N positions with a wavefront-uniform index
+// (d_i, e2m1_i broadcast from LDS), counting sign changes inside [b0, b1)
+// only: a zero coupling at the block start makes sign(p_b0 / p_(b0-1)) =
+// sign(d_b0 - x), the block's own first term.  Branch-free (sign bits and
+// masks), rescaled by a power of two every 8 positions; a zero counts with the
+// sign of its sign bit.
+__device__ inline int sturm_count_all(const double* __restrict__ d, const double* __restrict__ e2m1, int N, int b0,
+                                      int b1, double x) {
   double p = 1.0, pm = 0.0;
-  bool neg_prev = false;
-  int c = 0;
+  int sp = 0, c = 0;  // sign of the previous term: 0 non-negative, -1 negative
+  const int hi = b1 - 1;
   for (int i0 = 0; i0 < N; i0 += 8) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u;
       if (i < N) {
-        const double di = readlane_f64(dv, i), e2 = readlane_f64(e2v, i);
-        const double pn = (di - x) * p - e2 * pm;
+        const double pn = (d[i] - x) * p - e2m1[i] * pm;
         pm = p;
         p = pn;
-        const bool neg = pn < 0.0 || (pn == 0.0 && !neg_prev);
-        c += (neg != neg_prev && i >= b0 && i < b1) ? 1 : 0;
-        neg_prev = neg;
+        const int sg = __double2hiint(pn) >> 31;
+        c -= (sg ^ sp) & (~((i - b0) | (hi - i)) >> 31);
+        sp = sg;
       }
     }
-    const double m = fmax(fabs(p), fabs(pm));
-    if (m > 0x1p200 || (m < 0x1p-200 && m > 0.0)) {
-      const int ex = __builtin_amdgcn_frexp_exp(m);
-      p = ldexp(p, -ex);
-      pm = ldexp(pm, -ex);
-    }
+    const double mx = fmax(fabs(p), fabs(pm));
+    const int ex = (mx > 0x1p200 || mx < 0x1p-200) ? __builtin_amdgcn_frexp_exp(mx) : 0;
+    p = ldexp(p, -ex);
+    pm = ldexp(pm, -ex);
   }
   return c;
+}
+
+// Eigenvector of one slot (lane) of the tridiagonal matrix (d, e in LDS) for
+// the eigenvalue lk of its block [a0, a1), by the twisted factorisation:
+// top-down pivots Dp, bottom-up pivots Dm, the twist r where |gamma_i| = |Dm_i
+// - e_(i-1)^2 / Dp_(i-1)| is smallest, then z_r = 1 and the recurrences up and
+// down from r, normalised, into column col of Z.  Pivots smaller than eps
+// |T| are replaced by +-eps |T|.  kPred: per-lane block bounds (the matrix
+// splits); otherwise every slot's block is [0, N) and the positions run with
+// one uniform index.  Inactive lanes (act false) store nothing.
+template <bool kPred>
+__device__ inline void twisted_slot(const double* __restrict__ d, const double* __restrict__ e, int N, int ld, int a0,
+                                    int a1, double lk, bool act, int col, double* __restrict__ rdp,
+                                    double* __restrict__ rdm, double* __restrict__ Z) {
+  auto in = [&](int i) { return !kPred || (i >= a0 && i < a1); };
+  double sc = 0.0;
+  for (int i = 0; i < N; ++i)
+    if (in(i)) sc = fmax(sc, fabs(d[i]) + (i < a1 - 1 ? fabs(e[i]) : 0.0));
+  const double tiny = 2.220446049250313e-16 * sc + 1e-300;
+  auto guard = [&](double v) { return fabs(v) >= tiny ? v : (v < 0.0 ? -tiny : tiny); };
+  double r_prev = 0.0;
+  for (int i = 0; i < N; ++i) {
+    if (in(i)) {
+      const double em = i > a0 ? e[i - 1] : 0.0;
+      r_prev = recip_nr(guard((d[i] - lk) - em * em * r_prev));
+      if (act) rdp[i] = r_prev;
+    }
+  }
+  double rnext = 0.0, gbest = 1.0 / 0.0;
+  int r = a1 - 1;
+  for (int i = N - 1; i >= 0; --i) {
+    if (in(i)) {
+      const double ei = i < a1 - 1 ? e[i] : 0.0, em = i > a0 ? e[i - 1] : 0.0;
+      const double dm = guard((d[i] - lk) - ei * ei * rnext);
+      rnext = recip_nr(dm);
+      if (act) rdm[i] = rnext;
+      const double g = fabs(i > a0 ? dm - em * em * rdp[i - 1] : dm);
+      if (g < gbest) {
+        gbest = g;
+        r = i;
+      }
+    }
+  }
+  if (!act) return;
+  double z = 1.0, nrm = 1.0;
+  Z[r * ld + col] = 1.0;
+  for (int i = r - 1; i >= a0; --i) {
+    z = -e[i] * rdp[i] * z;
+    Z[i * ld + col] = z;
+    nrm += z * z;
+  }
+  z = 1.0;
+  for (int i = r + 1; i < a1; ++i) {
+    z = -e[i - 1] * rdm[i] * z;
+    Z[i * ld + col] = z;
+    nrm += z * z;
+  }
+  const double inv = 1.0 / sqrt(nrm);
+  for (int i = a0; i < a1; ++i) Z[i * ld + col] *= inv;
 }
 
 // Buffers of one tri_eigh call: A (input, overwritten) and Z (eigenvectors)
@@ -747,28 +813,34 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   }
   // 1. tridiagonalisation: H_k = I - 2 v v^T (|v| = 1) zeroes column k below
   // the subdiagonal; with p = 2 A v and w = p - (v^T p) v the two-sided update
-  // is A <- A - v w^T - w v^T, and Q <- Q - (2 Q v) v^T
+  // is A <- A - v w^T - w v^T, and Q <- Q - (2 Q v) v^T.  Two barriers a step:
+  // the products p, Q v; then the updates, during which wave 0 updates the
+  // next column itself and forms the next reflector from it.
+  auto householder = [&](int k) {  // wave 0: v into column k below the diagonal, e_k, d_k
+    const int m = N - k - 1;
+    const double x = lane < m ? A[(k + 1 + lane) * ld + k] : 0.0;
+    const double ss = wave_sum(x * x);
+    const double x0 = readlane_f64(x, 0);
+    double alpha = 0.0, v = 0.0;
+    if (ss > 0.0) {
+      const double nx = sqrt(ss);
+      alpha = x0 >= 0.0 ? -nx : nx;
+      const double v0 = x0 - alpha;
+      v = (lane == 0 ? v0 : x) / sqrt(ss - x0 * x0 + v0 * v0);
+    }
+    if (lane < m) A[(k + 1 + lane) * ld + k] = v;
+    if (lane == 0) {
+      e[k] = alpha;
+      d[k] = A[k * ld + k];
+    }
+  };
+  if (wave == 0) {
+    if (lane == 0) *fail = 0;
+    if (N >= 3) householder(0);
+  }
+  __syncthreads();
   for (int k = 0; k + 2 < N; ++k) {
     const int m = N - k - 1;
-    if (wave == 0) {
-      const double x = lane < m ? A[(k + 1 + lane) * ld + k] : 0.0;
-      const double ss = wave_sum(x * x);
-      const double x0 = __shfl(x, 0, 64);
-      double alpha = 0.0, v = 0.0;
-      if (ss > 0.0) {
-        const double nx = sqrt(ss);
-        alpha = x0 >= 0.0 ? -nx : nx;
-        const double v0 = x0 - alpha;
-        v = (lane == 0 ? v0 : x) / sqrt(ss - x0 * x0 + v0 * v0);
-      }
-      if (lane < m) A[(k + 1 + lane) * ld + k] = v;
-      if (lane == 0) {
-        e[k] = alpha;
-        d[k] = A[k * ld + k];
-        *fail = 0;
-      }
-    }
-    __syncthreads();
     {
       const int i = tid >> 4, g = tid & 15;  // row i, 16 lanes per row
       double s = 0.0, y = 0.0;
@@ -782,21 +854,33 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
       const double vi = i < m ? A[(k + 1 + i) * ld + k] : 0.0;
       const double kp = wave_sum((g == 0 && i < m) ? vi * p : 0.0);
       if (lane == 0) kpart[wave] = kp;
+      if (g == 0 && i < m) wv[i] = p;
       if (g == 0 && i < N) dot[i] = 2.0 * y;
-      __syncthreads();
-      double K = 0.0;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) K += kpart[q];
-      if (g == 0 && i < m) wv[i] = p - K * vi;
     }
     __syncthreads();
-    for (int x = tid; x < m * m; x += blockDim.x) {
-      const int i = x / m, j = x - i * m;
-      A[(k + 1 + i) * ld + k + 1 + j] -= A[(k + 1 + i) * ld + k] * wv[j] + wv[i] * A[(k + 1 + j) * ld + k];
-    }
-    for (int x = tid; x < N * m; x += blockDim.x) {
-      const int i = x / m, j = x - i * m;
-      Z[i * ld + k + 1 + j] -= dot[i] * A[(k + 1 + j) * ld + k];
+    double K = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) K += kpart[q];
+    if (wave == 0) {  // column k + 1 (trailing column 0), then the next reflector
+      if (lane < m) {
+        const double va = A[(k + 1 + lane) * ld + k], v0 = A[(k + 1) * ld + k];
+        A[(k + 1 + lane) * ld + k + 1] -= va * (wv[0] - K * v0) + (wv[lane] - K * va) * v0;
+      }
+      if (k + 3 < N) {
+        wave_sync();
+        householder(k + 1);
+      }
+    } else {  // the trailing block past column k + 1, and Q
+      const int t = tid - 64, nt = blockDim.x - 64, mb = m - 1;
+      for (int x = t; x < mb * mb; x += nt) {
+        const int i = x / mb + 1, j = x - (i - 1) * mb + 1;
+        const double vi = A[(k + 1 + i) * ld + k], vj = A[(k + 1 + j) * ld + k];
+        A[(k + 1 + i) * ld + k + 1 + j] -= vi * (wv[j] - K * vj) + (wv[i] - K * vi) * vj;
+      }
+      for (int x = t; x < N * m; x += nt) {
+        const int i = x / m, j = x - i * m;
+        Z[i * ld + k + 1 + j] -= dot[i] * A[(k + 1 + j) * ld + k];
+      }
     }
     __syncthreads();
   }
@@ -821,13 +905,21 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
       bhi[lane] = at ? lane + __ffsll(at) : N;       // lowest split at or above lane, + 1
     }
     wave_sync();
-    // Gershgorin interval of slot lane's block
+    // Gershgorin interval of slot lane's block (the whole matrix's when it
+    // does not split: one wavefront reduction)
+    const double rad = lane < N ? (lane > 0 ? fabs(e[lane - 1]) : 0.0) + (lane + 1 < N ? fabs(e[lane]) : 0.0) : 0.0;
+    const double gl = -wave_max(lane < N ? rad - d[lane] : -1e300);
+    const double gh = wave_max(lane < N ? d[lane] + rad : -1e300);
     if (lane < N) {
-      double a = 1e300, b = -1e300;
-      for (int i = blo[lane]; i < bhi[lane]; ++i) {
-        const double r = (i > blo[lane] ? fabs(e[i - 1]) : 0.0) + (i + 1 < bhi[lane] ? fabs(e[i]) : 0.0);
-        a = fmin(a, d[i] - r);
-        b = fmax(b, d[i] + r);
+      double a = gl, b = gh;
+      if (mask != 0ull) {
+        a = 1e300;
+        b = -1e300;
+        for (int i = blo[lane]; i < bhi[lane]; ++i) {
+          const double r = (i > blo[lane] ? fabs(e[i - 1]) : 0.0) + (i + 1 < bhi[lane] ? fabs(e[i]) : 0.0);
+          a = fmin(a, d[i] - r);
+          b = fmax(b, d[i] + r);
+        }
       }
       const double pad = 4.0 * 2.220446049250313e-16 * fmax(fabs(a), fabs(b)) + 1e-300;
       lo[lane] = a - pad;
@@ -848,11 +940,10 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   // slot (lane), TRI_ROUNDS rounds narrow each interval (TRI_SECT + 1)-fold
   {
     const int b0 = blo[lane], b1 = bhi[lane];
-    const double dv = lane < N ? d[lane] : 0.0, e2v = wv[lane];
     for (int round = 0; round < TRI_ROUNDS; ++round) {
       if (wave < TRI_SECT) {
         const double x = lo[lane] + (wave + 1) * ((hi[lane] - lo[lane]) * (1.0 / (TRI_SECT + 1)));
-        cnt[wave * 64 + lane] = sturm_count_all(dv, e2v, N, b0, b1, x);
+        cnt[wave * 64 + lane] = sturm_count_all(d, wv, N, b0, b1, x);
       }
       __syncthreads();
       if (wave == 0 && lane < N) {
@@ -891,71 +982,19 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   __syncthreads();
   if (T.W2 != nullptr) {
     // all slots in one pass (lane = slot): 1 / Dp in W, 1 / Dm in W2 at the
-    // odd stride ld, loops over all positions with a uniform index so that
-    // d_i and e_i come from registers by readlane
+    // odd stride ld.  Without splits every slot's block is [0, N) and the
+    // loops run unpredicated.
     if (wave == 0) {
       const bool act = lane < N;
       const int a0 = act ? blo[lane] : 0, a1 = act ? bhi[lane] : 0;
       const double lk = act ? lam[lane] : 0.0;
-      const double dv = act ? d[lane] : 0.0, ev = lane < N - 1 ? e[lane] : 0.0;
+      const int col = act ? T.rank[lane] : 0;
       double* rdp = W + (act ? lane : 0) * ld;
       double* rdm = T.W2 + (act ? lane : 0) * ld;
-      double sc = 0.0;
-      for (int i = 0; i < N; ++i) {
-        const double di = readlane_f64(dv, i), ei = readlane_f64(ev, i);
-        if (i >= a0 && i < a1) sc = fmax(sc, fabs(di) + (i < a1 - 1 ? fabs(ei) : 0.0));
-      }
-      const double tiny = 2.220446049250313e-16 * sc + 1e-300;
-      auto guard = [&](double v) { return fabs(v) >= tiny ? v : (v < 0.0 ? -tiny : tiny); };
-      double r_prev = 0.0;  // top-down pivots Dp
-      for (int i = 0; i < N; ++i) {
-        const double di = readlane_f64(dv, i), em = i > 0 ? readlane_f64(ev, i - 1) : 0.0;
-        if (i >= a0 && i < a1) {
-          r_prev = recip_nr(guard((di - lk) - (i > a0 ? em * em * r_prev : 0.0)));
-          rdp[i] = r_prev;
-        }
-      }
-      // bottom-up pivots Dm and gamma_i = Dm_i - e_(i-1)^2 / Dp_(i-1): the
-      // twist r is where |gamma| is smallest
-      double rnext = 0.0, gbest = 1.0 / 0.0;
-      int r = a1 - 1;
-      for (int i = N - 1; i >= 0; --i) {
-        const double di = readlane_f64(dv, i), ei = readlane_f64(ev, i);
-        const double em = i > 0 ? readlane_f64(ev, i - 1) : 0.0;
-        if (i >= a0 && i < a1) {
-          const double dm = guard((di - lk) - (i < a1 - 1 ? ei * ei * rnext : 0.0));
-          rnext = recip_nr(dm);
-          rdm[i] = rnext;
-          const double g = fabs(i > a0 ? dm - em * em * rdp[i - 1] : dm);
-          if (g < gbest) {
-            gbest = g;
-            r = i;
-          }
-        }
-      }
-      const int col = act ? T.rank[lane] : 0;
-      double z = 1.0, nrm = 1.0;
-      if (act) Z[r * ld + col] = 1.0;
-      for (int i = N - 2; i >= 0; --i) {
-        const double ei = readlane_f64(ev, i);
-        if (act && i >= a0 && i < r) {
-          z = -ei * rdp[i] * z;
-          Z[i * ld + col] = z;
-          nrm += z * z;
-        }
-      }
-      z = 1.0;
-      for (int i = 1; i < N; ++i) {
-        const double em = readlane_f64(ev, i - 1);
-        if (act && i > r && i < a1) {
-          z = -em * rdm[i] * z;
-          Z[i * ld + col] = z;
-          nrm += z * z;
-        }
-      }
-      const double inv = 1.0 / sqrt(nrm);
-      if (act)
-        for (int i = a0; i < a1; ++i) Z[i * ld + col] *= inv;
+      if (__ballot(act && (a0 != 0 || a1 != N)) == 0ull)
+        twisted_slot<false>(d, e, N, ld, 0, N, lk, act, col, rdp, rdm, Z);
+      else
+        twisted_slot<true>(d, e, N, ld, a0, a1, lk, act, col, rdp, rdm, Z);
     }
     __syncthreads();
   }
@@ -1099,7 +1138,7 @@ __device__ __forceinline__ int eig_density(const ScfLds& L, int N, int nocc, con
     B[i * ld + j] = L.rank[j] < nocc ? A[i * ld + j] : 0.0;
   }
   __syncthreads();
-  lds_matmul<false, true>(B, A, U, N, ld, ld);  // U = C_occ C^T
+  lds_matmul_sym<false, true>(B, A, U, N, ld, ld);  // U = C_occ C^T
   __syncthreads();
   return sweeps;
 }
@@ -1208,24 +1247,28 @@ constexpr double OA_TAU = 0.2;       // |first-order correction| above which a p
 // first-order correction exceeds OA_TAU (or whose eigenvalues agree to 1e-13)
 // is near-degenerate: when it is the only such pair of both its rows, it
 // takes the exact 2 x 2 Jacobi rotation instead, E_ij = -E_ji = tan(theta)
-// with tan(2 theta) = 2 S_ij / (lam_j - lam_i) (bounded by 1); inside a larger
-// group it is only re-orthogonalised, E_ij = R_ij / 2 (its invariant subspace
-// still converges, and for an exactly degenerate group any basis of it is an
-// eigenbasis).  Converged when the off-diagonal share of |S|^2 is at most
-// 1e-18 (the Jacobi criterion) and |R| <= 1e-13: returns the number of
-// updates, with X in L.U and the eigenvalues in L.eig.  Returns -1 (X then
-// undefined) after OA_MAX updates or on a non-finite update.  L.A, L.B and
+// with tan(2 theta) = 2 S_ij / (lam_j - lam_i) (bounded by 1), plus R_ij / 2;
+// inside a larger group it is only re-orthogonalised, E_ij = R_ij / 2 (its
+// invariant subspace still converges, and for an exactly degenerate group any
+// basis of it is an eigenbasis).  Converged when the off-diagonal share of
+// |S|^2 is at most 1e-18 (the Jacobi criterion) and |R| <= 1e-13, or after an
+// update with every |E_ij| <= 1e-9 (the next S and R would be second order in
+// E): returns the number of updates, with X in L.U and the eigenvalues in
+// L.eig.  Returns -1 (X then undefined) after OA_MAX updates or on a
+// non-finite update.  s_ready: L.A already holds S = X^T A X.  L.A, L.B and
 // L.rp are scratch.  N <= 64, 1024 threads.
-__device__ __forceinline__ int oa_refine(const ScfLds& L, int N) {
+__device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready) {
   const int ld = lds_ld(N), tid = threadIdx.x;
   double *M = L.M, *X = L.U, *S = L.A, *G = L.B;
   int* nbig = L.rp;  // near-degenerate pairs per row
   for (int k = 0;; ++k) {
-    lds_matmul<false, false>(M, X, G, N, ld, ld);  // G = A X
-    __syncthreads();
-    lds_matmul<true, false>(X, G, S, N, ld, ld);  // S = X^T A X
-    __syncthreads();
-    lds_matmul<true, false>(X, X, G, N, ld, ld);  // G = X^T X
+    if (k > 0 || !s_ready) {
+      lds_matmul<false, false>(M, X, G, N, ld, ld);  // G = A X
+      __syncthreads();
+      lds_matmul_sym<true, false>(X, G, S, N, ld, ld);  // S = X^T A X
+      __syncthreads();
+    }
+    lds_matmul_sym<true, false>(X, X, G, N, ld, ld);  // G = X^T X
     __syncthreads();
     double off = 0.0, tot = 0.0, rmax = 0.0, lmax = 0.0, z = 0.0;
     for (int x = tid; x < N * N; x += blockDim.x) {
@@ -1259,12 +1302,14 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N) {
     }
     __syncthreads();
     // E in place of S, the pair (i, j), (j, i) by one thread
-    double bad = 0.0;
+    double bad = 0.0, emax = 0.0;
     for (int x = tid; x < N * N; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
       if (i > j) continue;
       if (i == j) {
-        S[i * ld + i] = 0.5 * (1.0 - G[i * ld + i]);
+        const double eii = 0.5 * (1.0 - G[i * ld + i]);
+        S[i * ld + i] = eii;
+        emax = fmax(emax, fabs(eii));
         continue;
       }
       const double rij = -G[i * ld + j], rji = -G[j * ld + i];
@@ -1272,12 +1317,12 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N) {
       const double li = L.eig[i], lj = L.eig[j], dl = lj - li;
       double eij = (sym + lj * rij) / dl, eji = -(sym + li * rji) / dl;  // |.| <= OA_TAU unless replaced
       if (fabs(sym + lj * rij) > OA_TAU * fabs(dl) || fabs(dl) <= degen) {
-        if (nbig[i] == 1 && nbig[j] == 1) {  // isolated pair: exact rotation
+        if (nbig[i] == 1 && nbig[j] == 1) {  // isolated pair: exact rotation (+ re-orthogonalisation)
           const double h = 0.5 * dl;
           const double t = h != 0.0 ? copysign(1.0, h) * sym / (fabs(h) + sqrt(h * h + sym * sym))
                                     : (sym >= 0.0 ? 1.0 : -1.0);
-          eij = t;
-          eji = -t;
+          eij = t + 0.5 * rij;
+          eji = -t + 0.5 * rji;
         } else {
           eij = 0.5 * rij;
           eji = 0.5 * rji;
@@ -1285,10 +1330,15 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N) {
       }
       S[i * ld + j] = eij;
       S[j * ld + i] = eji;
-      bad += (fabs(eij) <= 1.0 + 1e-12 && fabs(eji) <= 1.0 + 1e-12) ? 0.0 : 1.0;  // only non-finite entries
+      bad += (fabs(eij) <= 1.5 && fabs(eji) <= 1.5) ? 0.0 : 1.0;  // only non-finite entries
+      emax = fmax(emax, fmax(fabs(eij), fabs(eji)));
     }
-    bad = block_sum(bad, L.red);
+    double z1 = 0.0, z2 = 0.0, z3 = 0.0;
+    block_reduce5(bad, z1, emax, z2, z3, L.red);
     if (bad != 0.0) return -1;
+    // an update of at most 1e-9 leaves the off-diagonal share and X^T X - I at
+    // the rounding level (both are second order in E): no further check
+    const bool last = emax <= 1e-9;
     lds_matmul<false, false>(X, S, G, N, ld, ld);  // G = X E
     __syncthreads();
     for (int x = tid; x < N * N; x += blockDim.x) {
@@ -1296,6 +1346,7 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N) {
       X[i * ld + j] += G[i * ld + j];
     }
     __syncthreads();
+    if (last) return k + 1;
   }
 }
 
@@ -1307,7 +1358,7 @@ __device__ inline void orthonormal_basis(const ScfLds& L, int N) {
   double *A = L.A, *U = L.U, *B = L.B;
   lds_matmul<false, true>(A, B, U, N, ld, ld);  // U = H L^T
   __syncthreads();
-  lds_matmul<false, false>(B, U, A, N, ld, ld);  // A = L U
+  lds_matmul_sym<false, false>(B, U, A, N, ld, ld);  // A = L U
   __syncthreads();
   for (int x = tid; x < N * N; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
@@ -1482,7 +1533,7 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
     __syncthreads();
     lds_matmul<false, false>(A, B, U, N, ld, ld);  // U = A Uprev
     __syncthreads();
-    lds_matmul<true, false>(B, U, A, N, ld, ld);  // A = Uprev^T U
+    lds_matmul_sym<true, false>(B, U, A, N, ld, ld);  // A = Uprev^T U
     __syncthreads();
     for (int x = tid; x < N * ld; x += blockDim.x) U[x] = B[x];
     __syncthreads();
@@ -1519,7 +1570,7 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
     }
     block_reduce5(off, tot, z0, z1, z2, L.red);
     if (off <= OA_ENTRY * tot) {
-      refined = oa_refine(L, N);
+      refined = oa_refine(L, N, true);  // S of the first iteration is A' itself
       if (refined >= 0) {
         path = 2;
       } else {
