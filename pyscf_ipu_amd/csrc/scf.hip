@@ -688,24 +688,29 @@ __device__ inline double recip_nr(double x) {
 // only: a zero coupling at the block start makes sign(p_b0 / p_(b0-1)) =
 // sign(d_b0 - x), the block's own first term.  Branch-free (sign bits and
 // masks), rescaled by a power of two every 8 positions; a zero counts with the
-// sign of its sign bit.
+// sign of its sign bit.  d and e2m1 have 64 entries, zero from N on (the last
+// chunk of 8 runs past N without counting).
 __device__ inline int sturm_count_all(const double* __restrict__ d, const double* __restrict__ e2m1, int N, int b0,
                                       int b1, double x) {
   double p = 1.0, pm = 0.0;
   int sp = 0, c = 0;  // sign of the previous term: 0 non-negative, -1 negative
   const int hi = b1 - 1;
-  for (int i0 = 0; i0 < N; i0 += 8) {
+  for (int i0 = 0; i0 < N; i0 += 8) {  // positions N..63 hold d = 0, e2m1 = 0 and count nowhere
+    double dd[8], ee[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dd[u] = d[i0 + u] - x;
+      ee[u] = e2m1[i0 + u];
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u;
-      if (i < N) {
-        const double pn = (d[i] - x) * p - e2m1[i] * pm;
-        pm = p;
-        p = pn;
-        const int sg = __double2hiint(pn) >> 31;
-        c -= (sg ^ sp) & (~((i - b0) | (hi - i)) >> 31);
-        sp = sg;
-      }
+      const double pn = dd[u] * p - ee[u] * pm;
+      pm = p;
+      p = pn;
+      const int sg = __double2hiint(pn) >> 31;
+      c -= (sg ^ sp) & (~((i - b0) | (hi - i)) >> 31);
+      sp = sg;
     }
     const double mx = fmax(fabs(p), fabs(pm));
     const int ex = (mx > 0x1p200 || mx < 0x1p-200) ? __builtin_amdgcn_frexp_exp(mx) : 0;
@@ -929,6 +934,7 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
       hi[lane] = 0.0;
       blo[lane] = 0;
       bhi[lane] = 0;
+      d[lane] = 0.0;
     }
     wv[lane] = (lane > 0 && lane < N) ? e[lane - 1] * e[lane - 1] : 0.0;  // e_(i-1)^2, 0 at splits
   }
@@ -1244,13 +1250,13 @@ constexpr double OA_TAU = 0.2;       // |first-order correction| above which a p
 //   E_ij = (S_ij + lam_j R_ij) / (lam_j - lam_i),   E_ii = R_ii / 2,
 //   X <- X + X E.
 // Convergence is quadratic while every |E_ij| is small.  A pair whose
-// first-order correction exceeds OA_TAU (or whose eigenvalues agree to 1e-13)
-// is near-degenerate: when it is the only such pair of both its rows, it
-// takes the exact 2 x 2 Jacobi rotation instead, E_ij = -E_ji = tan(theta)
-// with tan(2 theta) = 2 S_ij / (lam_j - lam_i) (bounded by 1), plus R_ij / 2;
-// inside a larger group it is only re-orthogonalised, E_ij = R_ij / 2 (its
-// invariant subspace still converges, and for an exactly degenerate group any
-// basis of it is an eigenbasis).  Converged when the off-diagonal share of
+// first-order correction exceeds OA_TAU is near-degenerate: when it is the
+// only such pair of both its rows, it takes the exact 2 x 2 Jacobi rotation
+// instead, E_ij = -E_ji = tan(theta) with tan(2 theta) = 2 S_ij / (lam_j -
+// lam_i) (bounded by 1), plus R_ij / 2; inside a larger group, and when the
+// two eigenvalues agree to 1e-13, it is only re-orthogonalised, E_ij = R_ij /
+// 2 (its invariant subspace still converges, and for an exactly degenerate
+// group any basis of it is an eigenbasis).  Converged when the off-diagonal share of
 // |S|^2 is at most 1e-18 (the Jacobi criterion) and |R| <= 1e-13, or after an
 // update with every |E_ij| <= 1e-9 (the next S and R would be second order in
 // E): returns the number of updates, with X in L.U and the eigenvalues in
@@ -1261,6 +1267,7 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready) {
   const int ld = lds_ld(N), tid = threadIdx.x;
   double *M = L.M, *X = L.U, *S = L.A, *G = L.B;
   int* nbig = L.rp;  // near-degenerate pairs per row
+  if (tid < 64) nbig[tid] = 0;
   for (int k = 0;; ++k) {
     if (k > 0 || !s_ready) {
       lds_matmul<false, false>(M, X, G, N, ld, ld);  // G = A X
@@ -1270,75 +1277,72 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready) {
     }
     lds_matmul_sym<true, false>(X, X, G, N, ld, ld);  // G = X^T X
     __syncthreads();
-    double off = 0.0, tot = 0.0, rmax = 0.0, lmax = 0.0, z = 0.0;
-    for (int x = tid; x < N * N; x += blockDim.x) {
-      const int i = x / N, j = x - i * N;
-      const double sv = S[i * ld + j], v = sv * sv;
-      tot += v;
-      rmax = fmax(rmax, fabs((i == j ? 1.0 : 0.0) - G[i * ld + j]));
-      if (i != j) {
-        off += v;
-      } else {
-        const double lam = sv / G[i * ld + i];
-        L.eig[i] = lam;
-        lmax = fmax(lmax, fabs(lam));
-      }
-    }
-    if (tid < N) nbig[tid] = 0;
-    block_reduce5(off, tot, rmax, lmax, z, L.red);  // also publishes L.eig, nbig
-    if (off <= 1e-18 * tot && rmax <= 1e-13) return k;
-    if (k == OA_MAX || !(off == off)) return -1;
-    const double degen = 1e-13 * lmax;
-    // near-degenerate pairs per row (integer counts: order-independent)
-    for (int x = tid; x < N * N; x += blockDim.x) {
-      const int i = x / N, j = x - i * N;
-      if (i >= j) continue;
-      const double sym = 0.5 * (S[i * ld + j] + S[j * ld + i]);
-      const double dl = L.eig[j] - L.eig[i];
-      if (fabs(sym - L.eig[j] * G[i * ld + j]) > OA_TAU * fabs(dl) || fabs(dl) <= degen) {
-        atomicAdd(&nbig[i], 1);
-        atomicAdd(&nbig[j], 1);
-      }
-    }
+    if (tid < N) L.eig[tid] = S[tid * ld + tid] / G[tid * ld + tid];
     __syncthreads();
-    // E in place of S, the pair (i, j), (j, i) by one thread
-    double bad = 0.0, emax = 0.0;
+    // one pass over the pairs i <= j: convergence measures and the first-order
+    // corrections E (in place of S); near-degenerate pairs are only counted
+    // here (their S entries stay for the second pass)
+    double off = 0.0, tot = 0.0, flags = 0.0, rmax = 0.0, emax = 0.0;
     for (int x = tid; x < N * N; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
       if (i > j) continue;
+      const double sij = S[i * ld + j];
       if (i == j) {
-        const double eii = 0.5 * (1.0 - G[i * ld + i]);
-        S[i * ld + i] = eii;
-        emax = fmax(emax, fabs(eii));
+        const double rii = 1.0 - G[i * ld + i];
+        tot += sij * sij;
+        rmax = fmax(rmax, fabs(rii));
+        S[i * ld + i] = 0.5 * rii;
+        emax = fmax(emax, fabs(0.5 * rii));
         continue;
       }
+      const double sji = S[j * ld + i];
       const double rij = -G[i * ld + j], rji = -G[j * ld + i];
-      const double sym = 0.5 * (S[i * ld + j] + S[j * ld + i]);
+      off += sij * sij + sji * sji;
+      tot += sij * sij + sji * sji;
+      rmax = fmax(rmax, fmax(fabs(rij), fabs(rji)));
+      const double sym = 0.5 * (sij + sji);
       const double li = L.eig[i], lj = L.eig[j], dl = lj - li;
-      double eij = (sym + lj * rij) / dl, eji = -(sym + li * rji) / dl;  // |.| <= OA_TAU unless replaced
-      if (fabs(sym + lj * rij) > OA_TAU * fabs(dl) || fabs(dl) <= degen) {
-        if (nbig[i] == 1 && nbig[j] == 1) {  // isolated pair: exact rotation (+ re-orthogonalisation)
-          const double h = 0.5 * dl;
-          const double t = h != 0.0 ? copysign(1.0, h) * sym / (fabs(h) + sqrt(h * h + sym * sym))
-                                    : (sym >= 0.0 ? 1.0 : -1.0);
-          eij = t + 0.5 * rij;
-          eji = -t + 0.5 * rji;
-        } else {
-          eij = 0.5 * rij;
-          eji = 0.5 * rji;
-        }
+      const double nij = sym + lj * rij, nji = sym + li * rji;
+      if (fabs(dl) <= 1e-13 * fmax(fabs(li), fabs(lj)) || fabs(nij) > OA_TAU * fabs(dl)) {
+        atomicAdd(&nbig[i], 1);
+        atomicAdd(&nbig[j], 1);
+        flags += 1.0;
+        continue;
       }
+      const double eij = nij / dl, eji = -nji / dl;
       S[i * ld + j] = eij;
       S[j * ld + i] = eji;
-      bad += (fabs(eij) <= 1.5 && fabs(eji) <= 1.5) ? 0.0 : 1.0;  // only non-finite entries
+      flags += (fabs(eij) <= 1.5 && fabs(eji) <= 1.5) ? 0.0 : 1e6;  // non-finite entries
       emax = fmax(emax, fmax(fabs(eij), fabs(eji)));
     }
-    double z1 = 0.0, z2 = 0.0, z3 = 0.0;
-    block_reduce5(bad, z1, emax, z2, z3, L.red);
-    if (bad != 0.0) return -1;
+    block_reduce5(off, tot, rmax, emax, flags, L.red);  // also publishes nbig and E
+    if (off <= 1e-18 * tot && rmax <= 1e-13) return k;
+    if (k == OA_MAX || !(flags < 1e6)) return -1;
+    if (flags > 0.0) {  // near-degenerate pairs
+      for (int x = tid; x < N * N; x += blockDim.x) {
+        const int i = x / N, j = x - i * N;
+        if (i >= j) continue;
+        const double sym = 0.5 * (S[i * ld + j] + S[j * ld + i]);
+        const double rij = -G[i * ld + j], rji = -G[j * ld + i];
+        const double li = L.eig[i], lj = L.eig[j], dl = lj - li;
+        const bool degen = fabs(dl) <= 1e-13 * fmax(fabs(li), fabs(lj));
+        if (!degen && !(fabs(sym + lj * rij) > OA_TAU * fabs(dl))) continue;
+        double eij = 0.5 * rij, eji = 0.5 * rji;
+        if (!degen && nbig[i] == 1 && nbig[j] == 1) {  // isolated pair: exact rotation (+ re-orthogonalisation)
+          const double h = 0.5 * dl;
+          const double t = copysign(1.0, h) * sym / (fabs(h) + sqrt(h * h + sym * sym));
+          eij += t;
+          eji -= t;
+        }
+        S[i * ld + j] = eij;
+        S[j * ld + i] = eji;
+      }
+      __syncthreads();
+      if (tid < 64) nbig[tid] = 0;  // read again only after the next products
+    }
     // an update of at most 1e-9 leaves the off-diagonal share and X^T X - I at
     // the rounding level (both are second order in E): no further check
-    const bool last = emax <= 1e-9;
+    const bool last = flags == 0.0 && emax <= 1e-9;
     lds_matmul<false, false>(X, S, G, N, ld, ld);  // G = X E
     __syncthreads();
     for (int x = tid; x < N * N; x += blockDim.x) {
