@@ -6,10 +6,11 @@
 // eigvects = L_inv^T eigh(L_inv H L_inv^T)[1] (:69, linalg_eigh :370-393);
 // D = 2 C_occ C_occ^T (:72); energy (:20-38).
 // One workgroup does the whole step in FP64 with the working matrices in LDS.
-// The eigensolver is a parallel cyclic Jacobi (round-robin pairing, N/2
-// disjoint rotations per round), warm-started from the previous step's
-// eigenvectors: A' = U_prev^T A U_prev is nearly diagonal after the first SCF
-// iterations, so a few sweeps suffice.
+// The eigensolver (N <= 64) is a tridiagonal solver on the first step and
+// wherever the previous step's eigenvectors leave A' = U_prev^T A U_prev far
+// from diagonal, and otherwise an iterative refinement of U_prev on MFMA
+// products (Ogita-Aishima); a parallel cyclic Jacobi (round-robin pairing, N/2
+// disjoint rotations per round) is the fallback and serves N > 64.
 #include "batch.h"
 #include "common.h"
 #include "wave.h"
