@@ -281,6 +281,20 @@ int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, i
                   const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
                   int split, int tile, void* d_part_v, double* d_part_e, int64_t part_v_elems, int64_t part_e_elems,
                   void* stream);
+/* pscf_batch_xc with the AO values and gradients evaluated inside the kernel
+ * (eval_gto, nanoDFT.py:219-220, fused into exchange_correlation,
+ * nanoDFT.py:105-123) instead of read from a stored AO block: each
+ * workgroup tabulates its molecule's s and p shells (d_atm/d_bas/d_env as
+ * given to pscf_batch_ao) in LDS and evaluates the tile's points from
+ * d_coords [G][3].  nsh_max >= the shells of any molecule of the call (at most
+ * 80), kp >= the primitives of any shell (at most 16); a molecule with more
+ * shells gets NaN.  Same results as pscf_batch_ao + pscf_batch_xc up to
+ * rounding, without the 4 x nao x ngrid AO block in HBM. */
+int pscf_batch_xc_direct(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                         const double* d_ctl, const double* d_coords, const int* d_atm, const int* d_bas,
+                         const double* d_env, int nsh_max, int kp, const void* d_dm, const void* d_w, double eps,
+                         void* d_vxc, double* d_exc, int split, int tile, void* d_part_v, double* d_part_e,
+                         int64_t part_v_elems, int64_t part_e_elems, void* stream);
 /* Divides every molecule's packed s8 ERIs (pscf_batch_eri_s8) in place by
  * their multiplicity 2^([p==q] + [r==s] + [pq==rs]), as the reference does
  * before its contraction (sparse_symmetric_ERI.py:68-79, 180-182); max_nao

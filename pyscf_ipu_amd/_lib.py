@@ -76,6 +76,9 @@ _SIGS = {
     "pscf_batch_xc": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_double, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64,
                               c_void_p]),
+    "pscf_batch_xc_direct": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_void_p,
+                                     c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     "pscf_batch_chol": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pscf_batch_guess": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),

@@ -118,6 +118,10 @@ class BatchRKS:
         self.dtype = dtype
         self.tdt, self.code, _ = device.DTYPES[dtype]
         self.split = {"xc": 1, "jk": 1}
+        # FP32 XC evaluates the AO values in the kernel from the shells (no
+        # stored 4 x nao x ngrid AO block); FP64 keeps the stored form, whose
+        # tile pipeline at nao 65-80 leaves no LDS for the shell table
+        self.ao_direct = dtype == "fp32"
         self.xc_tile = 32
         self.jk_threads = 256
         self.tuning = None
@@ -218,6 +222,9 @@ class BatchRKS:
         self.atom_nao = np.bincount(sh_atom, weights=width, minlength=A).astype(np.int32)
         self.atom_elem, self.atom_pt0, self.atom_mol = eslot, atom_pt0, atom_mol
         self.desc = desc
+        # shell-table bounds of the direct XC form: shells per molecule, primitives per shell
+        self.nsh_max = int(nbas.max()) if M else 1
+        self.kp = int(bas[:, 2].max()) if len(bas) else 1
         self.sizes = {"eri": int(n_s8.sum()), "mat": int((nao * nao).sum()), "vec": int(nao.sum()),
                       "ao": int(ao_sz.sum()), "state": int(st.sum()), "grid": int(ngrid.sum())}
         self.grid_work = _chunks(apts, 256)
@@ -281,10 +288,11 @@ class BatchRKS:
         tm("becke")
         _lib.call("pscf_batch_becke", self.code, P(self.d_desc), P(self.ptmol), P(self.owner), P(self.coords),
                   P(self.vol), G, P(self.d_atm), P(self.d_env), P(self.w), None, st)
-        self.ao = device.empty(S["ao"], T)
-        tm("ao")
-        _lib.call("pscf_batch_ao", self.code, P(self.d_desc), P(self.ptmol), P(self.coords), G, P(self.d_atm),
-                  P(self.d_bas), P(self.d_env), P(self.ao), st)
+        if not self.ao_direct:
+            self.ao = device.empty(S["ao"], T)
+            tm("ao")
+            _lib.call("pscf_batch_ao", self.code, P(self.d_desc), P(self.ptmol), P(self.coords), G, P(self.d_atm),
+                      P(self.d_bas), P(self.d_env), P(self.ao), st)
         # one-electron part, orthogonaliser, guess
         nm = S["mat"]
         self.S, self.Hc, self.Linv = device.empty(nm, f64), device.empty(nm, f64), device.empty(nm, f64)
@@ -347,10 +355,16 @@ class BatchRKS:
         P, st = device.ptr, device.stream()
         for start, count, cmax in self.nao_classes:
             order = ctypes.c_void_p(self.d_order.data_ptr() + 4 * start)
-            _lib.call("pscf_batch_xc", self.code, P(self.d_desc), order, count, cmax, ctl_ptr, P(self.ao), P(self.Dt),
-                      P(self.w), 1e-20, P(self.Vxc if vxc is None else vxc), P(self.exc if exc is None else exc),
-                      class_split(split, count), tile, P(self.xc_parts), P(self.xc_parts_e), self.xc_parts_n,
-                      self.xc_parts_e_n, st)
+            out = (P(self.Vxc if vxc is None else vxc), P(self.exc if exc is None else exc),
+                   class_split(split, count), tile, P(self.xc_parts), P(self.xc_parts_e), self.xc_parts_n,
+                   self.xc_parts_e_n, st)
+            if self.ao_direct:
+                _lib.call("pscf_batch_xc_direct", self.code, P(self.d_desc), order, count, cmax, ctl_ptr,
+                          P(self.coords), P(self.d_atm), P(self.d_bas), P(self.d_env), self.nsh_max, self.kp,
+                          P(self.Dt), P(self.w), 1e-20, *out)
+            else:
+                _lib.call("pscf_batch_xc", self.code, P(self.d_desc), order, count, cmax, ctl_ptr, P(self.ao),
+                          P(self.Dt), P(self.w), 1e-20, *out)
 
     def _jk(self, ctl_ptr, split, threads, dm, out=None):
         P, st = device.ptr, device.stream()
@@ -373,7 +387,7 @@ class BatchRKS:
         `self.tuning` records the timings (ms per launch) next to the fixed
         tiling (32-point tiles, 256-thread J/K workgroups, one workgroup per
         molecule)."""
-        key = (self.dtype, self.M, self.max_nao, self.sizes["eri"], self.sizes["grid"])
+        key = (self.dtype, self.M, self.max_nao, self.sizes["eri"], self.sizes["grid"], self.ao_direct)
         if key in _TUNE_CACHE:
             self.tuning = dict(_TUNE_CACHE[key], cached=True)
             self.split = dict(self.tuning["chosen_split"])

@@ -217,9 +217,236 @@ struct XDReg {
 template <typename T, int XT, int NP>
 using XVacc = typename XMTraits<T>::v4[XTile<XT, NP>::NBW];
 
+// Staging of a tile's AO values and gradients into Phi[c][p][mu] (row stride
+// XS), mu < N; padding columns are left as they are.
+// Stored form: read from ao[c * sc + mu * smu + p * sp] in HBM.  Each wave
+// moves one (c, mu) row of XT points per 64 / XT rows and instruction (lanes
+// walk points, coalesced), eight rows' loads in flight before their LDS
+// stores; c = row / N comes from three compares, not a division.
 template <typename T, int XT, int NP>
+struct XcStageStored {
+  const T* ao;
+  int sc, smu, sp, g, N;
+  __device__ inline void load(int) {}
+  __device__ inline void operator()(T* Phi, int p0, int /*p0_next*/) {
+    constexpr int XS = XTile<XT, NP>::XS;
+    constexpr int RPW = 64 / XT;  // rows per wave instruction
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int p = lane % XT, sub = lane / XT, nrow = 4 * N, rstride = 4 * RPW;
+    const int pt = p0 + p;
+    const bool in = pt < g;
+    const int ptoff = pt * sp;
+    for (int base = wave * RPW + sub; base < nrow; base += 8 * rstride) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int row = base + u * rstride;
+        const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
+        v[u] = (row < nrow && in) ? ao[c * sc + mu * smu + ptoff] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int row = base + u * rstride;
+        const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
+        if (row < nrow) Phi[(c * XT + p) * XS + mu] = v[u];
+      }
+    }
+  }
+};
+
+// One molecule's shells in LDS for the direct form, as records: an s shell,
+// a p shell, or an s shell and the p shell after it on the same atom with the
+// same exponents (STO-3G's and 6-31G's sp shells), which then share their
+// exponentials.  Records are ordered s, sp, p, so the two records a wave
+// evaluates at once mostly take the same branch.
+//   hdr[4 r]:  centre x, y, z and the squared distance beyond which every
+//              primitive's exponential is below 2^-48 (such values are written as 0);
+//   info[r]:   mu | kind << 8 | nprim << 12, kind 0 = s, 1 = p, 2 = sp (s at
+//              mu, p at mu + 1 .. mu + 3), mu = the record's first AO;
+//   prim[r kp + k]: {e log2(e), e, c_s, c_p}, the contraction coefficients
+//              times the angular normalisation (c_s = 0 for p, c_p = 0 for s).
+template <typename T>
+struct alignas(16) XcPrim {
+  T a, e, cs, cp;
+};
+
+template <typename T>
+struct XcShells {
+  double* hdr;
+  int* info;
+  XcPrim<T>* prim;
+  int nrec, kp;
+};
+
+__host__ __device__ inline size_t xc_shell_prim_off(int nsh_max) {
+  return ((size_t)nsh_max * (4 * sizeof(double) + 3 * sizeof(int)) + 31) & ~(size_t)31;
+}
+
+__host__ __device__ inline size_t xc_shell_bytes(int nsh_max, int kp, size_t es) {
+  return xc_shell_prim_off(nsh_max) + (size_t)nsh_max * kp * 4 * es;
+}
+
+// Builds the table at `base` (32-byte aligned) from the molecule's shells
+// bas[bas0 .. bas0 + nsh) (the s and p shells eval_gto evaluates,
+// nanoDFT.py:219-220).  Ends with a barrier.  A shell with more than kp
+// primitives gets NaN coefficients, so its molecule's result is NaN.
+template <typename T>
+__device__ inline XcShells<T> xc_build_shells(unsigned char* base, int nsh_max, int kp, int bas0, int nsh,
+                                              const int* __restrict__ atm, const int* __restrict__ bas,
+                                              const double* __restrict__ env) {
+  XcShells<T> t;
+  t.hdr = reinterpret_cast<double*>(base);
+  t.info = reinterpret_cast<int*>(t.hdr + 4 * nsh_max);
+  int* lsh = t.info + nsh_max;     // l of each shell
+  int* okey = lsh + nsh_max;       // record order key: 0 s, 1 sp, 2 p, 3 none (p merged into the s before it)
+  t.prim = reinterpret_cast<XcPrim<T>*>(base + xc_shell_prim_off(nsh_max));
+  t.kp = kp;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nsh; i += blockDim.x) lsh[i] = bas[(bas0 + i) * BAS_SLOTS + 1];
+  __syncthreads();
+  // s shell i and p shell i + 1 merge when they share atom and exponents
+  auto merges = [&](int i) {
+    if (i + 1 >= nsh || lsh[i] != 0 || lsh[i + 1] != 1) return false;
+    const int* b = bas + (bas0 + i) * BAS_SLOTS;
+    const int* q = b + BAS_SLOTS;
+    if (b[0] != q[0] || b[2] != q[2]) return false;
+    for (int k = 0; k < b[2]; ++k)
+      if (env[b[5] + k] != env[q[5] + k]) return false;
+    return true;
+  };
+  for (int i = tid; i < nsh; i += blockDim.x) {
+    const bool into_prev = i > 0 && merges(i - 1);
+    okey[i] = into_prev ? 3 : (merges(i) ? 1 : (lsh[i] == 0 ? 0 : 2));
+  }
+  __syncthreads();
+  int nrec = 0;
+  for (int j = 0; j < nsh; ++j) nrec += okey[j] < 3;
+  t.nrec = nrec;
+  for (int i = tid; i < nsh; i += blockDim.x) {
+    const int key = okey[i];
+    if (key == 3) continue;
+    int mu = 0, pos = 0;
+    for (int j = 0; j < nsh; ++j) {
+      if (j < i) mu += 2 * lsh[j] + 1;
+      pos += okey[j] < key || (okey[j] == key && j < i);
+    }
+    const int* b = bas + (bas0 + i) * BAS_SLOTS;
+    const int* q = b + BAS_SLOTS;  // the p shell of an sp record
+    const double* R = env + atm[b[0] * ATM_SLOTS + 1];
+    const int np = b[2];
+    const int npk = np < kp ? np : kp;
+    const int kind = key == 0 ? 0 : (key == 1 ? 2 : 1);
+    double amin = 1e300;
+    for (int k = 0; k < npk; ++k) {
+      const double e = env[b[5] + k];
+      const double cs = kind == 1 ? 0.0 : ANG_S * env[b[6] + k];
+      const double cp = kind == 0 ? 0.0 : ANG_P * env[(kind == 2 ? q : b)[6] + k];
+      XcPrim<T> pr;
+      pr.a = (T)(e * 1.4426950408889634);  // log2(e)
+      pr.e = (T)e;
+      pr.cs = np > kp ? T(NAN) : (T)cs;
+      pr.cp = np > kp ? T(NAN) : (T)cp;
+      t.prim[pos * kp + k] = pr;
+      amin = fmin(amin, e * 1.4426950408889634);
+    }
+    // unused slots contribute 2^0 times zero coefficients
+    for (int k = npk; k < kp; ++k) t.prim[pos * kp + k] = XcPrim<T>{T(0), T(0), T(0), T(0)};
+    t.hdr[4 * pos] = R[0];
+    t.hdr[4 * pos + 1] = R[1];
+    t.hdr[4 * pos + 2] = R[2];
+    t.hdr[4 * pos + 3] = 48.0 / amin;
+    t.info[pos] = mu | (kind << 8) | (npk << 12);
+  }
+  __syncthreads();
+  return t;
+}
+
+// Direct form: the AO values and gradients of the tile's points evaluated
+// from the shell records (the values pscf_batch_ao stores).  Thread t takes
+// point t % XT of the tile (XC_THREADS is a multiple of XT, so a thread keeps
+// its point) and records t / XT, t / XT + XC_THREADS / XT, ...; the point's
+// coordinates for the next tile are loaded at the end of each stage, so their
+// latency hides behind the tile's GEMMs.  Differences to the centres are
+// formed in FP64 and rounded to T; the exponentials are 2^(-a r^2) with
+// a = e log2(e) (one v_exp_f32 in FP32).
+template <typename T, int XT, int NP>
+struct XcStageDirect {
+  XcShells<T> sh;
+  const double* coords;  // the molecule's points, [g][3]
+  int g;
+  double px, py, pz;
+  __device__ inline void load(int p0) {
+    const int pt = p0 + (int)(threadIdx.x % XT);
+    const bool in = pt < g;
+    px = in ? coords[3 * pt] : 0.0;
+    py = in ? coords[3 * pt + 1] : 0.0;
+    pz = in ? coords[3 * pt + 2] : 0.0;
+  }
+  __device__ static inline T exp2n(T x) {
+    if constexpr (sizeof(T) == 4)
+      return __builtin_amdgcn_exp2f(x);
+    else
+      return exp2(x);
+  }
+  __device__ inline void operator()(T* Phi, int p0, int p0_next) {
+    constexpr int XS = XTile<XT, NP>::XS;
+    const int tid = threadIdx.x, p = tid % XT;
+    const bool in = p0 + p < g;
+    const double x = px, y = py, z = pz;
+    load(p0_next);
+    for (int r = tid / XT; r < sh.nrec; r += XC_THREADS / XT) {
+      const int info = sh.info[r];
+      const int mu = info & 255, kind = (info >> 8) & 15, np = info >> 12;
+      const double* h = sh.hdr + 4 * r;
+      const T dx = (T)(x - h[0]), dy = (T)(y - h[1]), dz = (T)(z - h[2]);
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      // the first three primitive slots are evaluated unconditionally (kp >=
+      // 3, unused slots hold zero coefficients), so their loads issue with the
+      // header's and no lane branches on the primitive count
+      XcPrim<T> q3[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q3[k] = sh.prim[r * sh.kp + k];
+      T rs = T(0), ds = T(0), rp = T(0), dp = T(0);
+      auto add = [&](const XcPrim<T>& q) {
+        const T ex = exp2n(-q.a * r2);
+        const T te = q.e * ex;
+        rs += q.cs * ex;
+        ds += q.cs * te;
+        rp += q.cp * ex;
+        dp += q.cp * te;
+      };
+      if (in && (double)r2 <= h[3]) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) add(q3[k]);
+        for (int k = 3; k < np; ++k) add(sh.prim[r * sh.kp + k]);
+      }
+      ds *= T(-2);
+      dp *= T(-2);
+      T* qs = Phi + p * XS + mu;
+      if (kind != 1) {
+        qs[0] = rs;
+        qs[XT * XS] = ds * dx;
+        qs[2 * XT * XS] = ds * dy;
+        qs[3 * XT * XS] = ds * dz;
+      }
+      if (kind != 0) {
+        T* qp = qs + (kind == 2 ? 1 : 0);
+        const T d[3] = {dx, dy, dz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          qp[a] = d[a] * rp;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) qp[(1 + c) * XT * XS + a] = (a == c ? rp : T(0)) + d[a] * d[c] * dp;
+        }
+      }
+    }
+  }
+};
+
+template <typename T, int XT, int NP, class Stage>
 __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int tile0, int tstride, int tile_end,
-                                XVacc<T, XT, NP>& vacc, double& e_loc) {
+                                Stage& stage, XVacc<T, XT, NP>& vacc, double& e_loc) {
   typedef typename XMTraits<T>::v4 v4;
   typedef XTile<XT, NP> L;
   constexpr int PB = L::PB, NSLOT = L::NSLOT, NB = L::NB, XS = L::XS, XD = L::XD, NVB = L::NVB, NBW = L::NBW;
@@ -268,38 +495,18 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
   const int b1 = lg * XD + slot * 16 + li;                           // GEMM 1 B: + k0 XD + jb NSLOT 16
   const int e1 = (pb * 16 + XMTraits<T>::row(lg, 0)) * XS + slot * 16 + li;  // epilogue
   const int a2 = lg * XS + li, b2 = lg * XD + li;                    // GEMM 2: + block and k0 offsets
-  // AO strides as 32-bit element offsets (the host checks that a molecule's
-  // AO block is below 2^31 elements)
-  const int sc = (int)a.sc, smu = (int)a.smu, sp = (int)a.sp;
+  // Every load above (D, the first tile's coordinates) completes here.  The
+  // loop then carries only its own prefetches (the next tile's coordinates,
+  // this tile's weights), each waited for where it is used, instead of the
+  // compiler merging the loop entry's pending loads into a full wait inside
+  // every tile.
+  __builtin_amdgcn_s_waitcnt(0);
   for (int tile = tile0; tile < tile_end; tile += tstride) {
     const int p0 = tile * XT;
+    // this tile's weight for the pointwise step, loaded before the staging
+    const T wpre = (tid < XT && p0 + tid < a.g) ? a.w[p0 + tid] : T(0);
     __syncthreads();
-    // Stage the tile: Phi[c][p][mu] = ao[c][mu][p0 + p].  Each wave moves
-    // one (c, mu) row of XT points per 64 / XT rows and instruction (lanes
-    // walk points, coalesced), eight rows' loads in flight before their LDS
-    // stores; c = row / N comes from three compares, not a division.
-    {
-      constexpr int RPW = 64 / XT;  // rows per wave instruction
-      const int p = lane % XT, sub = lane / XT, nrow = 4 * N, rstride = 4 * RPW;
-      const int pt = p0 + p;
-      const bool in = pt < a.g;
-      const int ptoff = pt * sp;
-      for (int base = wave * RPW + sub; base < nrow; base += 8 * rstride) {
-        T v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int row = base + u * rstride;
-          const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
-          v[u] = (row < nrow && in) ? a.ao[c * sc + mu * smu + ptoff] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int row = base + u * rstride;
-          const int c = (row >= N) + (row >= 2 * N) + (row >= 3 * N), mu = row - c * N;
-          if (row < nrow) Phi[(c * XT + p) * XS + mu] = v[u];
-        }
-      }
-    }
+    stage(Phi, p0, p0 + tstride * XT);
     __syncthreads();
     // GEMM 1 and the density epilogue: wave (pb, slot) takes point block pb
     // and the column blocks slot, slot + NSLOT, ...; its 16 partial sums
@@ -333,21 +540,40 @@ __device__ inline void xc_tiles(const XCArgs<T>& a, unsigned char* smem, int til
       rpart[(slot * 4 + c) * XT + pb * 16 + XMTraits<T>::row(lg, j)] = tot;
     }
     __syncthreads();
-    if (tid < XT) {
-      const int pt = p0 + tid;
-      T rc[4];
+    // pointwise B3LYP: wave w < 3 evaluates part w of the functional
+    // (Slater + VWN, B88, LYP) at the tile's points (lane = point); the parts
+    // meet in LDS (Y's space, free until Y is formed) and the first XT
+    // threads, which hold rho and its gradient, combine them
+    T rc[4];
+    if (wave < 3 && lane < XT) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         T v = T(0);
 #pragma unroll
-        for (int q = 0; q < NSLOT; ++q) v += rpart[(q * 4 + c) * XT + tid];
+        for (int q = 0; q < NSLOT; ++q) v += rpart[(q * 4 + c) * XT + lane];
         rc[c] = v;
       }
+      T rho, gam, e = T(0), vr = T(0), vn = T(0);
+      b3lyp_inputs<T>(rc[0], rc[1], rc[2], rc[3], a.eps, rho, gam);
+      if (wave == 0)
+        b3lyp_part<T, 0>(rho, gam, e, vr, vn);
+      else if (wave == 1)
+        b3lyp_part<T, 1>(rho, gam, e, vr, vn);
+      else
+        b3lyp_part<T, 2>(rho, gam, e, vr, vn);
+      Y[(3 * wave) * XT + lane] = e;
+      Y[(3 * wave + 1) * XT + lane] = vr;
+      Y[(3 * wave + 2) * XT + lane] = vn;
+    }
+    __syncthreads();
+    if (tid < XT) {
+      const int pt = p0 + tid;
       T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0);
       if (pt < a.g) {
-        T e, vr, vn;
-        b3lyp_point<T>(rc[0], rc[1], rc[2], rc[3], a.eps, e, vr, vn);
-        const T wt = a.w[pt];
+        const T e = Y[tid] + Y[3 * XT + tid] + Y[6 * XT + tid];
+        const T vr = Y[XT + tid] + Y[4 * XT + tid] + Y[7 * XT + tid];
+        const T vn = Y[2 * XT + tid] + Y[5 * XT + tid] + Y[8 * XT + tid];
+        const T wt = wpre;
         e_loc += (double)rc[0] * (double)wt * (double)e;
         c0 = T(0.5) * vr * wt;
         const T f = T(4) * vn * wt;
@@ -420,7 +646,10 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
   constexpr int XT = XMTraits<T>::TILE;
   double e_loc = 0.0;
   XVacc<T, XT, NP> vacc;
-  xc_tiles<T, XT, NP>(a, smem, blockIdx.x, gridDim.x, (a.g + XT - 1) / XT, vacc, e_loc);
+  // AO strides as 32-bit element offsets (the host checks that the AO block
+  // is below 2^31 elements)
+  XcStageStored<T, XT, NP> stage{a.ao, (int)a.sc, (int)a.smu, (int)a.sp, a.g, a.N};
+  xc_tiles<T, XT, NP>(a, smem, blockIdx.x, gridDim.x, (a.g + XT - 1) / XT, stage, vacc, e_loc);
   xm_store_v<T, XT, NP>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
   double* red = reinterpret_cast<double*>(smem);
   __syncthreads();
@@ -446,14 +675,42 @@ __host__ __device__ inline bool xc_parts_fit(int nchunk, int nmol, int max_nao, 
   return (int64_t)nchunk * nmol * max_nao * max_nao <= part_v_elems && (int64_t)nchunk * nmol <= part_e_elems;
 }
 
+// Direct form of the batched kernel: AO values from the points' coordinates
+// and the molecules' shells (bas/env, as pscf_batch_ao reads them) instead of
+// a stored AO block; the shell table of nsh_max shells with kp primitive slots
+// sits at byte tab_off of the LDS.
+struct XcDirect {
+  const double* coords;
+  const int* atm;
+  const int* bas;
+  const double* env;
+  int nsh_max, kp, tab_off;
+};
+
+// A molecule the launch's shapes do not cover (nao above max_nao, a 32-bit
+// AO offset overflow, or in the direct form more shells than the table holds).
+__device__ inline bool xc_mol_outside(const BatchMol& Mo, int max_nao, int nsh_max) {
+  return Mo.nao > max_nao || (int64_t)4 * Mo.nao * Mo.ngrid >= ((int64_t)1 << 31) ||
+         (nsh_max > 0 && Mo.nbas > nsh_max);
+}
+
+// FP32 with 32-point tiles: three workgroups per CU up to nao 64 (their LDS
+// fits), two above; the register cap this sets costs a few spilled values
+// outside the GEMM loops and runs faster than fewer resident workgroups.
 template <typename T, int XT, int NP>
-__global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __restrict__ mols,
+constexpr int xc_batch_min_blocks() {
+  return (sizeof(T) == 4 && XT == 32) ? (NP <= 64 ? 3 : 2) : 1;
+}
+
+template <typename T, int XT, int NP, bool DIRECT>
+__global__ void __launch_bounds__(XC_THREADS, (xc_batch_min_blocks<T, XT, NP>())) xc_batch_kernel(const BatchMol* __restrict__ mols,
                                                               const int* __restrict__ order, int max_nao,
                                                               const double* __restrict__ ctl, const T* __restrict__ ao,
-                                                              const T* __restrict__ dm, const T* __restrict__ w,
-                                                              T eps, T* __restrict__ vxc, double* __restrict__ exc,
-                                                              T* __restrict__ part_v, double* __restrict__ part_e,
-                                                              int64_t part_v_elems, int64_t part_e_elems) {
+                                                              XcDirect da, const T* __restrict__ dm,
+                                                              const T* __restrict__ w, T eps, T* __restrict__ vxc,
+                                                              double* __restrict__ exc, T* __restrict__ part_v,
+                                                              double* __restrict__ part_e, int64_t part_v_elems,
+                                                              int64_t part_e_elems) {
   static_assert(XC_CHUNK_PTS % XT == 0, "a chunk is a whole number of tiles");
   extern __shared__ __align__(16) unsigned char smem[];
   const int m = order[blockIdx.x];
@@ -461,9 +718,9 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao;
   const int part = blockIdx.y, nparts = gridDim.y;
-  // LDS and tiles were sized for max_nao, offsets are 32-bit: flag a molecule
-  // outside those bounds
-  if (N > max_nao || (int64_t)4 * N * Mo.ngrid >= ((int64_t)1 << 31)) {
+  // LDS and tiles were sized for max_nao (and nsh_max shells), offsets are
+  // 32-bit: flag a molecule outside those bounds
+  if (xc_mol_outside(Mo, max_nao, DIRECT ? da.nsh_max : 0)) {
     if (part == 0) {
       for (int x = threadIdx.x; x < N * N; x += blockDim.x) vxc[Mo.mat0 + x] = T(NAN);
       if (threadIdx.x == 0) exc[m] = NAN;
@@ -473,7 +730,7 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   XCArgs<T> a;
   a.N = N;
   a.g = (int)Mo.ngrid;
-  a.ao = ao + Mo.ao0;
+  a.ao = DIRECT ? nullptr : ao + Mo.ao0;
   a.sc = (int64_t)a.N * a.g;
   a.smu = a.g;
   a.sp = 1;
@@ -485,33 +742,48 @@ __global__ void __launch_bounds__(XC_THREADS) xc_batch_kernel(const BatchMol* __
   const int nchunk = (a.g + XC_CHUNK_PTS - 1) / XC_CHUNK_PTS;
   double* red = reinterpret_cast<double*>(smem + sizeof(T) * N * N);
   XVacc<T, XT, NP> vacc;
-  if (nchunk <= 1) {
-    if (part != 0) return;
-    double e_loc = 0.0;
-    xc_tiles<T, XT, NP>(a, smem, 0, 1, ntiles, vacc, e_loc);
-    __syncthreads();
-    T* Vl = reinterpret_cast<T*>(smem);
-    xm_store_v<T, XT, NP>(vacc, N, Vl, N);
-    __syncthreads();
-    T* out = vxc + Mo.mat0;
-    for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
-      const int i = x / N, j = x - i * N;
-      out[x] = Vl[x] + Vl[j * N + i];
+  auto run = [&](auto& stage) {
+    if (nchunk <= 1) {
+      if (part != 0) return;
+      double e_loc = 0.0;
+      stage.load(0);
+      xc_tiles<T, XT, NP>(a, smem, 0, 1, ntiles, stage, vacc, e_loc);
+      __syncthreads();
+      T* Vl = reinterpret_cast<T*>(smem);
+      xm_store_v<T, XT, NP>(vacc, N, Vl, N);
+      __syncthreads();
+      T* out = vxc + Mo.mat0;
+      for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+        const int i = x / N, j = x - i * N;
+        out[x] = Vl[x] + Vl[j * N + i];
+      }
+      __syncthreads();
+      const double e = xc_block_energy(e_loc, red);
+      if (threadIdx.x == 0) exc[m] = e;
+      return;
     }
-    __syncthreads();
-    const double e = xc_block_energy(e_loc, red);
-    if (threadIdx.x == 0) exc[m] = e;
-    return;
-  }
-  if (!xc_parts_fit(nchunk, gridDim.x, max_nao, part_v_elems, part_e_elems)) return;  // marked by the reduction
-  for (int c = part; c < nchunk; c += nparts) {
-    double e_loc = 0.0;
-    xc_tiles<T, XT, NP>(a, smem, c * TPC, 1, min(ntiles, (c + 1) * TPC), vacc, e_loc);
-    __syncthreads();
-    const int64_t slot = (int64_t)c * gridDim.x + blockIdx.x;
-    xm_store_v<T, XT, NP>(vacc, N, part_v + slot * max_nao * max_nao, N);
-    const double e = xc_block_energy(e_loc, red);
-    if (threadIdx.x == 0) part_e[slot] = e;
+    if (!xc_parts_fit(nchunk, gridDim.x, max_nao, part_v_elems, part_e_elems)) return;  // marked by the reduction
+    for (int c = part; c < nchunk; c += nparts) {
+      double e_loc = 0.0;
+      stage.load(c * XC_CHUNK_PTS);
+      xc_tiles<T, XT, NP>(a, smem, c * TPC, 1, min(ntiles, (c + 1) * TPC), stage, vacc, e_loc);
+      __syncthreads();
+      const int64_t slot = (int64_t)c * gridDim.x + blockIdx.x;
+      xm_store_v<T, XT, NP>(vacc, N, part_v + slot * max_nao * max_nao, N);
+      const double e = xc_block_energy(e_loc, red);
+      if (threadIdx.x == 0) part_e[slot] = e;
+    }
+  };
+  if constexpr (DIRECT) {
+    XcStageDirect<T, XT, NP> stage;
+    stage.sh = xc_build_shells<T>(smem + da.tab_off, da.nsh_max, da.kp, (int)Mo.bas0, (int)Mo.nbas, da.atm, da.bas,
+                                  da.env);
+    stage.coords = da.coords + 3 * Mo.grid0;
+    stage.g = a.g;
+    run(stage);
+  } else {
+    XcStageStored<T, XT, NP> stage{a.ao, (int)a.sc, (int)a.smu, (int)a.sp, a.g, a.N};
+    run(stage);
   }
 }
 
@@ -524,12 +796,12 @@ __global__ void __launch_bounds__(256) xc_batch_reduce_kernel(const BatchMol* __
                                                               const T* __restrict__ part_v,
                                                               const double* __restrict__ part_e, int64_t part_v_elems,
                                                               int64_t part_e_elems, T* __restrict__ vxc,
-                                                              double* __restrict__ exc) {
+                                                              double* __restrict__ exc, int nsh_max) {
   const int m = order[blockIdx.x];
   if (ctl != nullptr && ctl[m * CTL + 1] == 0.0) return;
   const BatchMol Mo = mols[m];
   const int N = (int)Mo.nao;
-  if (N > max_nao || (int64_t)4 * N * Mo.ngrid >= ((int64_t)1 << 31)) return;  // flagged by xc_batch_kernel
+  if (xc_mol_outside(Mo, max_nao, nsh_max)) return;  // flagged by xc_batch_kernel
   const int nchunk = (int)((Mo.ngrid + XC_CHUNK_PTS - 1) / XC_CHUNK_PTS);
   if (nchunk <= 1) return;  // written by xc_batch_kernel
   if (!xc_parts_fit(nchunk, gridDim.x, max_nao, part_v_elems, part_e_elems)) {
@@ -1179,33 +1451,52 @@ extern "C" int pscf_batch_ao(int dtype, const void* d_mols, const int* d_ptmol, 
   return PSCF_OK;
 }
 
+// LDS of the batched kernel: the tile pipeline (or, after the tiles, V and
+// the energy partials), then in the direct form the shell table.
+template <typename T, int XT, int NP>
+static size_t xc_batch_lds(int max_nao, const XcDirect* da) {
+  const size_t base = (xc_lds<T, XT, NP>(max_nao) + 31) & ~(size_t)31;
+  return da ? base + xc_shell_bytes(da->nsh_max, da->kp, sizeof(T)) : base;
+}
+
 template <typename T, int XT, int NP>
 static int batch_xc_launch(const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
-                           const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc,
-                           double* d_exc, int split, void* d_part_v, double* d_part_e, int64_t part_v_elems,
-                           int64_t part_e_elems, hipStream_t st) {
-  const size_t lds = xc_lds<T, XT, NP>(max_nao);
+                           const void* d_ao, const XcDirect* da, const void* d_dm, const void* d_w, double eps,
+                           void* d_vxc, double* d_exc, int split, void* d_part_v, double* d_part_e,
+                           int64_t part_v_elems, int64_t part_e_elems, hipStream_t st) {
+  const size_t lds = xc_batch_lds<T, XT, NP>(max_nao, da);
   PSCF_REQUIRE((int)lds <= xc_max_lds(), "pscf_batch_xc: nao %d needs %zu B LDS", max_nao, lds);
-  PSCF_SET_LDS((xc_batch_kernel<T, XT, NP>), lds);
-  xc_batch_kernel<T, XT, NP><<<dim3(nmol, split), XC_THREADS, lds, st>>>(
-      (const BatchMol*)d_mols, d_order, max_nao, d_ctl, (const T*)d_ao, (const T*)d_dm, (const T*)d_w, (T)eps,
-      (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e, part_v_elems, part_e_elems);
+  const dim3 grid(nmol, split);
+  if (da) {
+    XcDirect a = *da;
+    a.tab_off = (int)((xc_lds<T, XT, NP>(max_nao) + 31) & ~(size_t)31);
+    PSCF_SET_LDS((xc_batch_kernel<T, XT, NP, true>), lds);
+    xc_batch_kernel<T, XT, NP, true><<<grid, XC_THREADS, lds, st>>>(
+        (const BatchMol*)d_mols, d_order, max_nao, d_ctl, nullptr, a, (const T*)d_dm, (const T*)d_w, (T)eps,
+        (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e, part_v_elems, part_e_elems);
+  } else {
+    PSCF_SET_LDS((xc_batch_kernel<T, XT, NP, false>), lds);
+    xc_batch_kernel<T, XT, NP, false><<<grid, XC_THREADS, lds, st>>>(
+        (const BatchMol*)d_mols, d_order, max_nao, d_ctl, (const T*)d_ao, XcDirect{}, (const T*)d_dm,
+        (const T*)d_w, (T)eps, (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e, part_v_elems, part_e_elems);
+  }
   PSCF_HIP(hipGetLastError());
   xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, max_nao, d_ctl,
                                                  (const T*)d_part_v, d_part_e, part_v_elems, part_e_elems,
-                                                 (T*)d_vxc, d_exc);
+                                                 (T*)d_vxc, d_exc, da ? da->nsh_max : 0);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
 
 template <typename T, int XT>
 static int batch_xc_np(int np, const void* d_mols, const int* d_order, int nmol, int max_nao, const double* d_ctl,
-                       const void* d_ao, const void* d_dm, const void* d_w, double eps, void* d_vxc, double* d_exc,
-                       int split, void* d_part_v, double* d_part_e, int64_t pv, int64_t pe, hipStream_t st) {
-#define PSCF_XC_NP(V)                                                                                              \
-  case V:                                                                                                          \
-    return batch_xc_launch<T, XT, V>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, \
-                                     split, d_part_v, d_part_e, pv, pe, st);
+                       const void* d_ao, const XcDirect* da, const void* d_dm, const void* d_w, double eps,
+                       void* d_vxc, double* d_exc, int split, void* d_part_v, double* d_part_e, int64_t pv,
+                       int64_t pe, hipStream_t st) {
+#define PSCF_XC_NP(V)                                                                                           \
+  case V:                                                                                                       \
+    return batch_xc_launch<T, XT, V>(d_mols, d_order, nmol, max_nao, d_ctl, d_ao, da, d_dm, d_w, eps, d_vxc,  \
+                                     d_exc, split, d_part_v, d_part_e, pv, pe, st);
   switch (np) {
     PSCF_XC_NP(16)
     PSCF_XC_NP(32)
@@ -1218,12 +1509,12 @@ static int batch_xc_np(int np, const void* d_mols, const int* d_order, int nmol,
   return PSCF_EINVAL;
 }
 
-extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
-                             const double* d_ctl, const void* d_ao, const void* d_dm, const void* d_w, double eps,
-                             void* d_vxc, double* d_exc, int split, int tile, void* d_part_v, double* d_part_e,
-                             int64_t part_v_elems, int64_t part_e_elems, void* stream) {
+static int batch_xc_any(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                        const double* d_ctl, const void* d_ao, const XcDirect* da, const void* d_dm, const void* d_w,
+                        double eps, void* d_vxc, double* d_exc, int split, int tile, void* d_part_v,
+                        double* d_part_e, int64_t part_v_elems, int64_t part_e_elems, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_xc: bad dtype");
-  PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && d_ao && d_dm && d_w && d_vxc && d_exc,
+  PSCF_REQUIRE(nmol > 0 && max_nao > 0 && d_mols && d_order && (d_ao || da) && d_dm && d_w && d_vxc && d_exc,
                "pscf_batch_xc: bad arguments");
   PSCF_REQUIRE(split >= 1 && split <= 64, "pscf_batch_xc: bad split %d", split);
   PSCF_REQUIRE(d_part_v && d_part_e && part_v_elems >= 0 && part_e_elems >= 0, "pscf_batch_xc: no chunk buffers");
@@ -1232,13 +1523,36 @@ extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, 
   hipStream_t st = (hipStream_t)stream;
   const int np = xm_np(max_nao);
   if (dtype == PSCF_F64)
-    return batch_xc_np<double, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                   split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
+    return batch_xc_np<double, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, da, d_dm, d_w, eps, d_vxc,
+                                   d_exc, split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
   if (tile == 64)
-    return batch_xc_np<float, 64>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc,
-                                  split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
-  return batch_xc_np<float, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, d_dm, d_w, eps, d_vxc, d_exc, split,
-                                d_part_v, d_part_e, part_v_elems, part_e_elems, st);
+    return batch_xc_np<float, 64>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, da, d_dm, d_w, eps, d_vxc,
+                                  d_exc, split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
+  return batch_xc_np<float, 32>(np, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, da, d_dm, d_w, eps, d_vxc, d_exc,
+                                split, d_part_v, d_part_e, part_v_elems, part_e_elems, st);
+}
+
+extern "C" int pscf_batch_xc(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                             const double* d_ctl, const void* d_ao, const void* d_dm, const void* d_w, double eps,
+                             void* d_vxc, double* d_exc, int split, int tile, void* d_part_v, double* d_part_e,
+                             int64_t part_v_elems, int64_t part_e_elems, void* stream) {
+  PSCF_REQUIRE(d_ao, "pscf_batch_xc: no AO block");
+  return batch_xc_any(dtype, d_mols, d_order, nmol, max_nao, d_ctl, d_ao, nullptr, d_dm, d_w, eps, d_vxc, d_exc,
+                      split, tile, d_part_v, d_part_e, part_v_elems, part_e_elems, stream);
+}
+
+extern "C" int pscf_batch_xc_direct(int dtype, const void* d_mols, const int* d_order, int nmol, int max_nao,
+                                    const double* d_ctl, const double* d_coords, const int* d_atm, const int* d_bas,
+                                    const double* d_env, int nsh_max, int kp, const void* d_dm, const void* d_w,
+                                    double eps, void* d_vxc, double* d_exc, int split, int tile, void* d_part_v,
+                                    double* d_part_e, int64_t part_v_elems, int64_t part_e_elems, void* stream) {
+  PSCF_REQUIRE(d_coords && d_atm && d_bas && d_env, "pscf_batch_xc_direct: null pointer");
+  PSCF_REQUIRE(nsh_max >= 1 && nsh_max <= XM_MAXN && kp >= 1 && kp <= 16,
+               "pscf_batch_xc_direct: nsh_max %d (1..%d) or kp %d (1..16) out of range", nsh_max, XM_MAXN, kp);
+  // at least three primitive slots per record: the stage reads the first three unconditionally
+  XcDirect da{d_coords, d_atm, d_bas, d_env, nsh_max, kp < 3 ? 3 : kp, 0};
+  return batch_xc_any(dtype, d_mols, d_order, nmol, max_nao, d_ctl, nullptr, &da, d_dm, d_w, eps, d_vxc, d_exc,
+                      split, tile, d_part_v, d_part_e, part_v_elems, part_e_elems, stream);
 }
 
 extern "C" size_t pscf_grad_xc_workspace_bytes(int nao, int nbas, int g) {

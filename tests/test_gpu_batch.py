@@ -128,6 +128,38 @@ def test_batch_split_and_tuner_fp64(hip):
     assert np.array_equal(got.iterations, ref.iterations)
 
 
+@pytest.mark.parametrize("dtype,basis,tiles,rtol", [("fp64", "sto-3g", (32,), 1e-12), ("fp32", "sto-3g", (32, 64), 2e-5),
+                                                    ("fp64", "6-31g", (32,), 1e-12)])
+def test_batch_xc_direct_equals_stored(hip, dtype, basis, tiles, rtol):
+    """XC with the AO values evaluated inside the kernel (shell table in LDS,
+    pscf_batch_xc_direct) equals XC over the AO block stored by
+    pscf_batch_ao, on the atomic-guess density: molecules with 1-9 heavy atoms
+    (nao classes 16-80; FP64, whose shell table fits the LDS up to nao 64, on 1-7
+    heavy atoms) in STO-3G, and 6-31G molecules (six-primitive shells)."""
+    import torch
+    from pyscf_ipu_amd import device
+    heavy = (5, 9) if dtype == "fp32" else (5, 7)
+    strs = STRS + molgen.batched_set(24, heavy, seed=33) if basis == "sto-3g" else STRS[:4]
+    mols = batch.parse(strs)
+    got = {}
+    for direct in (True, False):
+        b = batch.BatchRKS(mols, basis=basis, dtype=dtype)
+        b.ao_direct = direct
+        b.setup()
+        for tile in tiles:
+            for split in (1, 2):
+                print(f"xc direct={direct} {dtype} {basis} tile {tile} split {split}", flush=True)
+                b._xc(device.ptr(b.ctl), split, tile)
+                torch.cuda.synchronize()
+                got[direct, tile, split] = (b.Vxc.double().cpu().numpy(), b.exc.cpu().numpy())
+    for (direct, tile, split), (V, e) in got.items():
+        V0, e0 = got[False, tiles[0], 1]
+        assert np.isfinite(V).all() and np.isfinite(e).all()
+        assert np.abs(V - V0).max() <= rtol * np.abs(V0).max(), (direct, tile, split)
+        assert np.abs(e - e0).max() <= rtol * np.abs(e0).max(), (direct, tile, split)
+    assert np.abs(got[True, tiles[0], 1][0]).max() > 0.1
+
+
 def test_batch_fp32_tiles_agree(hip):
     """FP32 XC with 64-point tiles and split workgroups equals 32-point tiles."""
     mols = batch.parse(STRS)

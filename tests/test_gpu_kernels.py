@@ -204,10 +204,32 @@ def test_becke_weights(hip, mol):
     assert np.abs(_np(w) - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
 
 
-# FP32: inputs are rounded to 6e-8; the rho^(-5/3), rho^(-4/3) terms and the cancellation
-# between exchange and correlation amplify that to a few 1e-5 relative at worst.
-@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-11), ("fp32", 1e-4)])
-def test_b3lyp_dropin(hip, dtype, tol):
+def _b3lyp_part_scales(rho, eps):
+    """Per point, the sum of the magnitudes of the four parts (Slater, VWN,
+    B88, LYP) of each B3LYP output, from the oracle's parts: the scale that
+    the rounding errors of evaluating the parts are relative to.  The parts
+    can cancel (the gradient terms of B88 and LYP at moderate density), which
+    makes the total arbitrarily smaller than its parts."""
+    import torch
+    r = torch.as_tensor(rho)
+    r0 = torch.clamp(r[0], oracle_xc.CLIP_RHO_MIN, oracle_xc.CLIP_RHO_MAX).requires_grad_(True)
+    norms = (torch.sum((r[1:4] * 2) ** 2, dim=0) + eps).requires_grad_(True)
+    parts = [oracle_xc._lda(r0) * 0.08 * r0, oracle_xc._vwn(r0) * 0.19, oracle_xc._b88(r0, norms) * 0.72,
+             oracle_xc._lyp(r0, norms) * 0.81]
+    se, sv, sn = 0.0, 0.0, 0.0
+    for e in parts:
+        dr, dn = torch.autograd.grad(e.sum(), (r0, norms), retain_graph=True, allow_unused=True)
+        se = se + (e / r0).abs()
+        sv = sv + (dr.abs() if dr is not None else 0.0)
+        sn = sn + (dn.abs() if dn is not None else 0.0)
+    return [x.detach().numpy() for x in (se, sv, sn)]
+
+
+# FP32: inputs are rounded to 6e-8 and the elementary functions are the
+# hardware's (about 1 ulp), so each part carries a few 1e-7 of its own size;
+# where the parts cancel, that is what the total's error is relative to.
+@pytest.mark.parametrize("dtype,tol,tol_parts", [("fp64", 1e-11, 1e-13), ("fp32", None, 1e-5)])
+def test_b3lyp_dropin(hip, dtype, tol, tol_parts):
     from pyscf_ipu_amd.exchange_correlation.b3lyp import b3lyp
     rng = np.random.default_rng(3)
     g = 4096
@@ -216,9 +238,13 @@ def test_b3lyp_dropin(hip, dtype, tol):
     rho[1:4] = rng.normal(size=(3, g)) * rho[0] * 10 ** rng.uniform(-2, 0.5, g)
     npdt = np.float64 if dtype == "fp64" else np.float32
     got = b3lyp(rho.astype(npdt), 1e-20)
-    ref = oracle_xc.b3lyp(rho.astype(npdt).astype(np.float64), 1e-20)
-    for a, b in zip(got, ref):
-        assert np.max(np.abs(a - b) / (np.abs(b) + 1e-30)) < tol
+    r64 = rho.astype(npdt).astype(np.float64)
+    ref = oracle_xc.b3lyp(r64, 1e-20)
+    scales = _b3lyp_part_scales(r64, 1e-20)
+    for a, b, sc in zip(got, ref, scales):
+        if tol is not None:
+            assert np.max(np.abs(a - b) / (np.abs(b) + 1e-30)) < tol
+        assert np.max(np.abs(a - b) / (sc + 1e-30)) < tol_parts
 
 
 @pytest.mark.parametrize("dtype,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
