@@ -259,11 +259,13 @@ def _batched(mol_strs, workload: str, steps: int, warmup: int, its: int, golden=
         jk_ms = ms["jk"][f"threads{tun['chosen_jk_threads']}/split{tun['chosen_split']['jk']}"]
         xc_ms = ms["xc"][f"tile{tun['chosen_tile']}/split{tun['chosen_split']['xc']}"]
         jk_b = 4.0 * sizes_b.sizes["eri"]
-        xc_b = 4.0 * (sizes_b.sizes["ao"] + sizes_b.sizes["grid"])
+        nao_b, ng_b = sizes_b.desc[:, 2].astype(np.float64), sizes_b.desc[:, 9].astype(np.float64)
+        xc_fl = float(np.sum(4.0 * nao_b * nao_b * ng_b))
         rl["jk_s8"] = _roof("hbm", jk_b / (jk_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", jk_ms,
                             "4 B per unique s8 ERI of all molecules")
-        rl["xc"] = _roof("hbm", xc_b / (xc_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", xc_ms,
-                         "4 B per AO value/gradient (4 g N) + weight")
+        rl["xc"] = _roof("mfma", xc_fl / (xc_ms * 1e-3) / 1e12, PEAK_TFLOPS["fp32"], "TFLOP/s", xc_ms,
+                         "FP32 MFMA work 4 N^2 flops per grid point (X = phi0 D and V += phi0^T Y); the AO "
+                         "values are evaluated in the kernel, so no AO stream crosses HBM")
     sm = g.setup_ms or {}
     if "eri_s8" in sm:
         fl = roofline.batch_eri_flops(sizes_b)

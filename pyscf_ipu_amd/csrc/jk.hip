@@ -404,7 +404,6 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
       q = __builtin_amdgcn_readfirstlane(q);
       const T* row = eri + tri(pq);
       const T* gp = dm_g + p * N;
-      const T* gq = dm_g + q * N;
       const T cj = T(2) * gp[q];
       const T dp_s = lane < N ? dm[p * N + sl] : T(0);
       const T dq_s = lane < N ? dm[q * N + sl] : T(0);
@@ -413,12 +412,15 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
       // the loads of group g + 8 are issued before group g is used (software
       // pipeline); FP64 has no registers to spare for it.
       constexpr bool PIPE = sizeof(T) == 4;
+      // Loads are unconditional (a lane outside the row reads the row's first
+      // element, zeroed where it is used): no branch per load, and the
+      // compiler counts the loads in flight exactly.
+      auto lim_of = [&](int r) { return r < p ? r : (r == p ? q : -1); };  // last valid lane of r (uniform)
       auto load_group = [&](int g0, T (&dst)[JKC_GROUP]) {
 #pragma unroll
         for (int u = 0; u < JKC_GROUP; ++u) {
           const int r = g0 + u;
-          const bool valid = r <= p && lane <= (r == p ? q : r);
-          dst[u] = valid ? row[r * (r + 1) / 2 + lane] : T(0);
+          dst[u] = row[lane <= lim_of(r) ? r * (r + 1) / 2 + lane : 0];
         }
       };
       T vn[JKC_GROUP];
@@ -436,16 +438,17 @@ __global__ void __launch_bounds__(JKB_MAX_THREADS) jk_s8_cols_batch_kernel(const
         }
         // t[u] = w dm[q][s] (to K(p, g+u)), t[8+u] = w dm[p][s] (to K(q, g+u))
         T t[16];
+        // r > p only in a row's last group, where w = 0: no branch per r.  dm[p][r] and dm[q][r] come from the lanes holding
+        // rows p and q of dm (r is a constant of the unrolled loop).
 #pragma unroll
         for (int u = 0; u < JKC_GROUP; ++u) {
           const int r = g + u;
-          t[u] = t[JKC_GROUP + u] = T(0);
-          if (r > p) continue;  // uniform
-          const T w = v[u];
+          const T w = lane <= lim_of(r) ? v[u] : T(0);
           jacc[r] += w * cj;
-          jrow += w * dm[r * N + (lane <= r ? lane : r)];
-          kp += w * gq[r];
-          kq += w * gp[r];
+          const int rr = r < N ? r : N - 1;  // rows past N: w = 0, any in-range read
+          jrow += w * dm[rr * N + (lane <= rr ? lane : rr)];
+          kp += w * lane_read(dq_s, r);
+          kq += w * lane_read(dp_s, r);
           t[u] = w * dq_s;
           t[JKC_GROUP + u] = w * dp_s;
         }
