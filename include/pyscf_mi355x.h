@@ -340,6 +340,18 @@ int pscf_eri_s8_to_s1(int dtype_in, const void* d_s8, int dtype_out, void* d_s1,
 /* dJK = J - hyb/2 K (FP64, N x N) from the s1 tensor and an FP64 density
  * (the einsum definitions at nanoDFT.py:130-132), one pass over the tensor. */
 int pscf_jk_s1(int dtype, const void* d_s1, int N, const double* d_dm, double hyb, double* d_out, void* stream);
+/* dJK = J - hyb/2 K (FP64, N x N) from the packed s8 ERIs (raw values, as
+ * pscf_eri_s8 writes them; sparse_symmetric_ERI.py:86-137) and an FP64
+ * density, for nao beyond the LDS-resident pscf_jk_s8 (which holds dm and an
+ * N x N accumulator in LDS: nao <= 116 in FP32, 101 in FP64): only rows of dm
+ * and of the accumulator sit in LDS, the accumulator itself in the workspace
+ * (pscf_jk_s8_large_workspace_bytes(N): N^2 + N(N+1)/2 doubles).  Reads the
+ * s8 array twice instead of streaming the N^4 s1 tensor (pscf_jk_s1).
+ * Results repeat bit for bit (contributions rounded to multiples of 2^-40
+ * before their atomic sums).  N <= 1280. */
+size_t pscf_jk_s8_large_workspace_bytes(int N);
+int pscf_jk_s8_large(int dtype, const void* d_eri_s8, int N, const double* d_dm, double hyb, double* d_out,
+                     void* d_ws, size_t ws_bytes, void* stream);
 /* General GEMM C (FP64) = op(A) op(B) on MFMA: inputs stored as in_type
  * (0 BF16, 1 FP16, 2 FP32, 3 FP64), accumulation FP64 (acc64) or FP32;
  * row-major with leading dimensions; atomic = add into C (K split over

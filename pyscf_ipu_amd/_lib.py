@@ -90,6 +90,9 @@ _SIGS = {
     # large molecules and the precision study
     "pscf_eri_s8_to_s1": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "pscf_jk_s1": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p]),
+    "pscf_jk_s8_large_workspace_bytes": (c_size_t, [c_int]),
+    "pscf_jk_s8_large": (c_int, [c_int, c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_size_t,
+                                 c_void_p]),
     "pscf_gemm": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                           c_void_p, c_int64, c_int, c_void_p]),
     "pscf_cast_to": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_void_p]),

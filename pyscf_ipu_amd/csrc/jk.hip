@@ -799,6 +799,191 @@ extern "C" int pscf_jk_blocks(int dtype, const void* d_eri, const int* d_blocks,
                                (float*)d_vk, d_ws, ws_bytes, st);
 }
 
+// ------------------------------------------------- s8 for any nao ----
+// dJK = J - (hyb/2) K from the packed s8 ERIs (sparse_symmetric_ERI.py:86-137)
+// for nao beyond what the LDS-resident kernels hold (dm and an N x N
+// accumulator in LDS stop at nao 116 in FP32, 101 in FP64).  Only rows of dm
+// and of the accumulator are in LDS; M = J/2 - (hyb/4) K accumulates in
+// global memory (FP64), then dJK = M + M^T.  Every contribution added to M
+// is rounded to a multiple of 2^-40 (jk_quant), so the atomic sums from
+// different waves are exact and repeat bit for bit.
+//   rows pass: one wave per row pq = (p, q) of the packed lower triangle,
+//     lanes walking rs <= pq (as s8_row): targets (p, q) (the row's dot with
+//     D), (p, s) and (q, s) (LDS atomics into the wave's copies of rows p and
+//     q of M), (p, r) and (q, r) (segmented sums over runs of equal r);
+//     rows p and q of D sit in the wave's LDS, D[r][s] comes from a packed
+//     copy of D (index rs, coalesced).  After the row, the two row
+//     accumulators and the dot go to M with global atomics.
+//   columns pass: the remaining J target (r, s) += 2 w D[p][q], one wave per
+//     (block of 64 rs, block of rows pq): lane rs sums over the rows in a
+//     register, then adds to M once.
+// w = ERI / multiplicity 2^([p==q] + [r==s] + [pq==rs]) as in s8_row.
+constexpr int JK8L_WAVES = 4;      // waves per workgroup
+constexpr int JK8L_ROWS = 128;     // rows pq per item of the columns pass
+
+__global__ void __launch_bounds__(256) jk8l_prep_kernel(const double* __restrict__ dm, int N, double* __restrict__ M,
+                                                        double* __restrict__ dpk) {
+  const int64_t NN = (int64_t)N * N, P = tri(N);
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < NN + P; x += (int64_t)gridDim.x * blockDim.x) {
+    if (x < NN) {
+      M[x] = 0.0;
+    } else {
+      int r, s;
+      tri_split((int)(x - NN), r, s);
+      dpk[x - NN] = dm[r * N + s];
+    }
+  }
+}
+
+__device__ __forceinline__ void jk8l_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * JK8L_WAVES) jk8l_rows_kernel(const T* __restrict__ eri, int N,
+                                                                    const double* __restrict__ dm,
+                                                                    const double* __restrict__ dpk, double hhalf,
+                                                                    double* __restrict__ M) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  double* dp = reinterpret_cast<double*>(smem) + (size_t)4 * N * wl;  // row p of D
+  double* dq = dp + N;                                                  // row q of D
+  double* ap = dq + N;                                                  // row p of M (this row's part)
+  double* aq = ap + N;                                                  // row q of M
+  const int P = (int)tri(N);
+  const int nw = gridDim.x * JK8L_WAVES;
+  // longest rows first: row pq = P - 1 - item
+  for (int item = __builtin_amdgcn_readfirstlane(blockIdx.x * JK8L_WAVES + wl); item < P; item += nw) {
+    const int pq = P - 1 - item;
+    int p, q;
+    tri_split(pq, p, q);
+    p = __builtin_amdgcn_readfirstlane(p);
+    q = __builtin_amdgcn_readfirstlane(q);
+    for (int i = lane; i < N; i += 64) {
+      dp[i] = dm[(int64_t)p * N + i];
+      dq[i] = dm[(int64_t)q * N + i];
+      ap[i] = 0.0;
+      aq[i] = 0.0;
+    }
+    jk8l_wave_sync();
+    const T* row = eri + tri(pq);
+    const double fpq = p == q ? 0.5 : 1.0;
+    double jrow = 0.0;
+    for (int base = 0; base <= pq; base += 64) {
+      const int rs = base + lane;
+      const bool valid = rs <= pq;
+      const int rsc = valid ? rs : pq;
+      int r, s;
+      tri_split(rsc, r, s);
+      const double v = valid ? (double)row[rs] : 0.0;
+      double w = v * fpq;
+      if (r == s) w *= 0.5;
+      if (rs == pq) w *= 0.5;
+      jrow += 2.0 * w * dpk[rsc];
+      const double wk = -hhalf * w;
+      if (valid) {
+        atomicAdd(ap + s, wk * dq[r]);
+        atomicAdd(aq + s, wk * dp[r]);
+      }
+      double a = wk * dq[s], b = wk * dp[s];
+      seg_scan2(a, b, valid ? r : -1);
+      if (valid && (s == r || lane == 63 || rs == pq)) {
+        atomicAdd(ap + r, a);
+        atomicAdd(aq + r, b);
+      }
+    }
+    jk8l_wave_sync();
+    for (int i = lane; i < N; i += 64) {
+      atomicAdd(M + (int64_t)p * N + i, jk_quant(ap[i]));
+      atomicAdd(M + (int64_t)q * N + i, jk_quant(aq[i]));
+    }
+    jrow = wave_sum_last(jrow);
+    if (lane == 63) atomicAdd(M + (int64_t)p * N + q, jk_quant(jrow));
+    jk8l_wave_sync();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) jk8l_cols_kernel(const T* __restrict__ eri, int N,
+                                                        const double* __restrict__ dpk, double* __restrict__ M) {
+  const int lane = threadIdx.x & 63;
+  const int P = (int)tri(N);
+  const int ncb = (P + 63) / 64, nrb = (P + JK8L_ROWS - 1) / JK8L_ROWS;
+  const int64_t items = (int64_t)ncb * nrb;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); it < items;
+       it += nw) {
+    const int cb = (int)(it / nrb), rb = (int)(it - (int64_t)cb * nrb);
+    const int c0 = cb * 64;
+    const int row0 = max(rb * JK8L_ROWS, c0), row1 = min((rb + 1) * JK8L_ROWS, P);
+    if (row0 >= row1) continue;  // rows before the block's first column hold none of its elements
+    const int rs = c0 + lane;
+    int r, s;
+    tri_split(rs < P ? rs : P - 1, r, s);
+    const double frs = r == s ? 0.5 : 1.0;
+    double jt = 0.0;
+    for (int pq = row0; pq < row1; ++pq) {
+      int p, q;
+      tri_split(pq, p, q);
+      const double v = (rs <= pq) ? (double)eri[tri(pq) + rs] : 0.0;
+      double w = v * frs * (p == q ? 0.5 : 1.0);
+      if (rs == pq) w *= 0.5;
+      jt += 2.0 * w * dpk[pq];
+    }
+    if (rs < P) atomicAdd(M + (int64_t)r * N + s, jk_quant(jt));
+  }
+}
+
+__global__ void __launch_bounds__(256) jk8l_sym_kernel(const double* __restrict__ M, int N, double* __restrict__ out) {
+  const int64_t NN = (int64_t)N * N;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < NN; x += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(x / N), j = (int)(x - (int64_t)i * N);
+    out[x] = M[x] + M[(int64_t)j * N + i];
+  }
+}
+
+extern "C" size_t pscf_jk_s8_large_workspace_bytes(int N) {
+  return sizeof(double) * ((size_t)N * N + (size_t)tri(N));
+}
+
+extern "C" int pscf_jk_s8_large(int dtype, const void* d_eri_s8, int N, const double* d_dm, double hyb,
+                                double* d_out, void* d_ws, size_t ws_bytes, void* stream) {
+  PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_jk_s8_large: bad dtype %d", dtype);
+  PSCF_REQUIRE(N > 0 && N <= 4096 && d_eri_s8 && d_dm && d_out, "pscf_jk_s8_large: bad arguments");
+  PSCF_REQUIRE(d_ws && ws_bytes >= pscf_jk_s8_large_workspace_bytes(N), "pscf_jk_s8_large: workspace too small");
+  const size_t lds = sizeof(double) * 4 * (size_t)N * JK8L_WAVES;
+  PSCF_REQUIRE((int)lds <= max_lds_bytes(), "pscf_jk_s8_large: nao %d too large for the row buffers", N);
+  hipStream_t st = (hipStream_t)stream;
+  double* M = (double*)d_ws;
+  double* dpk = M + (size_t)N * N;
+  const int64_t P = tri(N);
+  const int64_t prep = ((int64_t)N * N + P + 255) / 256;
+  jk8l_prep_kernel<<<(unsigned)(prep < 65536 ? prep : 65536), 256, 0, st>>>(d_dm, N, M, dpk);
+  PSCF_HIP(hipGetLastError());
+  const int64_t rows_wg = (P + JK8L_WAVES - 1) / JK8L_WAVES;
+  const unsigned g1 = (unsigned)(rows_wg < 4096 ? rows_wg : 4096);
+  const int64_t items = ((P + 63) / 64) * ((P + JK8L_ROWS - 1) / JK8L_ROWS);
+  const unsigned g2 = (unsigned)((items + 3) / 4 < 8192 ? (items + 3) / 4 : 8192);
+  if (dtype == PSCF_F64) {
+    PSCF_SET_LDS(jk8l_rows_kernel<double>, lds);
+    jk8l_rows_kernel<double><<<g1, 64 * JK8L_WAVES, lds, st>>>((const double*)d_eri_s8, N, d_dm, dpk, hyb / 2.0, M);
+    PSCF_HIP(hipGetLastError());
+    jk8l_cols_kernel<double><<<g2, 256, 0, st>>>((const double*)d_eri_s8, N, dpk, M);
+  } else {
+    PSCF_SET_LDS(jk8l_rows_kernel<float>, lds);
+    jk8l_rows_kernel<float><<<g1, 64 * JK8L_WAVES, lds, st>>>((const float*)d_eri_s8, N, d_dm, dpk, hyb / 2.0, M);
+    PSCF_HIP(hipGetLastError());
+    jk8l_cols_kernel<float><<<g2, 256, 0, st>>>((const float*)d_eri_s8, N, dpk, M);
+  }
+  PSCF_HIP(hipGetLastError());
+  const int64_t nb = ((int64_t)N * N + 255) / 256;
+  jk8l_sym_kernel<<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0, st>>>(M, N, d_out);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
 extern "C" size_t pscf_jk_s8_workspace_bytes(int dtype, int N) {
   return jk_s8_ws_bytes(dtype == PSCF_F64 ? 8 : 4, N);
 }

@@ -5,9 +5,10 @@ products.
 energy, pyscf_ipu/nanoDFT/nanoDFT.py:40-102, 338-400) with data paths sized
 for large nao:
 
-  * ERIs: packed s8 from `pscf_eri_s8` (FP32 or FP64), expanded once to the
-    full s1 tensor; J/K in one streaming pass over it (`pscf_jk_s1`), FP64
-    accumulation;
+  * ERIs: packed s8 from `pscf_eri_s8` (FP32 or FP64), kept packed; J/K
+    from them with rows of dm and of the accumulator in LDS
+    (`pscf_jk_s8_large`, FP64 accumulation), 8x fewer bytes per iteration
+    than streaming the N^4 s1 tensor (`pscf_jk_s1`, still in the library);
   * XC: the AO values (4, N, g) are stored in the study's input precision and
     the two AO-grid products X = D phi0 and V = phi0 Y^T run on `pscf_gemm`
     (BF16 / FP16 / FP32 / FP64 inputs, FP32 or FP64 accumulation); the
@@ -92,13 +93,11 @@ class LargeRKS(RKS):
                   device.ptr(self.coords), G, device.ptr(ao64), device.stream())
         self.phi = self._cast(ao64)
         del ao64
-        # ERIs: packed s8, expanded to s1
-        code = _lib.F64 if self.eri_dtype == "fp64" else _lib.F32
-        s8 = direct.eri_s8(mol, self.eri_dtype, dev_mol=self.dev_mol)
-        self.s1 = device.empty(N ** 4, torch.float64 if code == _lib.F64 else torch.float32)
-        _lib.call("pscf_eri_s8_to_s1", code, device.ptr(s8), code, device.ptr(self.s1), N, device.stream())
-        self.eri_code = code
-        del s8
+        # ERIs: packed s8 (J/K reads them packed)
+        self.eri_code = _lib.F64 if self.eri_dtype == "fp64" else _lib.F32
+        self.s8 = direct.eri_s8(mol, self.eri_dtype, dev_mol=self.dev_mol)
+        self.jk_ws_b = _lib.lib().pscf_jk_s8_large_workspace_bytes(N)
+        self.jk_ws = device.workspace(self.jk_ws_b)
         # buffers
         self.D = device.to_dev(guess.minao(mol, S_h) if dm0 is None else dm0, f64)
         self.Dg = device.empty((N, N), self.gemm_tdt)
@@ -143,7 +142,8 @@ class LargeRKS(RKS):
         _lib.call("pscf_gemm", self.gemm_type, self.acc64, 0, 1, 1, N, N, G, P(self.phi), G, P(self.Y), G,
                   P(self.Vh), N, ksplit, st)
         _lib.call("pscf_xc_large_finish", N, P(self.Vh), P(self.e_part), self.nparts, P(self.Vxc), P(self.exc), st)
-        _lib.call("pscf_jk_s1", self.eri_code, P(self.s1), N, P(self.D), HYB_B3LYP, P(self.dJK), st)
+        _lib.call("pscf_jk_s8_large", self.eri_code, P(self.s8), N, P(self.D), HYB_B3LYP, P(self.dJK), P(self.jk_ws),
+                  self.jk_ws_b, st)
 
     def energy_terms(self):
         _lib.call("pscf_energy", _lib.F64, self.N, device.ptr(self.D), device.ptr(self.Hcore), device.ptr(self.dJK),

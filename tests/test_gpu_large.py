@@ -78,6 +78,62 @@ def test_s8_to_s1_and_jk_s1(hip):
     np.testing.assert_allclose(out.cpu().numpy(), J - 0.1 * K, atol=1e-11)
 
 
+def _jk_s8_large(s8, code, N, D):
+    ws_b = _lib.lib().pscf_jk_s8_large_workspace_bytes(N)
+    ws = device.workspace(ws_b)
+    out = device.empty((N, N), torch.float64)
+    _lib.call("pscf_jk_s8_large", code, device.ptr(s8), N, device.ptr(device.to_dev(D)), 0.2, device.ptr(out),
+              device.ptr(ws), ws_b, device.stream())
+    return out.cpu().numpy()
+
+
+def _sym_density(N, seed):
+    D = np.random.default_rng(seed).normal(size=(N, N))
+    return D + D.T
+
+
+@pytest.mark.parametrize("eri", ["fp64", "fp32"])
+def test_jk_s8_large_matches_oracle_631g(hip, eri):
+    """J - hyb/2 K from the packed s8 ERIs with row-tiled dm and accumulator
+    equals the einsum definitions over the oracle's s1 expansion of the same
+    ERIs (a 5-heavy-atom molecule in 6-31G, nao about 60)."""
+    mol = build_mol(molgen.plain_set(1, 5, seed=7)[0], "6-31g")
+    N = mol.nao_nr()
+    print(f"jk_s8_large {eri} nao {N}", flush=True)
+    s8 = direct.eri_s8(mol, eri)
+    code = _lib.F64 if eri == "fp64" else _lib.F32
+    ref_s1 = cint.s8_to_s1(s8.double().cpu().numpy(), N)
+    D = _sym_density(N, 1)
+    got = _jk_s8_large(s8, code, N, D)
+    J = np.einsum("ijkl,ji->kl", ref_s1, D)
+    K = np.einsum("ijkl,jk->il", ref_s1, D)
+    ref = J - 0.1 * K
+    # FP64 accumulation of the same ERI values; the 2^-40 rounding of each
+    # contribution bounds the difference
+    assert np.abs(got - ref).max() < 1e-9 * max(1.0, np.abs(ref).max()), np.abs(got - ref).max()
+    np.testing.assert_array_equal(got, _jk_s8_large(s8, code, N, D))
+
+
+def test_jk_s8_large_equals_s1_beyond_lds_nao(hip):
+    """At nao above the LDS-resident s8 kernels' limit (a 12-heavy-atom
+    molecule in 6-31G) the packed form equals the s1 stream on the GPU."""
+    mol = build_mol(molgen.plain_set(1, 12, seed=5)[0], "6-31g")
+    N = mol.nao_nr()
+    assert N > 116
+    print(f"jk_s8_large vs s1 nao {N}", flush=True)
+    s8 = direct.eri_s8(mol, "fp64")
+    D = _sym_density(N, 2)
+    got = _jk_s8_large(s8, _lib.F64, N, D)
+    s1 = device.empty(N ** 4, torch.float64)
+    _lib.call("pscf_eri_s8_to_s1", _lib.F64, device.ptr(s8), _lib.F64, device.ptr(s1), N, device.stream())
+    out = device.empty((N, N), torch.float64)
+    _lib.call("pscf_jk_s1", _lib.F64, device.ptr(s1), N, device.ptr(device.to_dev(D)), 0.2, device.ptr(out),
+              device.stream())
+    ref = out.cpu().numpy()
+    del s1
+    assert np.abs(got - ref).max() < 1e-9 * np.abs(ref).max(), np.abs(got - ref).max()
+
+
 def test_large_rks_matches_oracle_631g_level1(hip):
     mol = build_mol(WATER, "6-31g")
     grid = grids.build_grid(mol, 1)
