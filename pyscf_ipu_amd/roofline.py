@@ -85,5 +85,6 @@ def per_launch_work(kernel: str, mols, dtype: str) -> dict:
         # FP64 arithmetic (MFMA products plus VALU rotations) whose peak is the
         # MI355X FP64 rate; what limits it is latency: one workgroup per molecule
         return {"bound": "mfma", "flops": f, "model": "38 N^3 FP64",
-                "limiter": "latency: one workgroup (one CU) per molecule, dependent Jacobi rounds"}
+                "limiter": "latency: one workgroup (one CU) per molecule; its eigen-decomposition is a chain of "
+                            "dependent reflector, bisection or refinement steps"}
     raise KeyError(kernel)
