@@ -240,10 +240,12 @@ __device__ __forceinline__ void s8_row(const T* __restrict__ row, int pq, int p,
 // nao^2 (sizeof(T) + 8) bytes do not fit: the waves' atomic adds come in an
 // order that changes from run to run, which shows at 1e-7 of the sum in FP32
 // and at 1e-16 in FP64 (ds_add_f64, no slower here than the FP32 adds).
+// out != nullptr (FP64): each workgroup adds its M + M^T to the zeroed out by
+// exact atomics instead of writing a partial.
 template <typename T, typename TA>
 __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__ eri, int N,
                                                            const T* __restrict__ dm_g, T hhalf,
-                                                           T* __restrict__ part) {
+                                                           T* __restrict__ part, double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NN = N * N;
   const int P = N * (N + 1) / 2;
@@ -263,6 +265,13 @@ __global__ void __launch_bounds__(JK_THREADS) jk_s8_kernel(const T* __restrict__
     s8_row<T, false, TA>(eri + tri(pq), pq, p, q, dm, acc, N, hhalf, lane);
   }
   __syncthreads();
+  if (out != nullptr) {
+    for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+      const int i = x / N, j = x - i * N;
+      atomicAdd(out + x, quant40((double)acc[x] + (double)acc[j * N + i]));
+    }
+    return;
+  }
   T* dst = part + (int64_t)blockIdx.x * NN;
   for (int x = threadIdx.x; x < NN; x += blockDim.x) dst[x] = (T)acc[x];
 }
@@ -699,14 +708,22 @@ static int jk_s8_impl(const T* eri, int N, const T* dm, double hyb, T* out, void
                max_lds_bytes());
   const int nb = jk_s8_blocks(N);
   PSCF_REQUIRE(ws && ws_bytes >= jk_s8_ws_bytes(sizeof(T), N), "pscf_jk_s8: workspace too small");
+  // FP64: exact atomic sums into the zeroed output, no partials to reduce
+  constexpr bool ATOMIC = sizeof(T) == 8;
+  double* aout = ATOMIC ? (double*)out : nullptr;
+  if (ATOMIC) {
+    const int rz = zero2(aout, (int64_t)N * N, nullptr, 0, st);
+    if (rz) return rz;
+  }
   if (acc64) {
     PSCF_SET_LDS((jk_s8_kernel<T, double>), lds);
-    jk_s8_kernel<T, double><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
+    jk_s8_kernel<T, double><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws, aout);
   } else {
     PSCF_SET_LDS((jk_s8_kernel<T, T>), lds);
-    jk_s8_kernel<T, T><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws);
+    jk_s8_kernel<T, T><<<nb, JK_THREADS, lds, st>>>(eri, N, dm, (T)(hyb / 2.0), (T*)ws, aout);
   }
   PSCF_HIP(hipGetLastError());
+  if (ATOMIC) return PSCF_OK;
   T* lvl = (T*)((char*)ws + red_align(sizeof(T) * (size_t)nb * N * N));
   return reduce_parts<T, true>((const T*)ws, nb, N, lvl, out, nullptr, nullptr, st);
 }

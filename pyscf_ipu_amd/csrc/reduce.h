@@ -14,6 +14,32 @@
 
 namespace pscf {
 
+// Order-independent FP64 sums for the single-molecule FP64 paths: each
+// workgroup's partial is rounded to a multiple of 2^-40 and added to the
+// output with an atomic.  Sums of such values are exact in FP64 while they
+// stay below 2^13 (Fock-matrix elements and E_xc here are far below), so the
+// result does not depend on the order of the atomics and repeats bit for
+// bit; the rounding adds at most 2^-41 per workgroup and element.
+__device__ __forceinline__ double quant40(double v) { return rint(v * 0x1p40) * 0x1p-40; }
+
+// Zeroes a[0..na) and b[0..nb) (the outputs of an atomic sum).
+static __global__ void __launch_bounds__(256) zero2_kernel(double* __restrict__ a, int64_t na, double* __restrict__ b,
+                                                            int nb) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < na + nb; x += (int64_t)gridDim.x * blockDim.x) {
+    if (x < na)
+      a[x] = 0.0;
+    else
+      b[x - na] = 0.0;
+  }
+}
+
+inline int zero2(double* a, int64_t na, double* b, int nb, hipStream_t st) {
+  const int64_t blocks = (na + nb + 255) / 256;
+  zero2_kernel<<<(unsigned)(blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks)), 256, 0, st>>>(a, na, b, nb);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
 constexpr int RED_CHUNK = 16;
 constexpr int RED_THREADS = 256;
 

@@ -146,6 +146,8 @@ struct XCArgs {
   T eps;
   T* part_v;        // [blocks][N*N]
   double* part_e;   // [blocks]
+  T* vout;          // FP64: V_xc (zeroed) and E_xc take each workgroup's part by
+  double* eout;     //   exact atomics instead of the partials (nullptr: partials)
 };
 
 // LDS layout: D[N][N] | phi0[N][TILE+1] | XY[N][TILE+1] | coef[4][TILE] | red[8]
@@ -650,8 +652,26 @@ __global__ void __launch_bounds__(XC_THREADS) xc_fused_kernel(XCArgs<T> a) {
   // is below 2^31 elements)
   XcStageStored<T, XT, NP> stage{a.ao, (int)a.sc, (int)a.smu, (int)a.sp, a.g, a.N};
   xc_tiles<T, XT, NP>(a, smem, blockIdx.x, gridDim.x, (a.g + XT - 1) / XT, stage, vacc, e_loc);
-  xm_store_v<T, XT, NP>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
   double* red = reinterpret_cast<double*>(smem);
+  if constexpr (sizeof(T) == 8) {
+    if (a.vout != nullptr) {
+      // this workgroup's V + V^T and energy into the outputs (exact atomics)
+      __syncthreads();
+      const double e = xc_block_energy(e_loc, red);
+      if (threadIdx.x == 0) atomicAdd(a.eout, quant40(e));
+      __syncthreads();
+      T* Vl = reinterpret_cast<T*>(smem);
+      xm_store_v<T, XT, NP>(vacc, a.N, Vl, a.N);
+      __syncthreads();
+      const int NN = a.N * a.N;
+      for (int x = threadIdx.x; x < NN; x += blockDim.x) {
+        const int i = x / a.N, j = x - i * a.N;
+        atomicAdd(a.vout + x, quant40(Vl[x] + Vl[j * a.N + i]));
+      }
+      return;
+    }
+  }
+  xm_store_v<T, XT, NP>(vacc, a.N, a.part_v + (int64_t)blockIdx.x * a.N * a.N, a.N);
   __syncthreads();
   const double e = xc_block_energy(e_loc, red);
   if (threadIdx.x == 0) a.part_e[blockIdx.x] = e;
@@ -728,6 +748,8 @@ __global__ void __launch_bounds__(XC_THREADS, (xc_batch_min_blocks<T, XT, NP>())
     return;
   }
   XCArgs<T> a;
+  a.vout = nullptr;
+  a.eout = nullptr;
   a.N = N;
   a.g = (int)Mo.ngrid;
   a.ao = DIRECT ? nullptr : ao + Mo.ao0;
@@ -1098,6 +1120,14 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
   a.eps = (T)eps;
   a.part_v = (T*)ws;
   a.part_e = (double*)((char*)ws + lay.e_off);
+  // FP64: exact atomic sums into the zeroed outputs, no partials to reduce
+  constexpr bool ATOMIC = sizeof(T) == 8;
+  a.vout = ATOMIC ? vxc : nullptr;
+  a.eout = ATOMIC ? exc : nullptr;
+  if (ATOMIC) {
+    const int rz = zero2((double*)vxc, (int64_t)N * N, exc, 1, st);
+    if (rz) return rz;
+  }
   int rc;
   switch (xm_np(N)) {
     case 16: rc = xc_fused_launch<T, 16>(a, nb, st); break;
@@ -1106,7 +1136,7 @@ static int xc_impl(const T* ao, int64_t sc, int64_t smu, int64_t sp, int g, int 
     case 64: rc = xc_fused_launch<T, 64>(a, nb, st); break;
     default: rc = xc_fused_launch<T, 80>(a, nb, st); break;
   }
-  if (rc) return rc;
+  if (rc || ATOMIC) return rc;
   return reduce_parts<T, true>(a.part_v, nb, N, (T*)((char*)ws + lay.lvl_off), vxc, a.part_e, exc, st);
 }
 
