@@ -889,6 +889,67 @@ __device__ inline T becke_f3(T g) {
   return (T(3) - g * g) * g * T(0.5);
 }
 
+// Molecules of at most BECKE_FAST_MAXA atoms: each point's atom distances in
+// registers, computed once (instead of once per ordered pair), and every
+// unordered pair visited once for both its ordered cell functions, each with
+// its own adjustment a_ij / a_ji from the table.
+constexpr int BECKE_FAST_MAXA = 32;
+
+__global__ void __launch_bounds__(256) becke_fast_kernel(const double* __restrict__ coords,
+                                                         const int* __restrict__ owner,
+                                                         const double* __restrict__ vol, int g,
+                                                         const double* __restrict__ atom_xyz,
+                                                         const double* __restrict__ adj, int natm,
+                                                         double* __restrict__ w) {
+  constexpr int MA = BECKE_FAST_MAXA;
+  __shared__ double s_xyz[3 * MA], s_inv[MA * MA], s_a[MA * MA];
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x) s_xyz[i] = atom_xyz[i];
+  for (int q = threadIdx.x; q < natm * natm; q += blockDim.x) {
+    const int p = q / natm, r = q - p * natm;
+    const double dx = atom_xyz[3 * p] - atom_xyz[3 * r], dy = atom_xyz[3 * p + 1] - atom_xyz[3 * r + 1],
+                 dz = atom_xyz[3 * p + 2] - atom_xyz[3 * r + 2];
+    s_inv[p * MA + r] = p == r ? 0.0 : 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    s_a[p * MA + r] = adj[q];
+  }
+  __syncthreads();
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= g) return;
+  const double x = coords[3 * pt], y = coords[3 * pt + 1], z = coords[3 * pt + 2];
+  double r[MA], P[MA];
+#pragma unroll
+  for (int i = 0; i < MA; ++i) {
+    P[i] = 1.0;
+    r[i] = 0.0;
+    if (i < natm) {
+      const double dx = x - s_xyz[3 * i], dy = y - s_xyz[3 * i + 1], dz = z - s_xyz[3 * i + 2];
+      r[i] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  // (a uniform branch per i rather than a cut-short loop, so both loops stay
+  // unrolled and r, P in registers)
+#pragma unroll
+  for (int i = 1; i < MA; ++i) {
+    if (i < natm) {
+#pragma unroll
+      for (int j = 0; j < i; ++j) {
+        const double mu = (r[i] - r[j]) * s_inv[i * MA + j];
+        const double om = 1.0 - mu * mu;
+        P[i] *= 0.5 * (1.0 - becke_f3(mu + s_a[i * MA + j] * om));
+        P[j] *= 0.5 * (1.0 - becke_f3(-mu + s_a[j * MA + i] * om));
+      }
+    }
+  }
+  const int own = owner[pt];
+  double psum = 0.0, pown = 0.0;
+#pragma unroll
+  for (int i = 0; i < MA; ++i) {
+    if (i < natm) psum += P[i];
+    if (i == own) pown = P[i];
+  }
+  w[pt] = vol[pt] * pown / psum;
+}
+
+
 // Weight of one point, every atom pair evaluated from scratch (blocks whose
 // points straddle two molecules, or molecules above BECKE_MAXA atoms).
 template <typename T>
@@ -1380,6 +1441,12 @@ extern "C" int pscf_becke_weights(const double* d_coords, const int* d_owner, co
   PSCF_REQUIRE(g >= 0 && natm > 0 && natm <= 88, "pscf_becke_weights: natm=%d out of range (1..88)", natm);
   if (g == 0) return PSCF_OK;
   PSCF_REQUIRE(d_coords && d_owner && d_vol && d_atom_xyz && d_adjust && d_w, "pscf_becke_weights: null pointer");
+  if (natm <= BECKE_FAST_MAXA) {
+    becke_fast_kernel<<<(g + 255) / 256, 256, 0, (hipStream_t)stream>>>(d_coords, d_owner, d_vol, g, d_atom_xyz,
+                                                                       d_adjust, natm, d_w);
+    PSCF_HIP(hipGetLastError());
+    return PSCF_OK;
+  }
   const size_t lds = sizeof(double) * (3 * (size_t)natm + (size_t)natm * natm);
   becke_kernel<<<(g + 255) / 256, 256, lds, (hipStream_t)stream>>>(d_coords, d_owner, d_vol, g, d_atom_xyz, d_adjust,
                                                                   natm, d_w);

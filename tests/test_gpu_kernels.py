@@ -191,6 +191,18 @@ def test_ao_eval(hip, mol, dtype, tol):
 
 
 def test_becke_weights(hip, mol):
+    _check_becke(mol)
+
+
+def test_becke_weights_many_atoms(hip):
+    """Past 32 atoms the weights come from the general kernel (every ordered
+    pair from scratch) instead of the register-resident one."""
+    m = build_mol(molgen.plain_set(1, 16, seed=3)[0])
+    assert m.natm > 32
+    _check_becke(m)
+
+
+def _check_becke(mol):
     from pyscf_ipu_amd import _lib
     g = grids.atomic_grids(mol, 0)
     w = device.empty(g.size, torch.float64)
