@@ -510,7 +510,8 @@ __global__ void __launch_bounds__(256) eri_quartets_kernel(const int* __restrict
 // shell pair computed by one 64-lane wave: lanes take (primitive pair, nucleus)
 // combinations, a fixed shuffle tree sums them, lane 0 writes both triangles
 // (row stride nao).  hcore: write T + V into Tk instead of T and V.
-__device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, int natm, const int* __restrict__ bas,
+template <int LA, int LB>
+__device__ __forceinline__ void int1e_pair_wave_l(const int* __restrict__ atm, int atom0, int natm, const int* __restrict__ bas,
                                        const double* __restrict__ env, const double* __restrict__ tab, int ia, int ib,
                                        int ao_a, int ao_b, int nao, double* __restrict__ S, double* __restrict__ Tk,
                                        double* __restrict__ V, bool hcore) {
@@ -519,12 +520,13 @@ __device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, i
   const int* bb = bas + ib * BAS_SLOTS;
   const double* RA = env + atm[ba[0] * ATM_SLOTS + 1];
   const double* RB = env + atm[bb[0] * ATM_SLOTS + 1];
-  const int la = ba[1], lb = bb[1], npa = ba[2], npb = bb[2];
+  constexpr int la = LA, lb = LB;
+  const int npa = ba[2], npb = bb[2];
   const double* ea = env + ba[5];
   const double* ca = env + ba[6];
   const double* eb = env + bb[5];
   const double* cb = env + bb[6];
-  const int na = la ? 3 : 1, nb = lb ? 3 : 1;
+  constexpr int na = la ? 3 : 1, nb = lb ? 3 : 1;
   const double fac = (la ? ANG_P : ANG_S) * (lb ? ANG_P : ANG_S);
   double s_acc[9] = {0}, t_acc[9] = {0}, v_acc[9] = {0};
   double AB[3] = {RA[0] - RB[0], RA[1] - RB[1], RA[2] - RB[2]};
@@ -579,7 +581,7 @@ __device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, i
     const double Z = (double)atm[c * ATM_SLOTS];
     double PC[3] = {P[0] - Cn[0], P[1] - Cn[1], P[2] - Cn[2]};
     const double tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-    const int nr = (la + lb) / 2 + 1;
+    constexpr int nr = (la + lb) / 2 + 1;
     double u[2], w[2];
     if (nr == 1) {
       double f[2];
@@ -629,6 +631,23 @@ __device__ inline void int1e_pair_wave(const int* __restrict__ atm, int atom0, i
           V[r * nao + c] = V[c * nao + r] = v_acc[fa * 3 + fb];
         }
       }
+}
+
+// Shell classes of the pair as compile-time constants: every loop over AO
+// components unrolls and the per-lane accumulators stay in registers.
+__device__ __forceinline__ void int1e_pair_wave(const int* __restrict__ atm, int atom0, int natm, const int* __restrict__ bas,
+                                       const double* __restrict__ env, const double* __restrict__ tab, int ia, int ib,
+                                       int ao_a, int ao_b, int nao, double* __restrict__ S, double* __restrict__ Tk,
+                                       double* __restrict__ V, bool hcore) {
+  const int la = bas[ia * BAS_SLOTS + 1], lb = bas[ib * BAS_SLOTS + 1];
+  if (la == 0 && lb == 0)
+    int1e_pair_wave_l<0, 0>(atm, atom0, natm, bas, env, tab, ia, ib, ao_a, ao_b, nao, S, Tk, V, hcore);
+  else if (la == 1 && lb == 0)
+    int1e_pair_wave_l<1, 0>(atm, atom0, natm, bas, env, tab, ia, ib, ao_a, ao_b, nao, S, Tk, V, hcore);
+  else if (la == 0 && lb == 1)
+    int1e_pair_wave_l<0, 1>(atm, atom0, natm, bas, env, tab, ia, ib, ao_a, ao_b, nao, S, Tk, V, hcore);
+  else
+    int1e_pair_wave_l<1, 1>(atm, atom0, natm, bas, env, tab, ia, ib, ao_a, ao_b, nao, S, Tk, V, hcore);
 }
 
 __device__ inline int64_t tri_root(int64_t q) {
