@@ -242,6 +242,13 @@ int pscf_grad_xc(const double* d_coords, const double* d_weights, int g, const i
 int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const int* const* h_work, const int* h_nwork,
                       const int* d_atm, const int* d_bas, const double* d_env, const int* d_shell_ao,
                       const int* d_shl_sorted, void* d_out, void* stream);
+/* pscf_batch_eri_s8 with every element divided by its multiplicity
+ * 2^([p==q] + [r==s] + [pq==rs]) as it is written, the form
+ * pscf_batch_eri_scale produces (sparse_symmetric_ERI.py:68-79, 180-182),
+ * without the second pass over the ERIs. */
+int pscf_batch_eri_s8_div(int dtype, const void* d_mols, int nmol, const int* const* h_work, const int* h_nwork,
+                          const int* d_atm, const int* d_bas, const double* d_env, const int* d_shell_ao,
+                          const int* d_shl_sorted, void* d_out, void* stream);
 /* S and H_core = T + V of all molecules (nanoDFT.py:236-238). */
 int pscf_batch_int1e(const void* d_mols, int nmol, const int* d_atm, const int* d_bas, const double* d_env,
                      const int* d_shell_ao, double* d_S, double* d_Hcore, void* stream);

@@ -319,11 +319,10 @@ class BatchRKS:
         arr = (ctypes.c_void_p * 6)(*[w.data_ptr() for w in works])
         cnt = (ctypes.c_int * 6)(*[len(w) for w in self.eri_work])
         tm("eri_s8")
-        _lib.call("pscf_batch_eri_s8", self.code, P(self.d_desc), self.M, arr, cnt, P(self.d_atm), P(self.d_bas),
-                  P(self.d_env), P(self.d_shell_ao), P(self.d_shl), P(self.eri), st)
-        # stored over their multiplicity, the form the J/K contraction takes
-        tm("eri_scale")
-        _lib.call("pscf_batch_eri_scale", self.code, P(self.d_desc), self.M, self.max_nao, P(self.eri), st)
+        # stored over their multiplicity as they are written, the form the J/K
+        # contraction takes
+        _lib.call("pscf_batch_eri_s8_div", self.code, P(self.d_desc), self.M, arr, cnt, P(self.d_atm),
+                  P(self.d_bas), P(self.d_env), P(self.d_shell_ao), P(self.d_shl), P(self.eri), st)
         tm("end")
         # iteration buffers.  dJK is FP64; in FP32 mode it is built
         # incrementally, dJK += JK(D_new - D_old) with the change in FP32, so the

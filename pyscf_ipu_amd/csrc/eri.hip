@@ -704,7 +704,8 @@ __global__ void __launch_bounds__(256) eri_s8_batch_kernel(const BatchMol* __res
                                                            const int* __restrict__ atm, const int* __restrict__ bas,
                                                            const double* __restrict__ env,
                                                            const int* __restrict__ shell_ao,
-                                                           const int* __restrict__ shl_sorted, T* __restrict__ out) {
+                                                           const int* __restrict__ shl_sorted, T* __restrict__ out,
+                                                           int div) {
   constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
   constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
   using BT = typename BoysType<T, NR>::type;
@@ -757,7 +758,10 @@ __global__ void __launch_bounds__(256) eri_s8_batch_kernel(const BatchMol* __res
           const int64_t rs = r >= s ? tri(r) + s : tri(s) + r;
           if (same_pair && rs > pq) continue;
           const int64_t idx = pq >= rs ? tri(pq) + rs : tri(rs) + pq;
-          dst[idx] = acc[((fi * NJ + fj) * NK + fk) * NL + fl];
+          T v = acc[((fi * NJ + fj) * NK + fk) * NL + fl];
+          // div: over the element's multiplicity 2^([p==q] + [r==s] + [pq==rs])
+          if (div) v *= T(1.0 / (double)(1 << ((p == qq) + (r == s) + (pq == rs))));
+          dst[idx] = v;
         }
 }
 
@@ -1354,33 +1358,33 @@ extern "C" int pscf_int1e(const int* d_atm, int natm, const int* d_bas, int nbas
 
 template <typename T, int X, int Y>
 static int launch_batch_s8(const BatchMol* mols, const int* work, int nwork, const int* atm, const int* bas,
-                           const double* env, const int* shell_ao, const int* shl, T* out, hipStream_t st) {
+                           const double* env, const int* shell_ao, const int* shl, T* out, int div, hipStream_t st) {
   if (nwork == 0) return PSCF_OK;
   constexpr int LI = PairL<X>::a, LJ = PairL<X>::b, LK = PairL<Y>::a, LL = PairL<Y>::b;
   constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
   using BT = typename BoysType<T, NR>::type;
   const size_t shm = sizeof(BT) * BOYS_NT * BOYS_NM;
-  eri_s8_batch_kernel<T, X, Y><<<nwork, 256, shm, st>>>(mols, work, atm, bas, env, shell_ao, shl, out);
+  eri_s8_batch_kernel<T, X, Y><<<nwork, 256, shm, st>>>(mols, work, atm, bas, env, shell_ao, shl, out, div);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
 
 template <typename T>
 static int batch_s8_all(const BatchMol* mols, const int* const* work, const int* nwork, const int* atm, const int* bas,
-                        const double* env, const int* shell_ao, const int* shl, T* out, hipStream_t st) {
+                        const double* env, const int* shell_ao, const int* shl, T* out, int div, hipStream_t st) {
   int rc;
-  if ((rc = launch_batch_s8<T, 0, 0>(mols, work[0], nwork[0], atm, bas, env, shell_ao, shl, out, st))) return rc;
-  if ((rc = launch_batch_s8<T, 1, 0>(mols, work[1], nwork[1], atm, bas, env, shell_ao, shl, out, st))) return rc;
-  if ((rc = launch_batch_s8<T, 1, 1>(mols, work[2], nwork[2], atm, bas, env, shell_ao, shl, out, st))) return rc;
-  if ((rc = launch_batch_s8<T, 2, 0>(mols, work[3], nwork[3], atm, bas, env, shell_ao, shl, out, st))) return rc;
-  if ((rc = launch_batch_s8<T, 2, 1>(mols, work[4], nwork[4], atm, bas, env, shell_ao, shl, out, st))) return rc;
-  if ((rc = launch_batch_s8<T, 2, 2>(mols, work[5], nwork[5], atm, bas, env, shell_ao, shl, out, st))) return rc;
+  if ((rc = launch_batch_s8<T, 0, 0>(mols, work[0], nwork[0], atm, bas, env, shell_ao, shl, out, div, st))) return rc;
+  if ((rc = launch_batch_s8<T, 1, 0>(mols, work[1], nwork[1], atm, bas, env, shell_ao, shl, out, div, st))) return rc;
+  if ((rc = launch_batch_s8<T, 1, 1>(mols, work[2], nwork[2], atm, bas, env, shell_ao, shl, out, div, st))) return rc;
+  if ((rc = launch_batch_s8<T, 2, 0>(mols, work[3], nwork[3], atm, bas, env, shell_ao, shl, out, div, st))) return rc;
+  if ((rc = launch_batch_s8<T, 2, 1>(mols, work[4], nwork[4], atm, bas, env, shell_ao, shl, out, div, st))) return rc;
+  if ((rc = launch_batch_s8<T, 2, 2>(mols, work[5], nwork[5], atm, bas, env, shell_ao, shl, out, div, st))) return rc;
   return PSCF_OK;
 }
 
-extern "C" int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const int* const* h_work,
-                                 const int* h_nwork, const int* d_atm, const int* d_bas, const double* d_env,
-                                 const int* d_shell_ao, const int* d_shl_sorted, void* d_out, void* stream) {
+static int batch_eri_s8_any(int dtype, const void* d_mols, int nmol, const int* const* h_work, const int* h_nwork,
+                            const int* d_atm, const int* d_bas, const double* d_env, const int* d_shell_ao,
+                            const int* d_shl_sorted, void* d_out, int div, void* stream) {
   PSCF_REQUIRE(dtype == PSCF_F64 || dtype == PSCF_F32, "pscf_batch_eri_s8: bad dtype");
   PSCF_REQUIRE(nmol > 0 && d_mols && h_work && h_nwork && d_atm && d_bas && d_env && d_shell_ao && d_shl_sorted &&
                    d_out,
@@ -1391,9 +1395,23 @@ extern "C" int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const 
   hipStream_t st = (hipStream_t)stream;
   if (dtype == PSCF_F64)
     return batch_s8_all<double>((const BatchMol*)d_mols, h_work, h_nwork, d_atm, d_bas, d_env, d_shell_ao,
-                                d_shl_sorted, (double*)d_out, st);
+                                d_shl_sorted, (double*)d_out, div, st);
   return batch_s8_all<float>((const BatchMol*)d_mols, h_work, h_nwork, d_atm, d_bas, d_env, d_shell_ao, d_shl_sorted,
-                             (float*)d_out, st);
+                             (float*)d_out, div, st);
+}
+
+extern "C" int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const int* const* h_work,
+                                 const int* h_nwork, const int* d_atm, const int* d_bas, const double* d_env,
+                                 const int* d_shell_ao, const int* d_shl_sorted, void* d_out, void* stream) {
+  return batch_eri_s8_any(dtype, d_mols, nmol, h_work, h_nwork, d_atm, d_bas, d_env, d_shell_ao, d_shl_sorted, d_out,
+                          0, stream);
+}
+
+extern "C" int pscf_batch_eri_s8_div(int dtype, const void* d_mols, int nmol, const int* const* h_work,
+                                     const int* h_nwork, const int* d_atm, const int* d_bas, const double* d_env,
+                                     const int* d_shell_ao, const int* d_shl_sorted, void* d_out, void* stream) {
+  return batch_eri_s8_any(dtype, d_mols, nmol, h_work, h_nwork, d_atm, d_bas, d_env, d_shell_ao, d_shl_sorted, d_out,
+                          1, stream);
 }
 
 extern "C" int pscf_batch_int1e(const void* d_mols, int nmol, const int* d_atm, const int* d_bas, const double* d_env,

@@ -196,6 +196,20 @@ def test_batch_eri_scale_matches_oracle(hip):
         o = int(b.desc[m, 11])
         got = eri[o:o + ref.size]
         np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-13)
+    # the divided-as-written form equals the raw ERIs divided in place, bit for bit
+    from pyscf_ipu_amd import _lib, device
+    import ctypes
+    import torch
+    raw = torch.empty_like(b.eri)
+    works = [device.to_dev(w) for w in b.eri_work]
+    arr = (ctypes.c_void_p * 6)(*[w.data_ptr() for w in works])
+    cnt = (ctypes.c_int * 6)(*[len(w) for w in b.eri_work])
+    P, st = device.ptr, device.stream()
+    _lib.call("pscf_batch_eri_s8", b.code, P(b.d_desc), b.M, arr, cnt, P(b.d_atm), P(b.d_bas), P(b.d_env),
+              P(b.d_shell_ao), P(b.d_shl), P(raw), st)
+    _lib.call("pscf_batch_eri_scale", b.code, P(b.d_desc), b.M, b.max_nao, P(raw), st)
+    torch.cuda.synchronize()
+    assert torch.equal(raw, b.eri)
 
 
 @pytest.mark.parametrize("dtype", ["fp64", "fp32"])
