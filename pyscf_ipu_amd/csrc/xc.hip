@@ -831,14 +831,19 @@ __global__ void __launch_bounds__(256) xc_batch_reduce_kernel(const BatchMol* __
     if (threadIdx.x == 0) exc[m] = NAN;
     return;
   }
+  // chunk sums element by element (coalesced), in chunk order, into LDS;
+  // then V + V^T from LDS
+  extern __shared__ __align__(16) unsigned char smem_r[];
+  T* Vs = reinterpret_cast<T*>(smem_r);
+  for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
+    T v = T(0);
+    for (int c = 0; c < nchunk; ++c) v += part_v[((int64_t)c * gridDim.x + blockIdx.x) * max_nao * max_nao + x];
+    Vs[x] = v;
+  }
+  __syncthreads();
   for (int x = threadIdx.x; x < N * N; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
-    T v = T(0);
-    for (int c = 0; c < nchunk; ++c) {
-      const T* P = part_v + ((int64_t)c * gridDim.x + blockIdx.x) * max_nao * max_nao;
-      v += P[x] + P[j * N + i];
-    }
-    vxc[Mo.mat0 + x] = v;
+    vxc[Mo.mat0 + x] = Vs[x] + Vs[j * N + i];
   }
   if (threadIdx.x == 0) {
     double e = 0.0;
@@ -1578,9 +1583,9 @@ static int batch_xc_launch(const void* d_mols, const int* d_order, int nmol, int
         (const T*)d_w, (T)eps, (T*)d_vxc, d_exc, (T*)d_part_v, d_part_e, part_v_elems, part_e_elems);
   }
   PSCF_HIP(hipGetLastError());
-  xc_batch_reduce_kernel<T><<<nmol, 256, 0, st>>>((const BatchMol*)d_mols, d_order, max_nao, d_ctl,
-                                                 (const T*)d_part_v, d_part_e, part_v_elems, part_e_elems,
-                                                 (T*)d_vxc, d_exc, da ? da->nsh_max : 0);
+  xc_batch_reduce_kernel<T><<<nmol, 256, sizeof(T) * (size_t)max_nao * max_nao, st>>>(
+      (const BatchMol*)d_mols, d_order, max_nao, d_ctl, (const T*)d_part_v, d_part_e, part_v_elems, part_e_elems,
+      (T*)d_vxc, d_exc, da ? da->nsh_max : 0);
   PSCF_HIP(hipGetLastError());
   return PSCF_OK;
 }
