@@ -249,6 +249,13 @@ int pscf_batch_eri_s8(int dtype, const void* d_mols, int nmol, const int* const*
 int pscf_batch_eri_s8_div(int dtype, const void* d_mols, int nmol, const int* const* h_work, const int* h_nwork,
                           const int* d_atm, const int* d_bas, const double* d_env, const int* d_shell_ao,
                           const int* d_shl_sorted, void* d_out, void* stream);
+/* Work list (owner, first index) pairs [total][2] covering range(count_i) of
+ * each owner i in steps of `size`, from d_first = the exclusive prefix sums
+ * of ceil(count_i / size) (n + 1 entries, d_first[n] = total): the chunk
+ * lists of pscf_batch_grid and pscf_batch_eri_s8, built on the device. */
+int pscf_batch_chunks(const int64_t* d_first, int n, int size, int* d_work, int64_t total, void* stream);
+/* Nuclear repulsion energy of every molecule (mol.energy_nuc(), nanoDFT.py). */
+int pscf_batch_enuc(const void* d_mols, int nmol, const int* d_atm, const double* d_env, double* d_enuc, void* stream);
 /* S and H_core = T + V of all molecules (nanoDFT.py:236-238). */
 int pscf_batch_int1e(const void* d_mols, int nmol, const int* d_atm, const int* d_bas, const double* d_env,
                      const int* d_shell_ao, double* d_S, double* d_Hcore, void* stream);

@@ -60,6 +60,8 @@ _SIGS = {
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
     "pscf_batch_eri_s8_div": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pscf_batch_chunks": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
+    "pscf_batch_enuc": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pscf_batch_int1e": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
     "pscf_rks_run": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_void_p, c_void_p, c_void_p,

@@ -76,18 +76,6 @@ def class_pairs(X: int, ns: np.ndarray, npp: np.ndarray) -> np.ndarray:
     return ns * (ns + 1) // 2 if X == 0 else (ns * npp if X == 1 else npp * (npp + 1) // 2)
 
 
-def _chunks(counts: np.ndarray, size: int) -> np.ndarray:
-    """(owner, start) pairs covering range(counts[i]) in steps of `size`."""
-    n = (counts.astype(np.int64) + size - 1) // size
-    total = int(n.sum())
-    out = np.empty((total, 2), np.int32)  # written in place, int32 throughout (2 M rows at 4096 molecules)
-    out[:, 0] = np.repeat(np.arange(len(counts), dtype=np.int32), n)
-    first = np.repeat((np.cumsum(n) - n).astype(np.int32), n)
-    np.subtract(np.arange(total, dtype=np.int32), first, out=out[:, 1])
-    out[:, 1] *= size
-    return out
-
-
 def parse(mol_strs: Sequence[str]) -> List[Tuple[np.ndarray, np.ndarray]]:
     """Atom strings (Angstrom) -> [(Z, R Bohr)]."""
     out = []
@@ -227,12 +215,14 @@ class BatchRKS:
         self.kp = int(bas[:, 2].max()) if len(bas) else 1
         self.sizes = {"eri": int(n_s8.sum()), "mat": int((nao * nao).sum()), "vec": int(nao.sum()),
                       "ao": int(ao_sz.sum()), "state": int(st.sum()), "grid": int(ngrid.sum())}
-        self.grid_work = _chunks(apts, 256)
-        self.eri_work = []
+        # work lists of the grid expansion (per atom) and of the six ERI classes
+        # (per molecule): chunk counts here, the (owner, first) lists built on
+        # the device at set-up (pscf_batch_chunks)
+        self.grid_counts = apts
+        self.eri_counts = []
         for X, Y in CLASSES:
             nx, ny = class_pairs(X, ns, npp), class_pairs(Y, ns, npp)
-            nq = nx * (nx + 1) // 2 if X == Y else nx * ny
-            self.eri_work.append(_chunks(nq, 256))
+            self.eri_counts.append(nx * (nx + 1) // 2 if X == Y else nx * ny)
         self.order = np.argsort(-nao, kind="stable").astype(np.int32)
         self.max_nao = int(nao.max())
         # XC and J/K launches per class of nao rounded up to 16 (LDS sized per
@@ -242,20 +232,22 @@ class BatchRKS:
         for v in np.unique(npad)[::-1]:
             pos = np.where(npad == v)[0]
             self.nao_classes.append((int(pos[0]), len(pos), int(nao[self.order[pos[0]]])))
-        # nuclear repulsion, vectorised over molecules of equal atom count
-        e_nuc = np.zeros(M)
-        for na in np.unique(natm):
-            ms = np.where(natm == na)[0]
-            if na < 2:
-                continue
-            idx = atm0[ms][:, None] + np.arange(na)[None, :]
-            r, z = R[idx], Z[idx].astype(np.float64)           # (k, na, 3), (k, na)
-            d = np.linalg.norm(r[:, :, None] - r[:, None, :], axis=-1)
-            iu = np.triu_indices(na, 1)
-            e_nuc[ms] = np.sum((z[:, :, None] * z[:, None, :])[:, iu[0], iu[1]] / d[:, iu[0], iu[1]], axis=1)
-        self.e_nuc = e_nuc
+
 
     # ------------------------------------------------------------ device set-up ---
+    @staticmethod
+    def _work_list(counts: np.ndarray, size: int = 256) -> torch.Tensor:
+        """(owner, first) int32 pairs covering range(counts[i]) in steps of
+        `size`, built on the device from the owners' chunk counts."""
+        n = (np.asarray(counts, np.int64) + size - 1) // size
+        first = np.zeros(len(n) + 1, np.int64)
+        np.cumsum(n, out=first[1:])
+        total = int(first[-1])
+        work = device.empty((max(total, 1), 2), torch.int32)
+        d_first = device.to_dev(first)
+        _lib.call("pscf_batch_chunks", device.ptr(d_first), len(n), size, device.ptr(work), total, device.stream())
+        return work[:total]
+
     def setup(self, dm0: torch.Tensor = None, guess=None):
         """Device set-up.  The initial density is `dm0` (concatenated FP64
         matrices) if given, else what `guess(self)` writes into self.D and
@@ -267,7 +259,9 @@ class BatchRKS:
         self.d_atm, self.d_bas, self.d_env = (device.to_dev(self.atm), device.to_dev(self.bas), device.to_dev(self.env))
         self.d_shell_ao, self.d_shl = device.to_dev(self.shell_ao), device.to_dev(self.shl_sorted)
         self.d_order = device.to_dev(self.order)
-        self.d_enuc = device.to_dev(self.e_nuc)
+        self.d_enuc = device.empty(self.M, torch.float64)
+        _lib.call("pscf_batch_enuc", device.ptr(self.d_desc), self.M, device.ptr(self.d_atm), device.ptr(self.d_env),
+                  device.ptr(self.d_enuc), device.stream())
         # grid
         G = S["grid"]
         self.G = G
@@ -275,14 +269,14 @@ class BatchRKS:
         self.vol = device.empty(G, f64)
         self.owner = device.empty(G, torch.int32)
         self.ptmol = device.empty(G, torch.int32)
-        gw = device.to_dev(self.grid_work)
+        gw = self._work_list(self.grid_counts)
         ae, ap0, am = (device.to_dev(self.atom_elem), device.to_dev(self.atom_pt0), device.to_dev(self.atom_mol))
         toff, tn, tg = device.to_dev(self.tmpl_off), device.to_dev(self.tmpl_n), device.to_dev(self.tmpl_grid)
         P = device.ptr
         st = device.stream()
         tm = self.setup_timer = PhaseTimer()
         tm("grid")
-        _lib.call("pscf_batch_grid", P(gw), len(self.grid_work), P(ae), P(ap0), P(am), P(toff), P(tn), P(tg),
+        _lib.call("pscf_batch_grid", P(gw), gw.shape[0], P(ae), P(ap0), P(am), P(toff), P(tn), P(tg),
                   P(self.d_atm), P(self.d_env), P(self.coords), P(self.vol), P(self.owner), P(self.ptmol), st)
         self.w = device.empty(G, T)
         tm("becke")
@@ -315,9 +309,9 @@ class BatchRKS:
             self.Dt.copy_(dm0)
         # ERIs
         self.eri = device.empty(S["eri"], T)
-        works = [device.to_dev(w) for w in self.eri_work]
+        works = [self._work_list(c) for c in self.eri_counts]
         arr = (ctypes.c_void_p * 6)(*[w.data_ptr() for w in works])
-        cnt = (ctypes.c_int * 6)(*[len(w) for w in self.eri_work])
+        cnt = (ctypes.c_int * 6)(*[w.shape[0] for w in works])
         tm("eri_s8")
         # stored over their multiplicity as they are written, the form the J/K
         # contraction takes

@@ -1414,6 +1414,67 @@ extern "C" int pscf_batch_eri_s8_div(int dtype, const void* d_mols, int nmol, co
                           1, stream);
 }
 
+// Work list of (owner, first index) pairs covering range(count_i) of every
+// owner i in steps of `size`: entry t belongs to the owner i with first[i] <=
+// t < first[i + 1], first being the exclusive prefix sum of the owners' chunk
+// counts ceil(count_i / size) (first[n] = total).
+__global__ void __launch_bounds__(256) chunks_kernel(const int64_t* __restrict__ first, int n, int size,
+                                                     int* __restrict__ work, int64_t total) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = n - 1;  // largest i with first[i] <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    work[2 * t] = lo;
+    work[2 * t + 1] = (int)((t - first[lo]) * size);
+  }
+}
+
+extern "C" int pscf_batch_chunks(const int64_t* d_first, int n, int size, int* d_work, int64_t total,
+                                 void* stream) {
+  PSCF_REQUIRE(n > 0 && size > 0 && total >= 0 && d_first && (total == 0 || d_work), "pscf_batch_chunks: bad arguments");
+  if (total == 0) return PSCF_OK;
+  const int64_t blocks = (total + 255) / 256;
+  chunks_kernel<<<(unsigned)(blocks < 8192 ? blocks : 8192), 256, 0, (hipStream_t)stream>>>(d_first, n, size, d_work,
+                                                                                             total);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
+// Nuclear repulsion of every molecule, one thread per molecule, atom pairs
+// i > j in order (nanoDFT.py: mol.energy_nuc()).
+__global__ void __launch_bounds__(256) enuc_kernel(const BatchMol* __restrict__ mols, int nmol,
+                                                   const int* __restrict__ atm, const double* __restrict__ env,
+                                                   double* __restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  const int a0 = (int)mols[m].atm0, na = (int)mols[m].natm;
+  double e = 0.0;
+  for (int i = 1; i < na; ++i) {
+    const double* Ri = env + atm[(a0 + i) * ATM_SLOTS + 1];
+    const double Zi = (double)atm[(a0 + i) * ATM_SLOTS];
+    for (int j = 0; j < i; ++j) {
+      const double* Rj = env + atm[(a0 + j) * ATM_SLOTS + 1];
+      const double dx = Ri[0] - Rj[0], dy = Ri[1] - Rj[1], dz = Ri[2] - Rj[2];
+      e += Zi * (double)atm[(a0 + j) * ATM_SLOTS] / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  out[m] = e;
+}
+
+extern "C" int pscf_batch_enuc(const void* d_mols, int nmol, const int* d_atm, const double* d_env, double* d_enuc,
+                               void* stream) {
+  PSCF_REQUIRE(nmol > 0 && d_mols && d_atm && d_env && d_enuc, "pscf_batch_enuc: bad arguments");
+  enuc_kernel<<<(nmol + 255) / 256, 256, 0, (hipStream_t)stream>>>((const BatchMol*)d_mols, nmol, d_atm, d_env,
+                                                                    d_enuc);
+  PSCF_HIP(hipGetLastError());
+  return PSCF_OK;
+}
+
 extern "C" int pscf_batch_int1e(const void* d_mols, int nmol, const int* d_atm, const int* d_bas, const double* d_env,
                                 const int* d_shell_ao, double* d_S, double* d_Hcore, void* stream) {
   PSCF_REQUIRE(nmol > 0 && d_mols && d_atm && d_bas && d_env && d_shell_ao && d_S && d_Hcore,

@@ -183,6 +183,28 @@ def _multiplicity(N):
     return 2.0 ** (pq_diag[a] + pq_diag[b] + (a == b))
 
 
+def test_batch_nuclear_repulsion(hip):
+    """pscf_batch_enuc equals the Coulomb sum over atom pairs of each molecule."""
+    mols = batch.parse(STRS + molgen.batched_set(24, (5, 9), seed=33))
+    b = batch.BatchRKS(mols, dtype="fp32")
+    b.setup()
+    got = b.d_enuc.cpu().numpy()
+    for m, (Z, R) in enumerate(mols):
+        ref = sum(Z[i] * Z[j] / np.linalg.norm(R[i] - R[j]) for i in range(len(Z)) for j in range(i))
+        assert abs(got[m] - ref) < 1e-10 * max(1.0, ref), m
+
+
+def test_batch_work_lists(hip):
+    """pscf_batch_chunks: (owner, first) pairs covering range(count) of every
+    owner in steps of 256, owners with zero count included, in owner order."""
+    rng = np.random.default_rng(4)
+    counts = rng.integers(0, 2000, size=300)
+    counts[[0, 7, 299]] = 0
+    got = batch.BatchRKS._work_list(counts).cpu().numpy()
+    ref = [(i, s) for i, c in enumerate(counts) for s in range(0, int(c), 256)]
+    assert np.array_equal(got, np.array(ref, np.int32).reshape(-1, 2))
+
+
 def test_batch_eri_scale_matches_oracle(hip):
     """The batch ERI buffer holds the oracle's s8 ERIs over their multiplicity."""
     from oracle import cint
@@ -201,9 +223,9 @@ def test_batch_eri_scale_matches_oracle(hip):
     import ctypes
     import torch
     raw = torch.empty_like(b.eri)
-    works = [device.to_dev(w) for w in b.eri_work]
+    works = [b._work_list(c) for c in b.eri_counts]
     arr = (ctypes.c_void_p * 6)(*[w.data_ptr() for w in works])
-    cnt = (ctypes.c_int * 6)(*[len(w) for w in b.eri_work])
+    cnt = (ctypes.c_int * 6)(*[w.shape[0] for w in works])
     P, st = device.ptr, device.stream()
     _lib.call("pscf_batch_eri_s8", b.code, P(b.d_desc), b.M, arr, cnt, P(b.d_atm), P(b.d_bas), P(b.d_env),
               P(b.d_shell_ao), P(b.d_shl), P(raw), st)
