@@ -176,6 +176,16 @@ __device__ inline void shell_quartet_np(const ShellD& A, const ShellD& B, const 
       const T aj = (T)B.e[pb];
       const T aij = ai + aj, r_aij = drcp<T>(aij);
       const T kab = dexp<T>(-ai * aj * r_aij * rab2) * ca * (T)B.c[pb];
+      // FP32: a bra primitive pair whose Gaussian-product prefactor is below
+      // the bound skips its ket loop.  What it would add to an ERI is at most
+      // 34.99 r_aij |kab| (akl^-3/2 <= 5.1) (sum of |kcd| over the ket
+      // primitives, <= 9 x 15^2 in STO-3G) angc (<= 0.06) times the 1D-integral
+      // products (< 1e5 for molecules under 20 bohr): below 1e-10, far under
+      // the FP32 rounding of the ERIs themselves (~1e-7 of the largest).  The
+      // lanes of a wave mostly share their bra pair, so whole waves skip
+      // together.
+      if constexpr (sizeof(T) == 4)
+        if (fabsf(kab) * r_aij < 4e-20f) continue;
       T PA[3], P[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
