@@ -129,9 +129,11 @@ __device__ constexpr int cexp(int f, int d) {
 // unroll and the ket-pair data (exponent sum, its reciprocal, the Gaussian
 // product prefactor with its exp) is computed once per contracted quartet
 // instead of once per bra primitive pair; NPC = 0 reads the counts at run time.
-template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL, int NPC>
+// SCREEN (FP32, quartets of four 3-primitive shells): bra primitive pairs with
+// a negligible Gaussian-product prefactor skip their ket loop (bound below).
+template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL, int NPC, bool SCREEN>
 __device__ inline void shell_quartet_np(const ShellD& A, const ShellD& B, const ShellD& C, const ShellD& D,
-                                        const BT* __restrict__ tab, T* acc) {
+                                        const BT* __restrict__ tab, T* acc, bool screen) {
   constexpr int NR = (LI + LJ + LK + LL) / 2 + 1;
   constexpr int NMAX = LI + LJ, MMAX = LK + LL;
   constexpr int NI = NComp<LI>::v, NJ = NComp<LJ>::v, NK = NComp<LK>::v, NL = NComp<LL>::v;
@@ -176,16 +178,16 @@ __device__ inline void shell_quartet_np(const ShellD& A, const ShellD& B, const 
       const T aj = (T)B.e[pb];
       const T aij = ai + aj, r_aij = drcp<T>(aij);
       const T kab = dexp<T>(-ai * aj * r_aij * rab2) * ca * (T)B.c[pb];
-      // FP32: a bra primitive pair whose Gaussian-product prefactor is below
+      // SCREEN: a bra primitive pair whose Gaussian-product prefactor is below
       // the bound skips its ket loop.  What it would add to an ERI is at most
-      // 34.99 r_aij |kab| (akl^-3/2 <= 5.1) (sum of |kcd| over the ket
-      // primitives, <= 9 x 15^2 in STO-3G) angc (<= 0.06) times the 1D-integral
-      // products (< 1e5 for molecules under 20 bohr): below 1e-10, far under
-      // the FP32 rounding of the ERIs themselves (~1e-7 of the largest).  The
-      // lanes of a wave mostly share their bra pair, so whole waves skip
-      // together.
-      if constexpr (sizeof(T) == 4)
-        if (fabsf(kab) * r_aij < 4e-20f) continue;
+      // 34.99 r_aij |kab| (akl^-3/2 <= 5.1) (sum of |kcd| over the 3 x 3 ket
+      // primitive pairs, <= 9 x 15^2 for STO-3G shells) angc (<= 0.06) times
+      // the 1D-integral products (< 1e5 for molecules under 20 bohr): below
+      // 1e-10, far under the FP32 rounding of the ERIs themselves (~1e-7 of the
+      // largest).  The lanes of a wave mostly share their bra pair, so whole
+      // waves skip together.
+      if constexpr (SCREEN && sizeof(T) == 4)
+        if (screen && fabsf(kab) * r_aij < 4e-20f) continue;
       T PA[3], P[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -303,14 +305,17 @@ __device__ inline void shell_quartet_np(const ShellD& A, const ShellD& B, const 
 // (ps|ss), the most numerous) four shells with 3 primitives (every STO-3G
 // shell) take the unrolled, ket-hoisted form; the multi-root classes and FP64
 // keep rolled loops (unrolled, they need more registers than a lane has).
-template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL>
+// SCREEN (the batched STO-3G form) enables the primitive-pair screening of
+// shell_quartet_np on quartets of four 3-primitive shells.
+template <typename T, typename BT, int NTERMS, int LI, int LJ, int LK, int LL, bool SCREEN = false>
 __device__ inline void shell_quartet(const ShellD& A, const ShellD& B, const ShellD& C, const ShellD& D,
                                      const BT* __restrict__ tab, T* acc) {
   constexpr bool ONE_ROOT = LI + LJ + LK + LL <= 1 && sizeof(T) == 4;  // FP32 (FP64 would need 2x the registers)
-  if (ONE_ROOT && A.np == 3 && B.np == 3 && C.np == 3 && D.np == 3)
-    shell_quartet_np<T, BT, NTERMS, LI, LJ, LK, LL, ONE_ROOT ? 3 : 0>(A, B, C, D, tab, acc);
+  const bool np3 = A.np == 3 && B.np == 3 && C.np == 3 && D.np == 3;
+  if (ONE_ROOT && np3)
+    shell_quartet_np<T, BT, NTERMS, LI, LJ, LK, LL, ONE_ROOT ? 3 : 0, SCREEN>(A, B, C, D, tab, acc, true);
   else
-    shell_quartet_np<T, BT, NTERMS, LI, LJ, LK, LL, 0>(A, B, C, D, tab, acc);
+    shell_quartet_np<T, BT, NTERMS, LI, LJ, LK, LL, 0, SCREEN>(A, B, C, D, tab, acc, np3);
 }
 
 template <typename BT>
@@ -750,7 +755,7 @@ __global__ void __launch_bounds__(256) eri_s8_batch_kernel(const BatchMol* __res
   const ShellD C = load_shell(atm, bas, env, shell_ao, k), D = load_shell(atm, bas, env, shell_ao, l);
   constexpr int NI = NComp<LI>::v, NJ = NComp<LJ>::v, NK = NComp<LK>::v, NL = NComp<LL>::v;
   T acc[NI * NJ * NK * NL];
-  shell_quartet<T, BT, BoysTerms<BT>::v, LI, LJ, LK, LL>(A, B, C, D, tab, acc);
+  shell_quartet<T, BT, BoysTerms<BT>::v, LI, LJ, LK, LL, true>(A, B, C, D, tab, acc);
   const bool diag_bra = (i == j), diag_ket = (k == l), same_pair = tri_launch && (a == b);
   T* dst = out + M.eri0;
 #pragma unroll

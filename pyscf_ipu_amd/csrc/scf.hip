@@ -721,6 +721,150 @@ __device__ inline int sturm_count_all(const double* __restrict__ d, const double
   return c;
 }
 
+// sturm_count_all for a matrix without splits (every slot's block is [0, N)):
+// the full chunks of 8 positions count every sign change, only the last
+// partial chunk masks the positions from N on.
+__device__ inline int sturm_count_whole(const double* __restrict__ d, const double* __restrict__ e2m1, int N,
+                                        double x) {
+  double p = 1.0, pm = 0.0;
+  int sp = 0, c = 0;
+  auto chunk = [&](int i0, bool tail) {
+    double dd[8], ee[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dd[u] = d[i0 + u] - x;
+      ee[u] = e2m1[i0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double pn = dd[u] * p - ee[u] * pm;
+      pm = p;
+      p = pn;
+      const int sg = __double2hiint(pn) >> 31;
+      c -= tail ? ((sg ^ sp) & ((i0 + u - N) >> 31)) : (sg ^ sp);
+      sp = sg;
+    }
+    const double mx = fmax(fabs(p), fabs(pm));
+    const int ex = (mx > 0x1p200 || mx < 0x1p-200) ? __builtin_amdgcn_frexp_exp(mx) : 0;
+    p = ldexp(p, -ex);
+    pm = ldexp(pm, -ex);
+  };
+  int i0 = 0;
+  for (; i0 + 8 <= N; i0 += 8) chunk(i0, false);
+  if (i0 < N) chunk(i0, true);
+  return c;
+}
+
+// Eigenvectors of all slots (lane = slot, eigenvalue lk, column col of Z) of
+// a tridiagonal matrix without splits, by the twisted factorisation of
+// twisted_slot with every block [0, N).  Branch-free: the positions run in
+// chunks of 8 whose d, e and pivot values are loaded before the recurrence
+// over them (only the last chunk checks i < N), a pivot below tiny in
+// magnitude becomes +-tiny by fmax / copysign, and stores of inactive lanes
+// (or of positions a lane does not own) go to the one-double sink.  The
+// vectors are stored unnormalised; scale[col] receives 1 / |z|.  tiny is eps
+// max_i (|d_i| + |e_i|) + 1e-300.  d and e have 64 entries.
+__device__ inline void twisted_whole(const double* __restrict__ d, const double* __restrict__ e, int N, int ld,
+                                     double tiny, double lk, bool act, int col, double* __restrict__ rdp,
+                                     double* __restrict__ rdm, double* __restrict__ Z, double* __restrict__ scale,
+                                     double* __restrict__ sink) {
+  auto guard = [&](double v) { return copysign(fmax(fabs(v), tiny), v); };
+  double r_prev = 0.0;
+  auto down = [&](int i0, bool tail) {  // top-down pivots of positions i0 .. i0 + 7
+    double dd[8], em[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dd[u] = d[i0 + u] - lk;
+      em[u] = i0 + u > 0 ? e[i0 + u - 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (tail && i0 + u >= N) break;
+      r_prev = recip_nr(guard(dd[u] - em[u] * em[u] * r_prev));
+      *(act ? rdp + i0 + u : sink) = r_prev;
+    }
+  };
+  int i0 = 0;
+  for (; i0 + 8 <= N; i0 += 8) down(i0, false);
+  if (i0 < N) down(i0, true);
+  double rnext = 0.0, gbest = 1.0 / 0.0;
+  int r = N - 1;
+  auto up = [&](int i0, bool tail) {  // bottom-up pivots and the twist, positions i0, i0 - 1, .. i0 - 7
+    double dd[8], ei[8], em[8], rp[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 - u;
+      const int ic = i > 0 ? i : 0;
+      dd[u] = d[ic] - lk;
+      ei[u] = i < N - 1 ? e[ic] : 0.0;
+      em[u] = i > 0 ? e[ic - 1] : 0.0;
+      rp[u] = i > 0 ? rdp[ic - 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 - u;
+      if (tail && i < 0) break;
+      const double dm = guard(dd[u] - ei[u] * ei[u] * rnext);
+      rnext = recip_nr(dm);
+      *(act ? rdm + i : sink) = rnext;
+      const double gm = fabs(i > 0 ? dm - em[u] * em[u] * rp[u] : dm);
+      r = gm < gbest ? i : r;
+      gbest = fmin(gm, gbest);
+    }
+  };
+  i0 = N - 1;
+  for (; i0 >= 7; i0 -= 8) up(i0, false);
+  if (i0 >= 0) up(i0, true);
+  *(act ? Z + r * ld + col : sink) = 1.0;
+  double nrm = 1.0, z = 1.0;
+  auto above = [&](int i0, bool tail) {  // z at positions i0, i0 - 1, .. below r
+    double ee[8], rr[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int ic = i0 - u > 0 ? i0 - u : 0;
+      ee[u] = e[ic];
+      rr[u] = rdp[ic];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 - u;
+      if (tail && i < 0) break;
+      const bool own = act && i < r;
+      const double zn = -ee[u] * rr[u] * z;
+      z = own ? zn : z;
+      nrm += own ? zn * zn : 0.0;
+      *(own ? Z + i * ld + col : sink) = zn;
+    }
+  };
+  i0 = N - 2;
+  for (; i0 >= 7; i0 -= 8) above(i0, false);
+  if (i0 >= 0) above(i0, true);
+  z = 1.0;
+  auto below = [&](int i0, bool tail) {  // z at positions i0, i0 + 1, .. above r
+    double ee[8], rr[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int ic = i0 + u < N ? i0 + u : N - 1;
+      ee[u] = e[ic - 1];
+      rr[u] = rdm[ic];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u;
+      if (tail && i >= N) break;
+      const bool own = act && i > r;
+      const double zn = -ee[u] * rr[u] * z;
+      z = own ? zn : z;
+      nrm += own ? zn * zn : 0.0;
+      *(own ? Z + i * ld + col : sink) = zn;
+    }
+  };
+  i0 = 1;
+  for (; i0 + 8 <= N; i0 += 8) below(i0, false);
+  if (i0 < N) below(i0, true);
+  if (act) scale[col] = 1.0 / sqrt(nrm);
+}
+
 // Eigenvector of one slot (lane) of the tridiagonal matrix (d, e in LDS) for
 // the eigenvalue lk of its block [a0, a1), by the twisted factorisation:
 // top-down pivots Dp, bottom-up pivots Dm, the twist r where |gamma_i| = |Dm_i
@@ -811,99 +955,116 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   int* blo = cnt + 16 * 64;
   int* bhi = blo + 64;
   int* fail = bhi + 64;
-  // Q accumulates Q0 H_1 H_2 ... (Q0 or the identity to start) in Z during
-  // the reduction, so that no back-transformation pass is needed
-  for (int x = tid; x < N * ld; x += blockDim.x) {
-    const int i = x / ld, j = x - i * ld;
-    Z[x] = j < N ? (T.Q0 ? T.Q0[i * N + j] : (i == j ? 1.0 : 0.0)) : 0.0;
+  // 1. tridiagonalisation, register-resident in the first 512 threads (two
+  // wavefronts per SIMD): thread (row ri = tid / 8, g = tid % 8) holds A[ri][c]
+  // and Q[ri][c] for its eight columns c = g + 8 u (zero outside N x N), Q =
+  // Q0 (or the identity) to start.  Step k reflects column k below the
+  // diagonal, x = A[k+1..N-1][k], onto e_(k+1): alpha = -sign(x_0) |x|, u = x
+  // - alpha e_(k+1) (zero at positions <= k), H = I - tau u u^T with tau =
+  // 2 / u^T u = 1 / (|x| (|x| + |x_0|)).  Then p = tau A u, K = tau (u^T p) / 2,
+  // w = p - K u, A <- A - u w^T - w u^T (the two-sided H A H, applied to the
+  // whole matrix) and Q <- Q - tau (Q u) u^T, so that Q = Q0 H_0 H_1 ... needs
+  // no back-transformation pass.  The 8 lanes of a row hold all 64 positions
+  // between them, so row sums (DPP, no LDS) give A u, Q u, u^T p and |x|^2.
+  // Two barriers a step: p into LDS; the updated column k + 1 into LDS, from
+  // which every wavefront forms the next reflector itself (identical
+  // operations on identical values in every row, so every row holds the same
+  // u and tau).
+  const bool red = tid < 512;
+  const int ri = (tid >> 3) & 63, g = tid & 7;
+  double ar[8], qr[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int c = g + 8 * u;
+    const bool in = red && ri < N && c < N;
+    ar[u] = in ? A[ri * ld + c] : 0.0;
+    qr[u] = in ? (T.Q0 ? T.Q0[ri * N + c] : (ri == c ? 1.0 : 0.0)) : 0.0;
   }
-  // 1. tridiagonalisation: H_k = I - 2 v v^T (|v| = 1) zeroes column k below
-  // the subdiagonal; with p = 2 A v and w = p - (v^T p) v the two-sided update
-  // is A <- A - v w^T - w v^T, and Q <- Q - (2 Q v) v^T.  Two barriers a step:
-  // the products p, Q v; then the updates, during which wave 0 updates the
-  // next column itself and forms the next reflector from it.
-  auto householder = [&](int k) {  // wave 0: v into column k below the diagonal, e_k, d_k
-    const int m = N - k - 1;
-    const double x = lane < m ? A[(k + 1 + lane) * ld + k] : 0.0;
-    const double ss = wave_sum(x * x);
-    const double x0 = readlane_f64(x, 0);
-    double alpha = 0.0, v = 0.0;
-    if (ss > 0.0) {
-      const double nx = sqrt(ss);
-      alpha = x0 >= 0.0 ? -nx : nx;
-      const double v0 = x0 - alpha;
-      v = (lane == 0 ? v0 : x) / sqrt(ss - x0 * x0 + v0 * v0);
-    }
-    if (lane < m) A[(k + 1 + lane) * ld + k] = v;
-    if (lane == 0) {
-      e[k] = alpha;
-      d[k] = A[k * ld + k];
-    }
+  auto row8_sum = [](double v) {
+    v = dpp_add<0xb1, 0xf>(v);
+    v = dpp_add<0x4e, 0xf>(v);
+    return dpp_add<0x141, 0xf>(v);
   };
-  if (wave == 0) {
-    if (lane == 0) *fail = 0;
-    if (N >= 3) householder(0);
-  }
+  double* col = lo;  // column k of the current A (64 rows)
+  double* pb = wv;   // p (64 rows)
+  if (red && g == 0) col[ri] = ar[0];
+  if (tid == 0) *fail = 0;
   __syncthreads();
   for (int k = 0; k + 2 < N; ++k) {
-    const int m = N - k - 1;
-    {
-      const int i = tid >> 4, g = tid & 15;  // row i, 16 lanes per row
-      double s = 0.0, y = 0.0;
-      if (i < m)
-        for (int j = g; j < m; j += 16) s += A[(k + 1 + i) * ld + k + 1 + j] * A[(k + 1 + j) * ld + k];
-      if (i < N)
-        for (int j = g; j < m; j += 16) y += Z[i * ld + k + 1 + j] * A[(k + 1 + j) * ld + k];
-      s = row16_sum(s);
-      y = row16_sum(y);
-      const double p = 2.0 * s;
-      const double vi = i < m ? A[(k + 1 + i) * ld + k] : 0.0;
-      const double kp = wave_sum((g == 0 && i < m) ? vi * p : 0.0);
-      if (lane == 0) kpart[wave] = kp;
-      if (g == 0 && i < m) wv[i] = p;
-      if (g == 0 && i < N) dot[i] = 2.0 * y;
+    double uj[8], ui = 0.0, tau = 0.0, p = 0.0;
+    if (red) {  // reflector of column k from col
+      double ss = 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = g + 8 * u;
+        uj[u] = (c > k && c < N) ? col[c] : 0.0;
+        ss = fma(uj[u], uj[u], ss);
+      }
+      ss = row8_sum(ss);
+      const double ui0 = (ri > k && ri < N) ? col[ri] : 0.0;
+      const double x0 = col[k + 1];
+      double alpha = 0.0;
+      if (ss > 0.0) {
+        const double nx = sqrt(ss);
+        alpha = x0 >= 0.0 ? -nx : nx;
+        const double v0 = x0 - alpha;
+        tau = 1.0 / (nx * (nx + fabs(x0)));
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (g + 8 * u == k + 1) uj[u] = v0;
+        ui = ri == k + 1 ? v0 : ui0;
+      }
+      if (tid == 0) e[k] = alpha;
+      // p = tau A u and Q <- Q - tau (Q u) u^T
+      double sa = 0.0, sq = 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sa = fma(ar[u], uj[u], sa);
+        sq = fma(qr[u], uj[u], sq);
+      }
+      p = tau * row8_sum(sa);
+      const double y = tau * row8_sum(sq);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) qr[u] = fma(-y, uj[u], qr[u]);
+      if (g == 0) pb[ri] = p;
     }
     __syncthreads();
-    double K = 0.0;
+    if (red) {  // K = tau (u^T p) / 2, w = p - K u, A <- A - u w^T - w u^T
+      double pj[8], kv = 0.0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) K += kpart[q];
-    if (wave == 0) {  // column k + 1 (trailing column 0), then the next reflector
-      if (lane < m) {
-        const double va = A[(k + 1 + lane) * ld + k], v0 = A[(k + 1) * ld + k];
-        A[(k + 1 + lane) * ld + k + 1] -= va * (wv[0] - K * v0) + (wv[lane] - K * va) * v0;
+      for (int u = 0; u < 8; ++u) {
+        pj[u] = pb[g + 8 * u];
+        kv = fma(uj[u], pj[u], kv);
       }
-      if (k + 3 < N) {
-        wave_sync();
-        householder(k + 1);
-      }
-    } else {  // the trailing block past column k + 1, and Q
-      const int t = tid - 64, nt = blockDim.x - 64, mb = m - 1;
-      for (int x = t; x < mb * mb; x += nt) {
-        const int i = x / mb + 1, j = x - (i - 1) * mb + 1;
-        const double vi = A[(k + 1 + i) * ld + k], vj = A[(k + 1 + j) * ld + k];
-        A[(k + 1 + i) * ld + k + 1 + j] -= vi * (wv[j] - K * vj) + (wv[i] - K * vi) * vj;
-      }
-      for (int x = t; x < N * m; x += nt) {
-        const int i = x / m, j = x - i * m;
-        Z[i * ld + k + 1 + j] -= dot[i] * A[(k + 1 + j) * ld + k];
-      }
+      const double K = 0.5 * tau * row8_sum(kv);
+      const double wi = fma(-K, ui, p);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) ar[u] -= fma(ui, fma(-K, uj[u], pj[u]), wi * uj[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)  // constant indices keep ar in registers
+        if (g + 8 * u == k + 1) col[ri] = ar[u];
     }
     __syncthreads();
   }
-  // 2. last entries, splits and the block of every position
-  if (wave == 0) {
-    if (lane == 0) {
-      if (N >= 2) {
-        d[N - 2] = A[(N - 2) * ld + N - 2];
-        e[N - 2] = A[(N - 1) * ld + N - 2];
-      }
-      d[N - 1] = A[(N - 1) * ld + N - 1];
-      *fail = 0;
+  // diagonal (final once its step is past), the last off-diagonal, and Q into
+  // A (the input is no longer needed); Z is free for the eigenvectors of the
+  // tridiagonal matrix
+  if (red) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = g + 8 * u;
+      if (ri < N && c == ri) d[ri] = ar[u];
+      if (ri == N - 1 && c == N - 2) e[N - 2] = ar[u];
+      if (ri < N && c < N) A[ri * ld + c] = qr[u];
     }
-    wave_sync();
+  }
+  __syncthreads();
+  // 2. splits and the block of every position
+  if (wave == 0) {
     const bool sp = lane < N - 1 && fabs(e[lane]) <= 1e-12 * (fabs(d[lane]) + fabs(d[lane + 1]));
     if (sp) e[lane] = 0.0;
     const unsigned long long mask = __ballot(sp);  // bit i: split after position i
+    if (lane == 0) fail[1] = mask == 0ull ? 1 : 0;
     if (lane < N) {
       const unsigned long long below = lane > 0 ? (mask & ((1ull << lane) - 1)) : 0ull;
       blo[lane] = below ? 64 - __clzll(below) : 0;   // highest split below lane, + 1
@@ -940,17 +1101,15 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
     wv[lane] = (lane > 0 && lane < N) ? e[lane - 1] * e[lane - 1] : 0.0;  // e_(i-1)^2, 0 at splits
   }
   __syncthreads();
-  // the accumulated Q moves to A (its reflectors are no longer needed), Z is
-  // free for the eigenvectors of the tridiagonal matrix
-  for (int x = tid; x < N * ld; x += blockDim.x) A[x] = Z[x];
   // 3. multisection: wave q < TRI_SECT evaluates section point q of every
   // slot (lane), TRI_ROUNDS rounds narrow each interval (TRI_SECT + 1)-fold
   {
     const int b0 = blo[lane], b1 = bhi[lane];
+    const bool whole = fail[1] != 0;  // no splits
     for (int round = 0; round < TRI_ROUNDS; ++round) {
       if (wave < TRI_SECT) {
         const double x = lo[lane] + (wave + 1) * ((hi[lane] - lo[lane]) * (1.0 / (TRI_SECT + 1)));
-        cnt[wave * 64 + lane] = sturm_count_all(d, wv, N, b0, b1, x);
+        cnt[wave * 64 + lane] = whole ? sturm_count_whole(d, wv, N, x) : sturm_count_all(d, wv, N, b0, b1, x);
       }
       __syncthreads();
       if (wave == 0 && lane < N) {
@@ -991,6 +1150,7 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
     // all slots in one pass (lane = slot): 1 / Dp in W, 1 / Dm in W2 at the
     // odd stride ld.  Without splits every slot's block is [0, N) and the
     // loops run unpredicated.
+    const bool whole = fail[1] != 0;  // no splits
     if (wave == 0) {
       const bool act = lane < N;
       const int a0 = act ? blo[lane] : 0, a1 = act ? bhi[lane] : 0;
@@ -998,12 +1158,22 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
       const int col = act ? T.rank[lane] : 0;
       double* rdp = W + (act ? lane : 0) * ld;
       double* rdm = T.W2 + (act ? lane : 0) * ld;
-      if (__ballot(act && (a0 != 0 || a1 != N)) == 0ull)
-        twisted_slot<false>(d, e, N, ld, 0, N, lk, act, col, rdp, rdm, Z);
-      else
+      if (whole) {
+        const double sc = wave_max(act ? fabs(d[lane]) + (lane < N - 1 ? fabs(e[lane]) : 0.0) : 0.0);
+        twisted_whole(d, e, N, ld, 2.220446049250313e-16 * sc + 1e-300, lk, act, col, rdp, rdm, Z, dot,
+                      kpart);
+      } else {
         twisted_slot<true>(d, e, N, ld, a0, a1, lk, act, col, rdp, rdm, Z);
+      }
     }
     __syncthreads();
+    if (whole) {  // normalise the columns
+      for (int x = tid; x < N * N; x += blockDim.x) {
+        const int i = x / N, c = x - i * N;
+        Z[i * ld + c] *= dot[c];
+      }
+      __syncthreads();
+    }
   }
   // slots per pass, each with N doubles of 1 / Dp and of 1 / Dm at an odd
   // stride (distinct LDS banks across the lanes)
@@ -1284,9 +1454,13 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready) {
     // corrections E (in place of S); near-degenerate pairs are only counted
     // here (their S entries stay for the second pass)
     double off = 0.0, tot = 0.0, flags = 0.0, rmax = 0.0, emax = 0.0;
-    for (int x = tid; x < N * N; x += blockDim.x) {
-      const int i = x / N, j = x - i * N;
-      if (i > j) continue;
+    // pairs i <= j folded into lines of N + 1: line k holds row k from the
+    // diagonal (N - k pairs) then row N - 1 - k from the diagonal (k + 1
+    // pairs); for odd N the middle row is a last, shorter line
+    const float inv_line = 1.0f / (float)(N + 1);
+    for (int x = tid; x < N * (N + 1) / 2; x += blockDim.x) {
+      const int k = (int)(((float)x + 0.5f) * inv_line), c = x - k * (N + 1);
+      const int i = c < N - k ? k : N - 1 - k, j = c < N - k ? k + c : c - 1;
       const double sij = S[i * ld + j];
       if (i == j) {
         const double rii = 1.0 - G[i * ld + i];
