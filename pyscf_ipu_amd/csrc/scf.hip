@@ -649,10 +649,10 @@ __device__ inline ScfLds scf_lds_carve(unsigned char* smem, int N) {
 // Symmetric eigen-decomposition for N <= 64 by 1024 threads, in place of the
 // Jacobi sweeps: a fixed number of barrier phases instead of a number of
 // sweeps that depends on how far A is from diagonal.
-//   1. Householder tridiagonalisation T = Q^T A Q: step k reflects column k
-//      below the diagonal onto e_(k+1); its unit vector v (H = I - 2 v v^T) is
-//      kept in that column of A.  p = 2 A22 v by 16 lanes per row, K = v.p,
-//      w = p - K v, A22 -= v w^T + w v^T.
+//   1. Householder tridiagonalisation T = Q^T A Q with A and Q held in the
+//      registers of 512 threads (8 per row): step k reflects column k below
+//      the diagonal onto e_(k+1) (H = I - tau u u^T), p = tau A u, K = tau
+//      u.p / 2, w = p - K u, A -= u w^T + w u^T, Q -= tau (Q u) u^T.
 //   2. Off-diagonals below 1e-12 (|d_i| + |d_i+1|) are set to zero: T falls
 //      into unreduced blocks, so eigenvalues that are degenerate up to
 //      rounding come from different blocks.
@@ -1132,15 +1132,17 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
     }
   }
   // 4. ascending order of the slots, then the eigenvectors
-  if (wave == 0 && lane < N) {
-    const double lk = 0.5 * (lo[lane] + hi[lane]);
+  if (wave == 0) {  // slot q's eigenvalue from lane q by readlane
+    const double lk = lane < N ? 0.5 * (lo[lane] + hi[lane]) : 0.0;
     int r = 0;
     for (int q = 0; q < N; ++q) {
-      const double lq = 0.5 * (lo[q] + hi[q]);
+      const double lq = readlane_f64(lk, q);
       r += (lq < lk) || (lq == lk && q < lane);
     }
-    T.rank[lane] = r;
-    lam[lane] = lk;
+    if (lane < N) {
+      T.rank[lane] = r;
+      lam[lane] = lk;
+    }
   }
   __syncthreads();
   if (wave == 0 && lane < N) T.eig[T.rank[lane]] = lam[lane];
@@ -1237,11 +1239,12 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   }
   // 5. clusters: modified Gram-Schmidt in ascending order (one wave, lane = row)
   if (wave == 0) {
-    double sc = 0.0;
-    for (int i = 0; i < N; ++i) sc = fmax(sc, fabs(d[i]) + (i < N - 1 ? fabs(e[i]) : 0.0));
+    const double sc = wave_max(lane < N ? fabs(d[lane]) + (lane < N - 1 ? fabs(e[lane]) : 0.0) : 0.0);
     const double tau = 1e-6 * sc;
+    // columns closer than tau to their predecessor (none: nothing to do)
+    const bool any = __ballot(lane >= 1 && lane < N && !(T.eig[lane] - T.eig[lane - 1] >= tau)) != 0ull;
     int c0 = 0;
-    for (int c = 1; c < N; ++c) {
+    for (int c = 1; any && c < N; ++c) {
       if (T.eig[c] - T.eig[c - 1] >= tau) {
         c0 = c;
         continue;
