@@ -993,14 +993,14 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   for (int k = 0; k + 2 < N; ++k) {
     double uj[8], ui = 0.0, tau = 0.0, p = 0.0;
     if (red) {  // reflector of column k from col
-      double ss = 0.0;
+      double s2[2] = {0.0, 0.0};  // two partial sums: shorter dependency chains
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int c = g + 8 * u;
         uj[u] = (c > k && c < N) ? col[c] : 0.0;
-        ss = fma(uj[u], uj[u], ss);
+        s2[u & 1] = fma(uj[u], uj[u], s2[u & 1]);
       }
-      ss = row8_sum(ss);
+      const double ss = row8_sum(s2[0] + s2[1]);
       const double ui0 = (ri > k && ri < N) ? col[ri] : 0.0;
       const double x0 = col[k + 1];
       double alpha = 0.0;
@@ -1008,7 +1008,7 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
         const double nx = sqrt(ss);
         alpha = x0 >= 0.0 ? -nx : nx;
         const double v0 = x0 - alpha;
-        tau = 1.0 / (nx * (nx + fabs(x0)));
+        tau = recip_nr(nx * (nx + fabs(x0)));
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (g + 8 * u == k + 1) uj[u] = v0;
@@ -1016,27 +1016,27 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
       }
       if (tid == 0) e[k] = alpha;
       // p = tau A u and Q <- Q - tau (Q u) u^T
-      double sa = 0.0, sq = 0.0;
+      double sa[2] = {0.0, 0.0}, sq[2] = {0.0, 0.0};
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        sa = fma(ar[u], uj[u], sa);
-        sq = fma(qr[u], uj[u], sq);
+        sa[u & 1] = fma(ar[u], uj[u], sa[u & 1]);
+        sq[u & 1] = fma(qr[u], uj[u], sq[u & 1]);
       }
-      p = tau * row8_sum(sa);
-      const double y = tau * row8_sum(sq);
+      p = tau * row8_sum(sa[0] + sa[1]);
+      const double y = tau * row8_sum(sq[0] + sq[1]);
 #pragma unroll
       for (int u = 0; u < 8; ++u) qr[u] = fma(-y, uj[u], qr[u]);
       if (g == 0) pb[ri] = p;
     }
     __syncthreads();
     if (red) {  // K = tau (u^T p) / 2, w = p - K u, A <- A - u w^T - w u^T
-      double pj[8], kv = 0.0;
+      double pj[8], kv[2] = {0.0, 0.0};
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         pj[u] = pb[g + 8 * u];
-        kv = fma(uj[u], pj[u], kv);
+        kv[u & 1] = fma(uj[u], pj[u], kv[u & 1]);
       }
-      const double K = 0.5 * tau * row8_sum(kv);
+      const double K = 0.5 * tau * row8_sum(kv[0] + kv[1]);
       const double wi = fma(-K, ui, p);
 #pragma unroll
       for (int u = 0; u < 8; ++u) ar[u] -= fma(ui, fma(-K, uj[u], pj[u]), wi * uj[u]);
