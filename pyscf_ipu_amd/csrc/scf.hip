@@ -1115,16 +1115,12 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
       if (wave == 0 && lane < N) {
         const int j = lane - b0;
         const double a = lo[lane], w = (hi[lane] - a) * (1.0 / (TRI_SECT + 1));
-        double na = a, nb = hi[lane];
-        for (int q = 0; q < TRI_SECT; ++q) {
-          const double x = a + (q + 1) * w;
-          if (cnt[q * 64 + lane] <= j) {
-            na = x;
-          } else {
-            nb = x;
-            break;
-          }
-        }
+        // the first section point whose count exceeds j (branch-free)
+        int first = TRI_SECT;
+#pragma unroll
+        for (int q = TRI_SECT - 1; q >= 0; --q) first = cnt[q * 64 + lane] > j ? q : first;
+        const double na = first > 0 ? a + first * w : a;
+        const double nb = first < TRI_SECT ? a + (first + 1) * w : hi[lane];
         lo[lane] = na;
         hi[lane] = nb;
       }
