@@ -21,7 +21,8 @@ constexpr int SCF_THREADS = 1024;
 constexpr int DIIS_N = 9;
 constexpr int MAX_SWEEPS = 30;
 
-// state layout (doubles): Vhist[9][N*N] | Hhist[9][N*N] | B[10][10] | Uprev[N*N] | flags[2]
+// state layout (doubles): Vhist[9][N*N] | Hhist[9][N*N] | B[10][10] | Wprev[N*N] | flags[2]
+// (Wprev: the previous eigenvectors, C = L^-T U for N <= 64 with 1024 threads, else U)
 __host__ __device__ inline size_t state_doubles(int N) {
   return (size_t)2 * DIIS_N * N * N + 100 + (size_t)N * N + 2;
 }
@@ -1278,24 +1279,27 @@ __device__ inline bool tri_eigh(const TriBufs& T, int N) {
   return *fail == 0;
 }
 
-// From the symmetric matrix in the orthonormal basis (L.A) and the initial
-// rotation (L.U, identity or a warm start): Jacobi (eigenvectors optionally
-// stored to Uout), ascending ranks of the eigenvalues, C = L^-T U (L^-1 in
-// global Linv) left in L.A with columns unsorted, and C_occ C_occ^T (= D / 2)
-// left in L.U (leading dimension ld).
-__device__ __forceinline__ int eig_density(const ScfLds& L, int N, int nocc, const double* __restrict__ Linv,
-                                  double* __restrict__ Uout, bool solved = false) {
+// From the symmetric matrix (L.A) and the initial rotation (L.U, identity or
+// a warm start): Jacobi, ascending ranks of the eigenvalues, the eigenvectors
+// in the AO basis left in L.A with columns unsorted -- C = L^-T U (Cb = L^-1,
+// in the orthonormal basis) or C = C_prev U (cbasis: Cb = C_prev, in the
+// previous step's eigenbasis) -- and C_occ C_occ^T (= D / 2) left in L.U
+// (leading dimension ld).  Wout (the next step's warm start) receives U
+// (store_c false) or C (store_c true).
+__device__ __forceinline__ int eig_density(const ScfLds& L, int N, int nocc, const double* __restrict__ Cb,
+                                  double* __restrict__ Wout, bool solved = false, bool store_c = false,
+                                  bool cbasis = false) {
   const int ld = lds_ld(N), tid = threadIdx.x;
   double *A = L.A, *U = L.U, *B = L.B;
   const int sweeps = solved ? 0 : parallel_jacobi(A, U, N, ld, L.rp, L.rq, L.rc, L.rs, L.red);
-  if (Uout)  // eigenvectors in the orthonormal basis (next step's warm start)
+  if (Wout && !store_c)  // eigenvectors in the orthonormal basis
     for (int x = tid; x < N * N; x += blockDim.x) {
       const int i = x / N;
-      Uout[x] = U[i * ld + x - i * N];
+      Wout[x] = U[i * ld + x - i * N];
     }
   if (!solved)
     for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * ld + j];
-  lds_load(B, Linv, N, ld);
+  lds_load(B, Cb, N, ld);
   __syncthreads();
   for (int j = tid; j < N; j += blockDim.x) {
     const double lj = L.eig[j];
@@ -1306,8 +1310,16 @@ __device__ __forceinline__ int eig_density(const ScfLds& L, int N, int nocc, con
     }
     L.rank[j] = r;
   }
-  lds_matmul<true, false>(B, U, A, N, ld, ld);  // A = L^-T U (columns unsorted)
+  if (cbasis)
+    lds_matmul<false, false>(B, U, A, N, ld, ld);  // A = C_prev U (columns unsorted)
+  else
+    lds_matmul<true, false>(B, U, A, N, ld, ld);  // A = L^-T U (columns unsorted)
   __syncthreads();
+  if (Wout && store_c)  // eigenvectors in the AO basis
+    for (int x = tid; x < N * N; x += blockDim.x) {
+      const int i = x / N;
+      Wout[x] = A[i * ld + x - i * N];
+    }
   // occupied columns into B, then D = 2 B A^T into B via U
   for (int x = tid; x < N * N; x += blockDim.x) {
     const int i = x / N, j = x - i * N;
@@ -1432,8 +1444,9 @@ constexpr double OA_TAU = 0.2;       // |first-order correction| above which a p
 // E): returns the number of updates, with X in L.U and the eigenvalues in
 // L.eig.  Returns -1 (X then undefined) after OA_MAX updates or on a
 // non-finite update.  s_ready: L.A already holds S = X^T A X.  L.A, L.B and
-// L.rp are scratch.  N <= 64, 1024 threads.
-__device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready) {
+// L.rp are scratch.  x_id: X is the identity to start, so the first X^T X is
+// the identity without a product.  N <= 64, 1024 threads.
+__device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready, bool x_id = false) {
   const int ld = lds_ld(N), tid = threadIdx.x;
   double *M = L.M, *X = L.U, *S = L.A, *G = L.B;
   int* nbig = L.rp;  // near-degenerate pairs per row
@@ -1445,7 +1458,10 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready) {
       lds_matmul_sym<true, false>(X, G, S, N, ld, ld);  // S = X^T A X
       __syncthreads();
     }
-    lds_matmul_sym<true, false>(X, X, G, N, ld, ld);  // G = X^T X
+    if (k == 0 && x_id)
+      for (int x = tid; x < N * ld; x += blockDim.x) G[x] = (x / ld == x % ld) ? 1.0 : 0.0;
+    else
+      lds_matmul_sym<true, false>(X, X, G, N, ld, ld);  // G = X^T X
     __syncthreads();
     if (tid < N) L.eig[tid] = S[tid * ld + tid] / G[tid * ld + tid];
     __syncthreads();
@@ -1564,9 +1580,15 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
   double* Vh = state;
   double* Hh = state + (size_t)DIIS_N * NN;
   double* Bst = Hh + (size_t)DIIS_N * NN;
-  double* Uprev = Bst + 100;
-  double* flags = Uprev + NN;
+  // the previous step's eigenvectors: C = L^-T U in the AO basis where the
+  // eigen-decomposition works in their basis (N <= 64 with 1024 threads), else
+  // U in the orthonormal basis
+  double* Wprev = Bst + 100;
+  double* flags = Wprev + NN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const bool warm = flags[0] != 0.0;
+  const bool small = L.M != nullptr && N <= 64 && blockDim.x == 1024;
+  const bool cb = small && warm;  // eigenbasis of the previous step
 
   // H = H_core + dJK + V_xc
   for (int x = tid; x < NN; x += blockDim.x) {
@@ -1689,25 +1711,25 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
       A[i * ld + x - i * N] = h;
     }
   }
-  lds_load(B, Linv, N, ld);
+  lds_load(B, cb ? Wprev : Linv, N, ld);
   __syncthreads();
   for (int x = tid; x < NN; x += blockDim.x) {
     const int i = x / N;
     Hout[x] = A[i * ld + x - i * N];
   }
-  orthonormal_basis(L, N);
-  // Eigen-decomposition of A = L^-1 H L^-T (N <= 64 with 1024 threads):
-  //  * first step (no previous eigenvectors): the tridiagonal solver;
-  //  * later steps: A' = Uprev^T A Uprev with the previous eigenvectors.  Where
-  //    its off-diagonal share of |A'|^2 is at most OA_ENTRY (the SCF's later
-  //    steps), the refinement from X = Uprev (a few MFMA products per
-  //    iteration); otherwise the tridiagonal solver on A' and U = Uprev Z.
-  // Cyclic Jacobi (from the identity or from Uprev) takes over when either
-  // fails, and serves N > 64.
-  const bool warm = flags[0] != 0.0;
-  const bool small = L.M != nullptr && N <= 64 && blockDim.x == 1024;
-  auto start = [&]() {  // A = Uprev^T A Uprev, U = Uprev
-    lds_load(B, Uprev, N, ld);
+  if (!cb) orthonormal_basis(L, N);
+  // Eigen-decomposition (N <= 64 with 1024 threads):
+  //  * first step (no previous eigenvectors): the tridiagonal solver on A =
+  //    L^-1 H L^-T;
+  //  * later steps: in the previous eigenbasis, A' = C_prev^T H C_prev (= U_prev^T
+  //    A U_prev: two products from H instead of four).  Where its off-diagonal
+  //    share of |A'|^2 is at most OA_ENTRY (the SCF's later steps), the
+  //    refinement from X = I (a few MFMA products per iteration); otherwise
+  //    the tridiagonal solver on A'.  C = C_prev X.
+  // Cyclic Jacobi (from the identity, or from U_prev for N > 64) takes over
+  // when either fails, and serves N > 64.
+  auto start = [&]() {  // N > 64: A = Uprev^T A Uprev, U = Uprev
+    lds_load(B, Wprev, N, ld);
     __syncthreads();
     lds_matmul<false, false>(A, B, U, N, ld, ld);  // U = A Uprev
     __syncthreads();
@@ -1715,6 +1737,17 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
     __syncthreads();
     for (int x = tid; x < N * ld; x += blockDim.x) U[x] = B[x];
     __syncthreads();
+  };
+  auto identity_u = [&]() {
+    for (int x = tid; x < N * ld; x += blockDim.x) U[x] = (x / ld == x % ld) ? 1.0 : 0.0;
+    __syncthreads();
+  };
+  auto start_c = [&]() {  // H in A, C_prev in B: A = C_prev^T H C_prev, U = I
+    lds_matmul<false, false>(A, B, U, N, ld, ld);  // U = H C_prev
+    __syncthreads();
+    lds_matmul_sym<true, false>(B, U, A, N, ld, ld);  // A = C_prev^T U
+    __syncthreads();
+    identity_u();
   };
   auto rebuild = [&]() {  // A from H after the tridiagonal solver overwrote it
     lds_load(A, Hout, N, ld);
@@ -1731,14 +1764,11 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
       else
         rebuild();
     }
-    if (path == 0) {
-      for (int x = tid; x < N * ld; x += blockDim.x) U[x] = (x / ld == x % ld) ? 1.0 : 0.0;
-      __syncthreads();
-    }
+    if (path == 0) identity_u();
   } else if (small) {
-    for (int x = tid; x < N * ld; x += blockDim.x) L.M[x] = A[x];  // A in the orthonormal basis, kept
+    start_c();
+    for (int x = tid; x < N * ld; x += blockDim.x) L.M[x] = A[x];  // A', kept
     __syncthreads();
-    start();
     double off = 0.0, tot = 0.0, z0 = 0.0, z1 = 0.0, z2 = 0.0;
     for (int x = tid; x < NN; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
@@ -1748,19 +1778,21 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
     }
     block_reduce5(off, tot, z0, z1, z2, L.red);
     if (off <= OA_ENTRY * tot) {
-      refined = oa_refine(L, N, true);  // S of the first iteration is A' itself
+      refined = oa_refine(L, N, true, true);  // X = I: S of the first iteration is A' itself
       if (refined >= 0) {
         path = 2;
-      } else {
+      } else {  // Jacobi on A' from the identity
         for (int x = tid; x < N * ld; x += blockDim.x) A[x] = L.M[x];
         __syncthreads();
-        start();
+        identity_u();
       }
-    } else if (tri_eigh(TriBufs{L.A, L.U, L.B, L.eig, L.rank, L.tri, ld, N * ld, L.M, Uprev}, N)) {
-      path = 1;  // U = Uprev (eigenvectors of A')
-    } else {
-      rebuild();
-      start();
+    } else if (tri_eigh(TriBufs{L.A, L.U, L.B, L.eig, L.rank, L.tri, ld, N * ld, L.M, nullptr}, N)) {
+      path = 1;  // U = eigenvectors of A'
+    } else {  // A' again (the solver overwrote A, B and M)
+      lds_load(A, Hout, N, ld);
+      lds_load(B, Wprev, N, ld);
+      __syncthreads();
+      start_c();
     }
   } else {
     start();
@@ -1771,7 +1803,7 @@ __device__ __forceinline__ void scf_step_body(const ScfLds& L, int it, int N, in
     for (int j = tid; j < N; j += blockDim.x) L.eig[j] = A[j * ld + j];
   }
   __syncthreads();
-  eig_density(L, N, nocc, Linv, Uprev, true);
+  eig_density(L, N, nocc, cb ? Wprev : Linv, Wprev, true, small, cb);
   if (tid == 0) {
     flags[0] = 1.0;
     // diagnostic: -1 tridiagonal solver, 100 (k + 1) refinement of k updates,
