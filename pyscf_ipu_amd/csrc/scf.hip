@@ -1533,11 +1533,15 @@ __device__ __forceinline__ int oa_refine(const ScfLds& L, int N, bool s_ready, b
     // an update of at most 1e-9 leaves the off-diagonal share and X^T X - I at
     // the rounding level (both are second order in E): no further check
     const bool last = flags == 0.0 && emax <= 1e-9;
-    lds_matmul<false, false>(X, S, G, N, ld, ld);  // G = X E
-    __syncthreads();
+    const double* XE = S;  // X = I (first update of x_id): X E = E exactly
+    if (k > 0 || !x_id) {
+      lds_matmul<false, false>(X, S, G, N, ld, ld);  // G = X E
+      __syncthreads();
+      XE = G;
+    }
     for (int x = tid; x < N * N; x += blockDim.x) {
       const int i = x / N, j = x - i * N;
-      X[i * ld + j] += G[i * ld + j];
+      X[i * ld + j] += XE[i * ld + j];
     }
     __syncthreads();
     if (last) return k + 1;
