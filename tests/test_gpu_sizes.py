@@ -84,8 +84,10 @@ def test_large_631g_at_study_size(hip, gemm, eri, tol):
         mol = build_mol(m["mol_str"], gold["basis"])
         print(f"large {gemm} eri {eri} nao {mol.nao_nr()}", flush=True)
         grid = grids.atomic_grids(mol, gold["grid_level"])
+        # FP32 GEMMs: stop at |dE| < 1e-6, above their energy noise (the stop
+        # at 1e-7 sat at that noise and missed the 60 iterations now and then)
         r = LargeRKS(mol, gold["grid_level"], gemm, eri, grid=grid).run(its=60, conv_tol=1e-9 if tol < 1e-6
-                                                                          else 1e-7)
+                                                                          else 1e-6)
         assert r.converged
         assert abs(r.energy - m["energy"]) < tol, (r.energy, m["energy"])
         assert abs(r.hlgaps[-1] - m["hlgap"]) < (1e-5 if tol < 1e-6 else 1e-4)
